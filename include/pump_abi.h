@@ -1,0 +1,256 @@
+/*
+ * pump_abi.h -- C ABI of the MI355X (gfx950) implementation of PUMP's
+ * roadmap-edge and collision-probability path.
+ *
+ * Every entry point replaces one reference kernel launch or host function
+ * (cited file:line in the reference StanfordASL/PUMP) and keeps its argument
+ * meaning and array layouts:
+ *   samples[i*6 + d]                       (roadMap.cuh:16)
+ *   obstacles[o*12 + {lo 0..5, hi 6..11}]  (mainPUMP.cu:214-221)
+ *   discMotions[(wp)*6 + d], wp = e*W + k  (collisionCheck.cu:67-70)
+ *   as[(wp*maxHS + h)*3 + d], bs[wp*maxHS + h], countStoredHSs[wp]
+ *                                          (hsmc.cu:49,86-89; PUMP.cu:642-648)
+ *   offsetsTime[(t*nMC + p)*3 + d]         (hardCoded.cuh:18-19)
+ *   xcomb[(p*Tmax + t)*3 + d]              (hardCoded.cuh:15-16)
+ *   validParticles[path*nP + p] (1 byte)   (PUMP.cu:630,654)
+ *
+ * Conventions (replacing the reference's print-and-continue / exit(1)):
+ *   - device pointers are caller-allocated, as in the reference;
+ *   - every function returns 0 (PUMP_OK) or a negative pump_status;
+ *   - work is enqueued on `stream` (a hipStream_t; NULL = default stream);
+ *     functions that return a host scalar synchronise that stream;
+ *   - all sizes are runtime arguments; pair/edge/waypoint counts are 64-bit;
+ *   - a variable number of waypoints per edge is given by wp_offsets[E+1]
+ *     (int64, exclusive prefix of W_e); pass NULL to use the reference's
+ *     fixed count W = waypoints_count for every edge.
+ * Nothing here prints or exits.  No global state: the reference's globals
+ * offsetMult / maxd2 (mainPUMP.cu:62-63) are arguments.
+ */
+#ifndef PUMP_ABI_H
+#define PUMP_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *pump_stream_t; /* hipStream_t */
+
+enum pump_status {
+    PUMP_OK = 0,
+    PUMP_E_INVALID = -1,  /* null pointer / bad size */
+    PUMP_E_HIP = -2,      /* HIP runtime error (launch or copy) */
+    PUMP_E_CAPACITY = -3, /* a caller buffer is too small */
+    PUMP_E_NOMEM = -4,    /* device allocation failed */
+    PUMP_E_RANGE = -5     /* an argument is outside the supported range */
+};
+
+const char *pump_status_string(int status);
+int pump_abi_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Roadmap precompute ("roadMap/precomp")
+ * ------------------------------------------------------------------------ */
+
+/* fillCoptsTopts -- 2pBVP.cu:133-150, launched at mainPUMP.cu:293-299.
+ * For every ordered pair i != j: topts[idx] = toptBisection(x_i, x_j, tmax),
+ * copts[idx] = cost(topts[idx], x_i, x_j), idx = i*(num-1) + j - [i<j].
+ * Arithmetic is the reference's host semantics (cost evaluated in double and
+ * rounded to float; float bisection). */
+int pump_fill_copts_topts(const float *d_samples, int num, float tmax, float *d_copts,
+                          float *d_topts, pump_stream_t stream);
+
+/* Chosen form (no reference counterpart): FP64 Newton polish of each bisected
+ * arrival time on the stationarity quartic of cost; d_topts64[idx]. */
+int pump_polish_topts(const float *d_samples, int num, float tmax, const float *d_topts,
+                      double *d_topts64, pump_stream_t stream);
+
+/* rn = sort(copts)[rn_idx] -- mainPUMP.cu:303-304 (thrust::sort + index).
+ * Exact k-th order statistic by device radix select; writes *rn_host and
+ * synchronises `stream`.  Workspace of pump_select_rn_workspace() bytes. */
+size_t pump_select_rn_workspace(int64_t n_pairs);
+int pump_select_rn(const float *d_copts, int64_t n_pairs, int64_t rn_idx, float *rn_host,
+                   void *d_workspace, size_t workspace_bytes, pump_stream_t stream);
+
+/* rnIdx -- mainPUMP.cu:262: (int)(pairs * rnPerc) evaluated in float. */
+int64_t pump_rn_index(int64_t num, float rn_perc);
+
+/* Edge numbering -- mainPUMP.cu:327-346: edges are the pairs with copts < rn
+ * (strict), numbered row-major (source i ascending, then destination j).
+ * pump_count_edges writes d_row_start[num+1] (out-edges of node i are
+ * [row_start[i], row_start[i+1])) and *n_edges_host (synchronises).
+ * Workspace: pump_scan_workspace(num+1) bytes. */
+size_t pump_scan_workspace(int64_t n);
+int pump_count_edges(const float *d_copts, int num, float rn, int64_t *d_row_start,
+                     int64_t *n_edges_host, void *d_workspace, size_t workspace_bytes,
+                     pump_stream_t stream);
+/* coptsEdge/toptsEdge (mainPUMP.cu:334-335) plus the edge endpoints; the
+ * reference's nnIdxs[j*NUM+i] = e is the inverse of (edge_src, edge_dst). */
+int pump_fill_edges(const float *d_copts, const float *d_topts, int num, float rn,
+                    const int64_t *d_row_start, int *d_edge_src, int *d_edge_dst,
+                    float *d_copts_edge, float *d_topts_edge, pump_stream_t stream);
+/* maxNNSize (mainPUMP.cu:362-368) and the padded tables nnGoEdges,
+ * nnComeEdges, distancesCome [num x max_nn] (mainPUMP.cu:371-392).
+ * pump_max_nn_size synchronises.  d_nn_idxs (dense [num*num], default -3,
+ * mainPUMP.cu:315,338) is optional (NULL to skip). */
+int pump_max_nn_size(const int64_t *d_row_start, const int *d_edge_dst, int64_t n_edges, int num,
+                     int *d_in_degree, int *max_nn_host, pump_stream_t stream);
+int pump_nn_tables(const int64_t *d_row_start, const int *d_edge_src, const int *d_edge_dst,
+                   const float *d_copts_edge, int64_t n_edges, int num, int max_nn,
+                   int *d_nn_go_edges, int *d_nn_come_edges, float *d_distances_come,
+                   int *d_nn_idxs, pump_stream_t stream);
+/* h[i] = cost(toptBisection(x_i, goal, tmax), x_i, goal) for i < num-1,
+ * h[num-1] = 0 -- mainPUMP.cu:351-355.  goal_host: 6 floats (host). */
+int pump_heuristic(const float *d_samples, int num, const float *goal_host, float tmax, float *d_h,
+                   pump_stream_t stream);
+
+/* Waypoints per edge: W_e = w_fixed if w_fixed > 0 (the reference's
+ * numDisc+1, mainPUMP.cu:57), else clamp(ceil(tau_e/wp_dt), 2, w_cap).
+ * Writes d_wp_offsets[n_edges+1] and *total_host (synchronises).
+ * Workspace: pump_scan_workspace(n_edges+1). */
+int pump_waypoint_offsets(const float *d_topts_edge, int64_t n_edges, int w_fixed, float wp_dt,
+                          int w_cap, int64_t *d_wp_offsets, int64_t *total_host,
+                          void *d_workspace, size_t workspace_bytes, pump_stream_t stream);
+
+/* Edge discretization -- the host loop mainPUMP.cu:327-346 calling
+ * findDiscretizedPath (2pBVP.cu:53-69) per edge, moved to the device.
+ * The arrival time is taken from d_topts_edge: it is the same toptBisection
+ * of the same states that the reference re-solves on the host. */
+int pump_discretize_edges(const float *d_samples, const int *d_edge_src, const int *d_edge_dst,
+                          const float *d_topts_edge, int64_t n_edges, const int64_t *d_wp_offsets,
+                          int waypoints_count, float *d_disc_motions, pump_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Collision checking ("collisionCheck")
+ * ------------------------------------------------------------------------ */
+
+/* sampleFree -- sampler.cu:61-88, launched at mainPUMP.cu:430-436. */
+int pump_sample_free(const float *d_obstacles, int n_obs, const float *d_samples, int num,
+                     uint8_t *d_is_free_samples, pump_stream_t stream);
+
+/* freeEdges -- collisionCheck.cu:5-37, launched at mainPUMP.cu:444-452.
+ * numDisc = waypoints_count-1 segments per edge (or wp_offsets). */
+int pump_free_edges(const float *d_obstacles, int n_obs, const float *d_samples,
+                    const uint8_t *d_is_free_samples, int waypoints_count,
+                    const int64_t *d_wp_offsets, const float *d_disc_motions,
+                    uint8_t *d_is_free_edges, int64_t n_edges, pump_stream_t stream);
+
+/* Fused edge pass (kernel b): waypoint generation (findDiscretizedPath) and
+ * the segment/AABB test (freeEdges) in one launch; writes discMotions and
+ * isFreeEdges, identical to pump_discretize_edges + pump_free_edges. */
+int pump_edges_discretize_check(const float *d_samples, const int *d_edge_src,
+                                const int *d_edge_dst, const float *d_topts_edge,
+                                int64_t n_edges, const int64_t *d_wp_offsets, int waypoints_count,
+                                const float *d_obstacles, int n_obs, float *d_disc_motions,
+                                uint8_t *d_is_free_edges, pump_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Half-spaces and HSMC ("hsmc")
+ * ------------------------------------------------------------------------ */
+
+/* setupAsBs -- hsmc.cu:201-211: as = -1, bs = 1111 (b >= 1000: empty). */
+int pump_setup_as_bs(int64_t hs_count, float *d_bs, float *d_as, pump_stream_t stream);
+
+/* maxd2 = 0.1*offsetMult^2 (double, narrowed to float) -- mainPUMP.cu:63,201 */
+float pump_maxd2(float offset_mult);
+
+/* cacheHalfspaces -- hsmc.cu:10-99 (+ bvlsShortcut, bvls.cu:259-273),
+ * launched at mainPUMP.cu:484-498.  Any obstacle count is supported (the
+ * reference stages only 1024 floats of obstacles, hsmc.cu:20-22).
+ * countStoredHSs of waypoints on blocked edges is written as 0 (the
+ * reference leaves it unwritten).  write_sentinels != 0 additionally writes
+ * the empty sentinel into every unused slot, which makes a preceding
+ * pump_setup_as_bs unnecessary (same final contents). */
+int pump_cache_halfspaces(int64_t n_edges, const float *d_disc_motions,
+                          const uint8_t *d_is_free_edges, int n_obs, int max_hs,
+                          const float *d_obstacles, int waypoints_count,
+                          const int64_t *d_wp_offsets, float *d_as, float *d_bs,
+                          int *d_count_stored, float maxd2, int write_sentinels,
+                          pump_stream_t stream);
+
+/* setupValidParticles -- PUMP.cu:766-777, launched at mainPUMP.cu:663-668. */
+int pump_setup_valid_particles(uint8_t *d_valid, int n_p, int64_t max_path_count,
+                               float *d_path_cp, float *d_path_time, pump_stream_t stream);
+
+/* propagateValidParticles (+ ...Many) -- PUMP.cu:613-705, launched at
+ * PUMP.cu:94-121.  Child path path_base+w inherits validity from
+ * wf_prev[w] and ANDs (offset.a)*offsetMult < b over every stored
+ * half-space of every waypoint of edge wf_edge[w].  tmax_steps is the row
+ * count of offsets_time; time indices past it are clamped (the reference
+ * reads past its table) and counted into *d_clamped if non-NULL. */
+int pump_propagate_valid_particles(int num_new_paths, int waypoints_count,
+                                   const int64_t *d_wp_offsets, const int *d_wf_edge,
+                                   const int *d_wf_prev, int64_t path_base, const float *d_as,
+                                   const float *d_bs, const int *d_count_stored,
+                                   const float *d_times, const float *d_topts, float dt,
+                                   int max_hs, const float *d_offsets_time, int tmax_steps,
+                                   float offset_mult, int num_mc_particles, uint8_t *d_valid,
+                                   int n_p, unsigned long long *d_clamped, pump_stream_t stream);
+
+/* calculateCP_HSMC -- PUMP.cu:745-758, launched at PUMP.cu:136-141. */
+int pump_calculate_cp_hsmc(int num_new_paths, int64_t path_base, float *d_path_cp,
+                           const uint8_t *d_valid, int n_p, pump_stream_t stream);
+
+/* Propagate + CP in one launch with separate parent/child validity arrays
+ * (child rows dst_base+w of d_valid_dst).  Same results as the two calls
+ * above; used by the search consumer so that compaction needs no aliasing
+ * copies (PUMP.cu:236-237 issues one D2D memcpy per surviving path). */
+int pump_propagate_cp(int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets,
+                      const int *d_wf_edge, const int *d_wf_prev, const float *d_as,
+                      const float *d_bs, const float *d_times, const float *d_topts, float dt,
+                      int max_hs, const float *d_offsets_time, int tmax_steps, float offset_mult,
+                      int num_mc_particles, const uint8_t *d_valid_src, uint8_t *d_valid_dst,
+                      int64_t dst_base, int n_p, float *d_path_cp, int64_t cp_base,
+                      unsigned long long *d_clamped, pump_stream_t stream);
+
+/* Compaction of surviving paths -- the per-path D2D copies of
+ * PUMP.cu:236-237 as one gather: dst row dst_base+k = src row src_rows[k]. */
+int pump_gather_rows(const uint8_t *d_src, const int *d_src_rows, int n_rows, uint8_t *d_dst,
+                     int64_t dst_base, int64_t row_bytes, pump_stream_t stream);
+
+/* Chosen form: validity as bit masks (bit p%64 of word path*ceil(nP/64) +
+ * p/64), child masks written by ballots; CP from popcounts.  Identical
+ * per-particle decisions to pump_propagate_cp.  precision: 0 = FP32 (the
+ * reference's operation order); 1 = split-BF16 hi/lo products on MFMA with
+ * FP32 accumulation and an FP32 re-test of every particle whose
+ * |offset.a*offsetMult - b| lies within the stated margin band. */
+int pump_propagate_cp_bits(int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets,
+                           const int *d_wf_edge, const int *d_wf_prev, const float *d_as,
+                           const float *d_bs, const float *d_times, const float *d_topts, float dt,
+                           int max_hs, const float *d_offsets_time, int tmax_steps,
+                           float offset_mult, int num_mc_particles,
+                           const uint64_t *d_valid_src, uint64_t *d_valid_dst, int64_t dst_base,
+                           int n_p, float *d_path_cp, int64_t cp_base, int precision,
+                           unsigned long long *d_band_count, unsigned long long *d_clamped,
+                           pump_stream_t stream);
+
+/* Initial bit masks for pump_propagate_cp_bits: all n_p particles valid in
+ * each of n_paths rows (the bit form of setupValidParticles). */
+int pump_setup_valid_bits(uint64_t *d_valid, int n_p, int64_t n_paths, pump_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Full Monte Carlo CP ("collisionProbability")
+ * ------------------------------------------------------------------------ */
+
+/* collisionProbability + MCCP -- collisionProbability.cu:4-92.  Writes
+ * cp = hits/n_mc to *cp_host (synchronises).  Allocates a 4-byte hit
+ * counter per call, as the reference allocates its per-call scratch. */
+int pump_collision_probability(const float *d_obstacles, int n_obs, const float *d_xcomb,
+                               int tmax_steps, int n_mc, float offset_mult, const float *d_path,
+                               int T, float *cp_host, pump_stream_t stream);
+
+/* Batched form: n_paths paths, path k at d_paths + k*path_stride_rows*6
+ * with T = d_T[k]; hit counts to d_hits[k] (caller-zeroed or zeroed here
+ * when zero_hits != 0). */
+int pump_collision_probability_many(const float *d_obstacles, int n_obs, const float *d_xcomb,
+                                    int tmax_steps, int n_mc, float offset_mult,
+                                    const float *d_paths, int64_t path_stride_rows,
+                                    const int *d_T, int n_paths, int *d_hits, int zero_hits,
+                                    pump_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PUMP_ABI_H */

@@ -1,0 +1,160 @@
+/*
+ * pump_host.h -- host-side C++ pipeline above the C ABI (pump_abi.h):
+ * problem and noise-table generation, the roadmap precompute of
+ * mainPUMP.cu:248-504,621-681 and the PUMP search consumer of
+ * PUMP.cu:3-405, driven entirely through the pump_* device entry points.
+ * Exposed as C for the Python tests and the benchmark.
+ */
+#ifndef PUMP_HOST_H
+#define PUMP_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pump_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- synthetic problems (counter-based Philox4x32-10 draws) ------------- */
+enum pump_obstacle_kind { PUMP_OBS_BOXES = 0, PUMP_OBS_WALLS = 1 };
+
+typedef struct pump_problem_spec {
+    int num;             /* samples; sample 0 = start, sample num-1 = goal */
+    uint64_t seed;
+    float lo[6], hi[6];  /* sampling bounds per state dimension */
+    float start[6], goal[6];
+    int n_obs;
+    int obstacle_kind;   /* pump_obstacle_kind */
+    float centre_lo, centre_hi; /* box centres uniform in [centre_lo, centre_hi]^3 */
+    float side_lo, side_hi;     /* box sides uniform (boxes) */
+    float wall_thickness, wall_long_lo, wall_long_hi, wall_short_lo, wall_short_hi;
+    float vel_span;      /* velocity bounds of every box: [-vel_span, vel_span] */
+} pump_problem_spec;
+
+/* Philox4x32-10 (Salmon et al., SC'11): 4 output words for counter ctr[4]
+ * and key key[2]. */
+void pump_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+int pump_gen_problem(const pump_problem_spec *spec, float *samples_host, float *obstacles_host);
+
+/* ---- noise tables ------------------------------------------------------
+ * Closed-loop double-integrator offsets from the nominal path: per axis
+ * x_{t+1} = A_cl x_t + L z_t with A_cl the dt-discretised double integrator
+ * under u = -kp p - kd v, Q = q [[dt^3/3, dt^2/2], [dt^2/2, dt]] per axis,
+ * L = chol(Q) of the 6x6 Q in FP64, z_t ~ N(0, I) from Philox + Box-Muller,
+ * x_0 = 0.  Positions are stored (unit scale; offsetMult scales them at
+ * use) in both layouts: offsets_time[(t*n_mc + p)*3 + d] and
+ * xcomb[(p*tmax_steps + t)*3 + d].  sigma_pos (optional,
+ * [tmax_steps*3*3]) receives the FP32-propagated position covariance. */
+typedef struct pump_noise_spec {
+    int n_mc, tmax_steps;
+    float dt;
+    double kp, kd, q;
+    uint64_t seed;
+} pump_noise_spec;
+
+void pump_noise_defaults(pump_noise_spec *spec);
+int pump_make_noise(const pump_noise_spec *spec, float *offsets_time_host, float *xcomb_host,
+                    float *sigma_pos_host);
+
+/* ---- roadmap precompute + search --------------------------------------- */
+typedef struct pump_params {
+    float tmax;          /* 2pBVP upper bracket (mainPUMP.cu:266) */
+    float rn_perc;       /* neighbour fraction (mainPUMP.cu:261) */
+    int w_fixed;         /* waypoints per edge (numDisc+1), or <= 0 for ceil(tau/wp_dt) */
+    float wp_dt;
+    int w_cap;
+    float offset_mult;   /* noise multiplier (mainPUMP.cu:62) */
+    int max_hs;          /* min(nObs, 5) in the reference (mainPUMP.cu:241) */
+    float dt;            /* noise table time step (mainPUMP.cu:60) */
+    int n_p;             /* HSMC particles per path */
+    float lambda;        /* mainPUMP.cu:66 */
+    int num_buckets;     /* mainPUMP.cu:67 */
+    float eps_cost, eps_cp;
+    float cp_target, cp_factor;
+    int max_itrs;        /* PUMP.cu:34 */
+    int64_t max_path_count; /* <= 0: 2*NUM*sqrt(NUM) (mainPUMP.cu:623) */
+    int validity_bits;   /* 0: byte validity (reference layout), 1: bit masks */
+    int precision;       /* 0: FP32 half-space products, 1: split-BF16 MFMA */
+    int fused_edges;     /* 1: pump_edges_discretize_check, 0: discretize + freeEdges */
+    int keep_pairs;      /* keep copts/topts on the device after the build */
+    int polish;          /* also compute FP64-polished arrival times (chosen form) */
+    int path_cap;        /* rows of a candidate path for full MC */
+    int mccp_batch;      /* goal candidates evaluated per full-MC launch */
+} pump_params;
+
+void pump_params_defaults(pump_params *p);
+
+typedef struct pump_roadmap pump_roadmap;
+
+typedef struct pump_roadmap_info {
+    int num, n_obs, max_nn, max_hs, n_mc, tmax_steps;
+    int64_t n_pairs, rn_idx, n_edges, n_waypoints, n_free_edges;
+    float rn, maxd2;
+    /* stage times (ms, hipEvent) of the last build */
+    float ms_pairs, ms_select, ms_edges_build, ms_nn, ms_sample_free, ms_edge_pass, ms_hs,
+        ms_total;
+} pump_roadmap_info;
+
+int pump_roadmap_build(const float *samples_host, int num, const float *obstacles_host, int n_obs,
+                       const float *goal_host, const float *offsets_time_host,
+                       const float *xcomb_host, int n_mc, int tmax_steps, const pump_params *prm,
+                       pump_roadmap **out);
+int pump_roadmap_info_get(const pump_roadmap *rm, pump_roadmap_info *info);
+void pump_roadmap_free(pump_roadmap *rm);
+
+/* host copies of the roadmap arrays (NULL pointers are skipped) */
+int pump_roadmap_copy_pairs(const pump_roadmap *rm, float *copts, float *topts);
+int pump_roadmap_copy_polished(const pump_roadmap *rm, double *topts64);
+int pump_roadmap_copy_edges(const pump_roadmap *rm, int64_t *row_start, int *edge_src,
+                            int *edge_dst, float *copts_edge, float *topts_edge, float *h);
+int pump_roadmap_copy_nn(const pump_roadmap *rm, int *nn_go_edges, int *nn_come_edges,
+                         float *distances_come, int *nn_idxs);
+int pump_roadmap_copy_flags(const pump_roadmap *rm, int64_t *wp_offsets, uint8_t *is_free_edges,
+                            uint8_t *is_free_samples);
+int pump_roadmap_copy_waypoints(const pump_roadmap *rm, float *disc_motions);
+int pump_roadmap_copy_halfspaces(const pump_roadmap *rm, float *as, float *bs, int *count_stored);
+
+/* device pointers, for callers that drive the entry points themselves */
+typedef struct pump_roadmap_dev {
+    float *samples, *obstacles, *copts, *topts, *copts_edge, *topts_edge, *h, *disc, *as, *bs;
+    float *offsets_time, *xcomb;
+    int64_t *row_start, *wp_offsets;
+    int *edge_src, *edge_dst, *count_stored;
+    uint8_t *is_free_edges, *is_free_samples;
+} pump_roadmap_dev;
+int pump_roadmap_device(const pump_roadmap *rm, pump_roadmap_dev *dev);
+
+typedef struct pump_search_result {
+    int status;           /* 0 success, 1 no goal path below target, <0 aborted */
+    int itrs;
+    int64_t num_paths, num_considered;
+    int num_goal_paths;
+    int soln_path_idx;
+    float soln_cost, soln_cp, soln_time, soln_hsmc_cp;
+    int soln_len;
+    int soln_nodes[256];
+    int64_t clamped_time_idx;
+    int64_t particle_steps;
+    int64_t band_particles; /* split-BF16 margin-band re-tests */
+    int mccp_calls;
+    float ms_propagate;    /* device time in the HSMC step (hipEvent) */
+    float ms_total;        /* wall time of the search */
+} pump_search_result;
+
+int pump_search_run(pump_roadmap *rm, const pump_params *prm, pump_search_result *res);
+
+/* findOptimalPath (2pBVP.cu:5-36) on the host with the library's arithmetic;
+ * returns rows written or -1 if more than cap rows would be needed. */
+int pump_find_optimal_path(float dt, const float *xs, int n_wp, float *split, int cap,
+                           float *total_time);
+/* single-pair helpers (host) for tests */
+float pump_host_topt(const float *x0, const float *x1, float tmax);
+float pump_host_cost(float tau, const float *x0, const float *x1);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PUMP_HOST_H */

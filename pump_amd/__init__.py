@@ -1,0 +1,365 @@
+"""pump_amd -- MI355X (gfx950) implementation of PUMP's roadmap-edge and
+collision-probability path behind a C ABI (include/pump_abi.h) and a C++
+host pipeline (include/pump_host.h).
+
+This module loads ``libpump_amd.so`` (built in-tree by ``pump_amd.build``)
+with ctypes.  There is no fallback: importing the package without the built
+library raises, so nothing can silently run a non-native path.
+
+Python mirrors of the reference entry points (fillCoptsTopts, freeEdges,
+cacheHalfspaces, propagateValidParticles, calculateCP_HSMC,
+collisionProbability, ...) are in ``pump_amd.ops`` (device tensors) and the
+roadmap/search pipeline is ``pump_amd.Roadmap``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpump_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is not built; run `python -m pump_amd.build` (hipcc, gfx950)")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+c_int, c_float, c_double, c_int64 = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_int64
+c_uint64, c_uint32, c_size_t, c_void_p = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_size_t, ctypes.c_void_p
+f32p = ctypes.POINTER(c_float)
+f64p = ctypes.POINTER(c_double)
+i32p = ctypes.POINTER(c_int)
+i64p = ctypes.POINTER(c_int64)
+u8p = ctypes.POINTER(ctypes.c_uint8)
+u32p = ctypes.POINTER(c_uint32)
+u64p = ctypes.POINTER(c_uint64)
+
+PUMP_OK = 0
+STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "buffer too small",
+          -4: "device allocation failed", -5: "argument outside the supported range"}
+
+
+class PumpError(RuntimeError):
+    def __init__(self, fn, code):
+        super().__init__(f"{fn} failed: {code} ({STATUS.get(code, 'unknown')})")
+        self.code = code
+
+
+class ProblemSpec(ctypes.Structure):
+    _fields_ = [
+        ("num", c_int), ("seed", c_uint64), ("lo", c_float * 6), ("hi", c_float * 6),
+        ("start", c_float * 6), ("goal", c_float * 6), ("n_obs", c_int), ("obstacle_kind", c_int),
+        ("centre_lo", c_float), ("centre_hi", c_float), ("side_lo", c_float), ("side_hi", c_float),
+        ("wall_thickness", c_float), ("wall_long_lo", c_float), ("wall_long_hi", c_float),
+        ("wall_short_lo", c_float), ("wall_short_hi", c_float), ("vel_span", c_float),
+    ]
+
+
+class NoiseSpec(ctypes.Structure):
+    _fields_ = [("n_mc", c_int), ("tmax_steps", c_int), ("dt", c_float), ("kp", c_double),
+                ("kd", c_double), ("q", c_double), ("seed", c_uint64)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("tmax", c_float), ("rn_perc", c_float), ("w_fixed", c_int), ("wp_dt", c_float),
+        ("w_cap", c_int), ("offset_mult", c_float), ("max_hs", c_int), ("dt", c_float),
+        ("n_p", c_int), ("lambda_", c_float), ("num_buckets", c_int), ("eps_cost", c_float),
+        ("eps_cp", c_float), ("cp_target", c_float), ("cp_factor", c_float), ("max_itrs", c_int),
+        ("max_path_count", c_int64), ("validity_bits", c_int), ("precision", c_int),
+        ("fused_edges", c_int), ("keep_pairs", c_int), ("polish", c_int), ("path_cap", c_int),
+        ("mccp_batch", c_int),
+    ]
+
+
+class RoadmapInfo(ctypes.Structure):
+    _fields_ = [
+        ("num", c_int), ("n_obs", c_int), ("max_nn", c_int), ("max_hs", c_int), ("n_mc", c_int),
+        ("tmax_steps", c_int), ("n_pairs", c_int64), ("rn_idx", c_int64), ("n_edges", c_int64),
+        ("n_waypoints", c_int64), ("n_free_edges", c_int64), ("rn", c_float), ("maxd2", c_float),
+        ("ms_pairs", c_float), ("ms_select", c_float), ("ms_edges_build", c_float),
+        ("ms_nn", c_float), ("ms_sample_free", c_float), ("ms_edge_pass", c_float),
+        ("ms_hs", c_float), ("ms_total", c_float),
+    ]
+
+
+class RoadmapDev(ctypes.Structure):
+    _fields_ = [(n, c_void_p) for n in (
+        "samples", "obstacles", "copts", "topts", "copts_edge", "topts_edge", "h", "disc", "as_",
+        "bs", "offsets_time", "xcomb", "row_start", "wp_offsets", "edge_src", "edge_dst",
+        "count_stored", "is_free_edges", "is_free_samples")]
+
+
+class SearchResult(ctypes.Structure):
+    _fields_ = [
+        ("status", c_int), ("itrs", c_int), ("num_paths", c_int64), ("num_considered", c_int64),
+        ("num_goal_paths", c_int), ("soln_path_idx", c_int), ("soln_cost", c_float),
+        ("soln_cp", c_float), ("soln_time", c_float), ("soln_hsmc_cp", c_float),
+        ("soln_len", c_int), ("soln_nodes", c_int * 256), ("clamped_time_idx", c_int64),
+        ("particle_steps", c_int64), ("band_particles", c_int64), ("mccp_calls", c_int),
+        ("ms_propagate", c_float), ("ms_total", c_float),
+    ]
+
+
+S = c_void_p  # stream / device pointer
+_SIGS = {
+    # pump_abi.h
+    "pump_status_string": (ctypes.c_char_p, [c_int]),
+    "pump_abi_version": (c_int, []),
+    "pump_fill_copts_topts": (c_int, [S, c_int, c_float, S, S, S]),
+    "pump_polish_topts": (c_int, [S, c_int, c_float, S, S, S]),
+    "pump_select_rn_workspace": (c_size_t, [c_int64]),
+    "pump_select_rn": (c_int, [S, c_int64, c_int64, f32p, S, c_size_t, S]),
+    "pump_rn_index": (c_int64, [c_int64, c_float]),
+    "pump_scan_workspace": (c_size_t, [c_int64]),
+    "pump_count_edges": (c_int, [S, c_int, c_float, S, i64p, S, c_size_t, S]),
+    "pump_fill_edges": (c_int, [S, S, c_int, c_float, S, S, S, S, S, S]),
+    "pump_max_nn_size": (c_int, [S, S, c_int64, c_int, S, i32p, S]),
+    "pump_nn_tables": (c_int, [S, S, S, S, c_int64, c_int, c_int, S, S, S, S, S]),
+    "pump_heuristic": (c_int, [S, c_int, f32p, c_float, S, S]),
+    "pump_waypoint_offsets": (c_int, [S, c_int64, c_int, c_float, c_int, S, i64p, S, c_size_t, S]),
+    "pump_discretize_edges": (c_int, [S, S, S, S, c_int64, S, c_int, S, S]),
+    "pump_sample_free": (c_int, [S, c_int, S, c_int, S, S]),
+    "pump_free_edges": (c_int, [S, c_int, S, S, c_int, S, S, S, c_int64, S]),
+    "pump_edges_discretize_check": (c_int, [S, S, S, S, c_int64, S, c_int, S, c_int, S, S, S]),
+    "pump_setup_as_bs": (c_int, [c_int64, S, S, S]),
+    "pump_maxd2": (c_float, [c_float]),
+    "pump_cache_halfspaces": (c_int, [c_int64, S, S, c_int, c_int, S, c_int, S, S, S, S, c_float,
+                                      c_int, S]),
+    "pump_setup_valid_particles": (c_int, [S, c_int, c_int64, S, S, S]),
+    "pump_propagate_valid_particles": (c_int, [c_int, c_int, S, S, S, c_int64, S, S, S, S, S,
+                                               c_float, c_int, S, c_int, c_float, c_int, S, c_int,
+                                               S, S]),
+    "pump_calculate_cp_hsmc": (c_int, [c_int, c_int64, S, S, c_int, S]),
+    "pump_propagate_cp": (c_int, [c_int, c_int, S, S, S, S, S, S, S, c_float, c_int, S, c_int,
+                                  c_float, c_int, S, S, c_int64, c_int, S, c_int64, S, S]),
+    "pump_gather_rows": (c_int, [S, S, c_int, S, c_int64, c_int64, S]),
+    "pump_propagate_cp_bits": (c_int, [c_int, c_int, S, S, S, S, S, S, S, c_float, c_int, S,
+                                       c_int, c_float, c_int, S, S, c_int64, c_int, S, c_int64,
+                                       c_int, S, S, S]),
+    "pump_setup_valid_bits": (c_int, [S, c_int, c_int64, S]),
+    "pump_collision_probability": (c_int, [S, c_int, S, c_int, c_int, c_float, S, c_int, f32p, S]),
+    "pump_collision_probability_many": (c_int, [S, c_int, S, c_int, c_int, c_float, S, c_int64, S,
+                                                c_int, S, c_int, S]),
+    # pump_host.h
+    "pump_philox4x32": (None, [u32p, u32p, u32p]),
+    "pump_gen_problem": (c_int, [ctypes.POINTER(ProblemSpec), f32p, f32p]),
+    "pump_noise_defaults": (None, [ctypes.POINTER(NoiseSpec)]),
+    "pump_make_noise": (c_int, [ctypes.POINTER(NoiseSpec), f32p, f32p, f32p]),
+    "pump_params_defaults": (None, [ctypes.POINTER(Params)]),
+    "pump_roadmap_build": (c_int, [f32p, c_int, f32p, c_int, f32p, f32p, f32p, c_int, c_int,
+                                   ctypes.POINTER(Params), ctypes.POINTER(c_void_p)]),
+    "pump_roadmap_info_get": (c_int, [c_void_p, ctypes.POINTER(RoadmapInfo)]),
+    "pump_roadmap_free": (None, [c_void_p]),
+    "pump_roadmap_copy_pairs": (c_int, [c_void_p, f32p, f32p]),
+    "pump_roadmap_copy_polished": (c_int, [c_void_p, f64p]),
+    "pump_roadmap_copy_edges": (c_int, [c_void_p, i64p, i32p, i32p, f32p, f32p, f32p]),
+    "pump_roadmap_copy_nn": (c_int, [c_void_p, i32p, i32p, f32p, i32p]),
+    "pump_roadmap_copy_flags": (c_int, [c_void_p, i64p, u8p, u8p]),
+    "pump_roadmap_copy_waypoints": (c_int, [c_void_p, f32p]),
+    "pump_roadmap_copy_halfspaces": (c_int, [c_void_p, f32p, f32p, i32p]),
+    "pump_roadmap_device": (c_int, [c_void_p, ctypes.POINTER(RoadmapDev)]),
+    "pump_search_run": (c_int, [c_void_p, ctypes.POINTER(Params), ctypes.POINTER(SearchResult)]),
+    "pump_find_optimal_path": (c_int, [c_float, f32p, c_int, f32p, c_int, f32p]),
+    "pump_host_topt": (c_float, [f32p, f32p, c_float]),
+    "pump_host_cost": (c_float, [c_float, f32p, f32p]),
+}
+
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+ABI_SYMBOLS = [n for n in _SIGS if not n.startswith("pump_philox") and n in _SIGS]
+
+
+def check(fn, code):
+    if code != PUMP_OK:
+        raise PumpError(fn, code)
+
+
+def _p(a, kind):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be contiguous"
+    return a.ctypes.data_as(kind)
+
+
+def fp(a):
+    return _p(a, f32p)
+
+
+def dp(a):
+    return _p(a, f64p)
+
+
+def ip(a):
+    return _p(a, i32p)
+
+
+def lp(a):
+    return _p(a, i64p)
+
+
+def bp(a):
+    return _p(a, u8p)
+
+
+# ---------------------------------------------------------------------------
+# generators
+# ---------------------------------------------------------------------------
+def philox4x32(ctr, key):
+    c = (c_uint32 * 4)(*ctr)
+    k = (c_uint32 * 2)(*key)
+    o = (c_uint32 * 4)()
+    lib.pump_philox4x32(c, k, o)
+    return list(o)
+
+
+def gen_problem(spec):
+    """(samples [num,6] f32, obstacles [n_obs,12] f32) for a ProblemSpec."""
+    s = np.zeros((spec.num, 6), np.float32)
+    o = np.zeros((max(spec.n_obs, 1), 12), np.float32)
+    check("pump_gen_problem", lib.pump_gen_problem(ctypes.byref(spec), fp(s), fp(o)))
+    return s, o[: spec.n_obs].copy()
+
+
+def noise_spec(**kw):
+    ns = NoiseSpec()
+    lib.pump_noise_defaults(ctypes.byref(ns))
+    for k, v in kw.items():
+        setattr(ns, k, v)
+    return ns
+
+
+def make_noise(ns, sigma=False):
+    """(offsets_time [T, n_mc, 3], xcomb [n_mc, T, 3][, sigma_pos [T,3,3]])."""
+    ot = np.zeros((ns.tmax_steps, ns.n_mc, 3), np.float32)
+    xc = np.zeros((ns.n_mc, ns.tmax_steps, 3), np.float32)
+    sg = np.zeros((ns.tmax_steps, 3, 3), np.float32) if sigma else None
+    check("pump_make_noise", lib.pump_make_noise(ctypes.byref(ns), fp(ot), fp(xc), fp(sg)))
+    return (ot, xc, sg) if sigma else (ot, xc)
+
+
+def params(**kw):
+    p = Params()
+    lib.pump_params_defaults(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, "lambda_" if k == "lambda" else k, v)
+    return p
+
+
+# ---------------------------------------------------------------------------
+# roadmap + search pipeline
+# ---------------------------------------------------------------------------
+class Roadmap:
+    """Roadmap precompute on the device (mainPUMP.cu:248-504) and the PUMP
+    search (PUMP.cu:3-405) over it."""
+
+    def __init__(self, samples, obstacles, goal, offsets_time, xcomb, prm):
+        self.samples = np.ascontiguousarray(samples, np.float32)
+        self.obstacles = np.ascontiguousarray(obstacles, np.float32).reshape(-1, 12)
+        self.goal = np.ascontiguousarray(goal, np.float32)
+        self.prm = prm
+        n_mc = offsets_time.shape[1] if offsets_time is not None else 0
+        tmax_steps = offsets_time.shape[0] if offsets_time is not None else 0
+        self._h = c_void_p()
+        obs = self.obstacles if self.obstacles.size else np.zeros((1, 12), np.float32)
+        code = lib.pump_roadmap_build(
+            fp(self.samples), self.samples.shape[0], fp(obs), self.obstacles.shape[0],
+            fp(self.goal), fp(offsets_time), fp(xcomb), n_mc, tmax_steps, ctypes.byref(prm),
+            ctypes.byref(self._h))
+        check("pump_roadmap_build", code)
+        self.info = RoadmapInfo()
+        check("pump_roadmap_info_get", lib.pump_roadmap_info_get(self._h, ctypes.byref(self.info)))
+
+    def close(self):
+        if self._h:
+            lib.pump_roadmap_free(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def pairs(self):
+        n = self.info.n_pairs
+        c, t = np.empty(n, np.float32), np.empty(n, np.float32)
+        check("pump_roadmap_copy_pairs", lib.pump_roadmap_copy_pairs(self._h, fp(c), fp(t)))
+        return c, t
+
+    def polished(self):
+        t = np.empty(self.info.n_pairs, np.float64)
+        check("pump_roadmap_copy_polished", lib.pump_roadmap_copy_polished(self._h, dp(t)))
+        return t
+
+    def edges(self):
+        num, E = self.info.num, self.info.n_edges
+        rs = np.empty(num + 1, np.int64)
+        src, dst = np.empty(E, np.int32), np.empty(E, np.int32)
+        ce, te, h = np.empty(E, np.float32), np.empty(E, np.float32), np.empty(num, np.float32)
+        check("pump_roadmap_copy_edges",
+              lib.pump_roadmap_copy_edges(self._h, lp(rs), ip(src), ip(dst), fp(ce), fp(te), fp(h)))
+        return dict(row_start=rs, edge_src=src, edge_dst=dst, copts_edge=ce, topts_edge=te, h=h)
+
+    def nn_tables(self, dense=False):
+        num, m = self.info.num, self.info.max_nn
+        go = np.empty((num, m), np.int32)
+        come = np.empty((num, m), np.int32)
+        dist = np.empty((num, m), np.float32)
+        idx = np.empty((num, num), np.int32) if dense else None
+        check("pump_roadmap_copy_nn",
+              lib.pump_roadmap_copy_nn(self._h, ip(go), ip(come), fp(dist), ip(idx)))
+        return dict(nn_go_edges=go, nn_come_edges=come, distances_come=dist, nn_idxs=idx)
+
+    def flags(self):
+        E, num = self.info.n_edges, self.info.num
+        wo = np.empty(E + 1, np.int64)
+        fe, fs = np.empty(E, np.uint8), np.empty(num, np.uint8)
+        check("pump_roadmap_copy_flags", lib.pump_roadmap_copy_flags(self._h, lp(wo), bp(fe), bp(fs)))
+        return dict(wp_offsets=wo, is_free_edges=fe, is_free_samples=fs)
+
+    def waypoints(self):
+        d = np.empty((self.info.n_waypoints, 6), np.float32)
+        check("pump_roadmap_copy_waypoints", lib.pump_roadmap_copy_waypoints(self._h, fp(d)))
+        return d
+
+    def halfspaces(self):
+        hs = self.info.n_waypoints * max(self.info.max_hs, 1)
+        a = np.empty((hs, 3), np.float32)
+        b = np.empty(hs, np.float32)
+        c = np.empty(self.info.n_waypoints, np.int32)
+        check("pump_roadmap_copy_halfspaces",
+              lib.pump_roadmap_copy_halfspaces(self._h, fp(a), fp(b), ip(c)))
+        return dict(as_=a, bs=b, count_stored=c)
+
+    def device(self):
+        d = RoadmapDev()
+        check("pump_roadmap_device", lib.pump_roadmap_device(self._h, ctypes.byref(d)))
+        return d
+
+    def search(self, prm=None):
+        r = SearchResult()
+        check("pump_search_run",
+              lib.pump_search_run(self._h, ctypes.byref(prm or self.prm), ctypes.byref(r)))
+        return r
+
+
+def search_result_dict(r):
+    d = {k: getattr(r, k) for k, _ in SearchResult._fields_ if k != "soln_nodes"}
+    d["soln_nodes"] = list(r.soln_nodes)[: r.soln_len]
+    return d
+
+
+def find_optimal_path(xs, dt=0.05, cap=4096):
+    xs = np.ascontiguousarray(xs, np.float32)
+    out = np.zeros((cap, 6), np.float32)
+    tt = c_float(0)
+    rows = lib.pump_find_optimal_path(c_float(dt), fp(xs), xs.shape[0], fp(out), cap,
+                                      ctypes.byref(tt))
+    if rows < 0:
+        raise PumpError("pump_find_optimal_path", -3)
+    return out[:rows].copy(), tt.value

@@ -1,0 +1,287 @@
+// bvp.hip -- kernel (a): 2pBVP optimal arrival time and cost for every
+// ordered sample pair (reference: fillCoptsTopts, 2pBVP.cu:133-150), the
+// cost-to-goal heuristic (mainPUMP.cu:351-355) and the FP64 Newton polish of
+// arrival times.
+//
+// One lane per pair.  Pairs are enumerated directly in the output order
+// idx = i*(num-1) + k (k skips the diagonal), so stores of copts/topts are
+// fully coalesced and the source state x_i is uniform across most of a wave.
+// The work per pair is ~40 dcost evaluations, each two FP64 cost
+// evaluations: the kernel is FP64-VALU bound.  Lanes finish their bisection
+// after different iteration counts; to keep them busy each wave pulls pairs
+// from a shared counter when its lanes run dry (persistent waves).
+#include "pump_device.hpp"
+#include "pump_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace pump {
+namespace {
+
+__device__ __forceinline__ void load_state(const float *__restrict__ s, int i, float *x)
+{
+    const float *p = s + (int64_t)i * kDim;
+    for (int d = 0; d < kDim; ++d)
+        x[d] = p[d];
+}
+
+__device__ __forceinline__ void pair_of(int64_t idx, int num, int &i, int &j)
+{
+    i = (int)(idx / (num - 1));
+    int k = (int)(idx - (int64_t)i * (num - 1));
+    j = k + (k >= i ? 1 : 0);
+}
+
+// Straight one-lane-per-pair form (used for small launches and as the
+// simplest statement of the mapping).
+__global__ __launch_bounds__(256) void k_fill_pairs(const float *__restrict__ samples, int num,
+                                                    float tmax, float *__restrict__ copts,
+                                                    float *__restrict__ topts, int64_t n_pairs)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n_pairs;
+         idx += stride) {
+        int i, j;
+        pair_of(idx, num, i, j);
+        float a[kDim], b[kDim];
+        load_state(samples, i, a);
+        load_state(samples, j, b);
+        CostTerms c;
+        cost_terms(c, a, b);
+        float tau = topt_bisection(c, tmax);
+        topts[idx] = tau;
+        copts[idx] = cost_eval(c, tau);
+    }
+}
+
+// Bisection as a per-lane state machine so that a lane that finishes takes
+// the next pair while its wave-mates are still iterating.  Each step does
+// exactly one dcost evaluation; the sequence of evaluations per pair is the
+// one toptBisection performs, so the results are identical.
+enum : int { ST_TMAX = 0, ST_LOW = 1, ST_BIS = 2 };
+
+__global__ __launch_bounds__(256) void k_fill_pairs_persistent(
+    const float *__restrict__ samples, int num, float tmax, float *__restrict__ copts,
+    float *__restrict__ topts, int64_t n_pairs, unsigned long long *__restrict__ next_chunk)
+{
+    const int lane = threadIdx.x & 63;
+    // each wave claims 64 pairs at a time from the global counter
+
+    int64_t my = -1;     // current pair of this lane, -1 when idle
+    int64_t pool = 0;    // next unclaimed pair index in the wave's chunk
+    int64_t pool_end = 0;
+    CostTerms c;
+    float tu = 0, tl = 0, topt = 0, dcval = 0;
+    int state = ST_TMAX, itr = 0, halvings = 0;
+    bool done_all = false;
+
+    while (true) {
+        // refill idle lanes from the wave's chunk, claiming a new chunk as needed
+        unsigned long long idle = __ballot(my < 0);
+        while (idle != 0 && !done_all) {
+            if (pool >= pool_end) {
+                unsigned long long claimed = 0;
+                if (lane == 0)
+                    claimed = atomicAdd(next_chunk, 64ull);
+                // every lane is active here, so the first active lane is lane 0
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)claimed);
+                const uint32_t hi =
+                    (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(claimed >> 32));
+                int64_t b = (int64_t)(((uint64_t)hi << 32) | lo);
+                if (b >= n_pairs) {
+                    done_all = true;
+                    break;
+                }
+                pool = b;
+                pool_end = b + 64 < n_pairs ? b + 64 : n_pairs;
+            }
+            // rank of this lane among idle lanes
+            int rank = __popcll(idle & ((1ull << lane) - 1ull));
+            int64_t avail = pool_end - pool;
+            bool take = (my < 0) && (rank < avail);
+            if (take) {
+                my = pool + rank;
+                int i, j;
+                pair_of(my, num, i, j);
+                float a[kDim], b[kDim];
+                load_state(samples, i, a);
+                load_state(samples, j, b);
+                cost_terms(c, a, b);
+                state = ST_TMAX;
+                tu = tmax;
+                tl = 0.01f;
+                topt = 0.0f;
+                dcval = 1.0f;
+                itr = 0;
+                halvings = 0;
+            }
+            int took = __popcll(idle) < avail ? __popcll(idle) : (int)avail;
+            pool += took;
+            idle = __ballot(my < 0);
+        }
+        if (__ballot(my >= 0) == 0)
+            break;
+        if (my >= 0) {
+            // one dcost evaluation of the current phase
+            float x = (state == ST_TMAX) ? tmax : (state == ST_LOW) ? tl : (tu + tl) / 2.0f;
+            float dc = dcost_eval(c, x);
+            bool finish = false;
+            float result = 0.0f;
+            if (state == ST_TMAX) {
+                if (dc < 0.0f) {
+                    finish = true;
+                    result = tmax;
+                } else {
+                    state = ST_LOW;
+                }
+            } else if (state == ST_LOW) {
+                if (dc > 0.0f && halvings < 400) {
+                    tl = tl / 2.0f;
+                    ++halvings;
+                } else {
+                    state = ST_BIS;
+                    if (!((double)fabsf(dcval) > 0.0001 && itr < 50)) {
+                        finish = true;
+                        result = topt;
+                    }
+                }
+            } else {
+                ++itr;
+                topt = x;
+                dcval = dc;
+                if (dcval > 0.0f)
+                    tu = topt;
+                else
+                    tl = topt;
+                if (!((double)fabsf(dcval) > 0.0001 && itr < 50)) {
+                    finish = true;
+                    result = topt;
+                }
+            }
+            if (finish) {
+                topts[my] = result;
+                copts[my] = cost_eval(c, result);
+                my = -1;
+            }
+        }
+    }
+}
+
+// h[i] = cost(toptBisection(x_i, goal)) for i < num-1, h[num-1] = 0
+__global__ __launch_bounds__(256) void k_heuristic(const float *__restrict__ samples, int num,
+                                                   float g0, float g1, float g2, float g3,
+                                                   float g4, float g5, float tmax,
+                                                   float *__restrict__ h)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num)
+        return;
+    if (i == num - 1) {
+        h[i] = 0.0f;
+        return;
+    }
+    float a[kDim], b[kDim] = {g0, g1, g2, g3, g4, g5};
+    load_state(samples, i, a);
+    CostTerms c;
+    cost_terms(c, a, b);
+    float tau = topt_bisection(c, tmax);
+    h[i] = cost_eval(c, tau);
+}
+
+// FP64 Newton polish on q(t) = t^4 c'(t) = (1+g^2)t^4 - 4R t^2 + 24 S t - 36 D.
+__global__ __launch_bounds__(256) void k_polish(const float *__restrict__ samples, int num,
+                                                float tmax, const float *__restrict__ topts,
+                                                double *__restrict__ out, int64_t n_pairs)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n_pairs;
+         idx += stride) {
+        int i, j;
+        pair_of(idx, num, i, j);
+        float a[kDim], b[kDim];
+        load_state(samples, i, a);
+        load_state(samples, j, b);
+        double D = 0, S = 0, R = 0;
+        for (int d = 0; d < 3; ++d) {
+            double dp = (double)b[d] - (double)a[d];
+            double v0 = a[d + 3], v1 = b[d + 3];
+            D += dp * dp;
+            S += dp * (v0 + v1);
+            R += v0 * v0 + v0 * v1 + v1 * v1;
+        }
+        const double k4 = 1.0 + (double)kGravity * (double)kGravity;
+        double t = (double)topts[idx];
+        if (t < (double)tmax) {
+            for (int it = 0; it < 4; ++it) {
+                double q = ((k4 * t * t - 4.0 * R) * t + 24.0 * S) * t - 36.0 * D;
+                double dq = (4.0 * k4 * t * t - 8.0 * R) * t + 24.0 * S;
+                if (dq <= 0.0)
+                    break;
+                double tn = t - q / dq;
+                if (!(tn > 0.0) || tn > (double)tmax)
+                    break;
+                t = tn;
+            }
+        }
+        out[idx] = t;
+    }
+}
+
+} // namespace
+} // namespace pump
+
+using namespace pump;
+
+extern "C" int pump_fill_copts_topts(const float *d_samples, int num, float tmax, float *d_copts,
+                                     float *d_topts, pump_stream_t stream)
+{
+    PUMP_REQUIRE(d_samples && d_copts && d_topts && num >= 2);
+    const int64_t n_pairs = (int64_t)num * (num - 1);
+    hipStream_t s = as_stream(stream);
+    if (n_pairs < (1 << 16)) {
+        hipLaunchKernelGGL(k_fill_pairs, dim3(grid_for(n_pairs, 256)), dim3(256), 0, s, d_samples,
+                           num, tmax, d_copts, d_topts, n_pairs);
+        PUMP_LAUNCHED();
+        return PUMP_OK;
+    }
+    unsigned long long *counter = nullptr;
+    PUMP_HIP(hipMallocAsync((void **)&counter, sizeof(unsigned long long), s));
+    PUMP_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), s));
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    // 8 resident 256-lane workgroups per CU
+    int64_t blocks = (int64_t)cus * 8;
+    int64_t need = (n_pairs + 255) / 256;
+    if (blocks > need)
+        blocks = need;
+    hipLaunchKernelGGL(k_fill_pairs_persistent, dim3((unsigned)blocks), dim3(256), 0, s,
+                       d_samples, num, tmax, d_copts, d_topts, n_pairs, counter);
+    hipError_t le = hipGetLastError();
+    PUMP_HIP(hipFreeAsync(counter, s));
+    if (le != hipSuccess)
+        return PUMP_E_HIP;
+    return PUMP_OK;
+}
+
+extern "C" int pump_polish_topts(const float *d_samples, int num, float tmax, const float *d_topts,
+                                 double *d_topts64, pump_stream_t stream)
+{
+    PUMP_REQUIRE(d_samples && d_topts && d_topts64 && num >= 2);
+    const int64_t n_pairs = (int64_t)num * (num - 1);
+    hipLaunchKernelGGL(k_polish, dim3(grid_for(n_pairs, 256)), dim3(256), 0, as_stream(stream),
+                       d_samples, num, tmax, d_topts, d_topts64, n_pairs);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+
+extern "C" int pump_heuristic(const float *d_samples, int num, const float *goal_host, float tmax,
+                              float *d_h, pump_stream_t stream)
+{
+    PUMP_REQUIRE(d_samples && goal_host && d_h && num >= 1);
+    const float *g = goal_host;
+    hipLaunchKernelGGL(k_heuristic, dim3((num + 255) / 256), dim3(256), 0, as_stream(stream),
+                       d_samples, num, g[0], g[1], g[2], g[3], g[4], g[5], tmax, d_h);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}

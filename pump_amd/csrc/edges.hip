@@ -1,0 +1,257 @@
+// edges.hip -- kernel (b): edge waypoint generation fused with the
+// segment/AABB test (reference: findDiscretizedPath 2pBVP.cu:53-69 driven by
+// the host loop mainPUMP.cu:327-346, then freeEdges collisionCheck.cu:5-37),
+// plus the reference-shaped freeEdges and sampleFree (sampler.cu:61-88).
+//
+// Layout of the fused kernel: a workgroup owns a run of consecutive edges
+// (kEdgesPerBlock) and walks their waypoints in tiles of 256 -- one waypoint
+// per lane.  Each tile's waypoints go to LDS and to discMotions (consecutive
+// lanes write consecutive 24-byte records); segment (k, k+1) is then tested
+// by the lane holding waypoint k against the obstacle boxes, which are
+// staged once per workgroup in LDS as structure-of-arrays and read as
+// broadcasts.  Tiles overlap by one waypoint so every segment has both ends
+// in LDS.  An edge already known to be blocked skips its remaining segments,
+// and a lane stops scanning boxes at its first hit.
+#include "pump_device.hpp"
+#include "pump_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace pump {
+namespace {
+
+constexpr int kEdgesPerBlock = 32;
+constexpr int kTile = 256;
+
+template <bool kCheck, bool kWrite>
+__global__ __launch_bounds__(kTile) void k_edges(
+    const float *__restrict__ samples, const int *__restrict__ edge_src,
+    const int *__restrict__ edge_dst, const float *__restrict__ topts_edge, int64_t n_edges,
+    const int64_t *__restrict__ wp_offsets, int w_fixed, const float *__restrict__ obstacles,
+    int n_obs, float *__restrict__ disc, uint8_t *__restrict__ is_free)
+{
+    __shared__ ObsLds o;
+    __shared__ float x0s[kEdgesPerBlock][kDim], x1s[kEdgesPerBlock][kDim];
+    __shared__ float taus[kEdgesPerBlock];
+    __shared__ int64_t off[kEdgesPerBlock + 1];
+    __shared__ int ok[kEdgesPerBlock];
+    __shared__ float wp[kTile][kDim];
+
+    const int64_t e0 = (int64_t)blockIdx.x * kEdgesPerBlock;
+    const int ne = (int)((n_edges - e0) < kEdgesPerBlock ? (n_edges - e0) : kEdgesPerBlock);
+    const int t = threadIdx.x;
+    if (kCheck)
+        stage_obstacles(o, obstacles, n_obs);
+    for (int q = t; q < ne * kDim; q += kTile) {
+        int le = q / kDim, d = q - kDim * (q / kDim);
+        x0s[le][d] = samples[(int64_t)edge_src[e0 + le] * kDim + d];
+        x1s[le][d] = samples[(int64_t)edge_dst[e0 + le] * kDim + d];
+    }
+    for (int q = t; q <= ne; q += kTile)
+        off[q] = wp_offsets ? wp_offsets[e0 + q] : (e0 + q) * (int64_t)w_fixed;
+    for (int q = t; q < ne; q += kTile) {
+        taus[q] = topts_edge[e0 + q];
+        ok[q] = 1;
+    }
+    __syncthreads();
+
+    const int64_t g_begin = off[0], g_end = off[ne];
+    for (int64_t base = g_begin; base < g_end; base += kTile - 1) {
+        const int64_t g = base + t;
+        int le = 0, k = 0, W = 0;
+        if (g < g_end) {
+            le = find_edge(off, ne, g);
+            k = (int)(g - off[le]);
+            W = (int)(off[le + 1] - off[le]);
+            float x[kDim];
+            TauPowers P = tau_powers(taus[le]);
+            waypoint(k, W, P, x0s[le], x1s[le], x);
+            for (int d = 0; d < kDim; ++d)
+                wp[t][d] = x[d];
+            // the last lane's waypoint is the first of the next tile
+            if (kWrite && (t < kTile - 1 || g + 1 >= g_end))
+                for (int d = 0; d < kDim; ++d)
+                    disc[g * kDim + d] = x[d];
+        }
+        if (!kCheck)
+            continue;
+        __syncthreads();
+        bool test = (g < g_end) && (t < kTile - 1) && (k < W - 1);
+        if (test && ok[le]) {
+            if (segment_blocked(wp[t], wp[t + 1], o, obstacles, n_obs))
+                ok[le] = 0;
+        }
+        __syncthreads();
+    }
+    if (kCheck) {
+        __syncthreads();
+        for (int q = t; q < ne; q += kTile)
+            is_free[e0 + q] = (uint8_t)ok[q];
+    }
+}
+
+// freeEdges over precomputed discMotions: one lane per segment; the edge's
+// flag lives in LDS for the workgroup that owns the edge.
+__global__ __launch_bounds__(kTile) void k_free_edges(const float *__restrict__ obstacles,
+                                                      int n_obs, int w_fixed,
+                                                      const int64_t *__restrict__ wp_offsets,
+                                                      const float *__restrict__ disc,
+                                                      uint8_t *__restrict__ is_free,
+                                                      int64_t n_edges)
+{
+    __shared__ ObsLds o;
+    __shared__ int64_t off[kEdgesPerBlock + 1];
+    __shared__ int ok[kEdgesPerBlock];
+    const int64_t e0 = (int64_t)blockIdx.x * kEdgesPerBlock;
+    const int ne = (int)((n_edges - e0) < kEdgesPerBlock ? (n_edges - e0) : kEdgesPerBlock);
+    const int t = threadIdx.x;
+    stage_obstacles(o, obstacles, n_obs);
+    for (int q = t; q <= ne; q += kTile)
+        off[q] = wp_offsets ? wp_offsets[e0 + q] : (e0 + q) * (int64_t)w_fixed;
+    for (int q = t; q < ne; q += kTile)
+        ok[q] = 1;
+    __syncthreads();
+    const int64_t g_begin = off[0], g_end = off[ne];
+    for (int64_t g = g_begin + t; g < g_end; g += kTile) {
+        int le = find_edge(off, ne, g);
+        int k = (int)(g - off[le]);
+        int W = (int)(off[le + 1] - off[le]);
+        if (k < W - 1 && ok[le]) {
+            float v[3], w[3];
+            for (int d = 0; d < 3; ++d) {
+                v[d] = disc[g * kDim + d];
+                w[d] = disc[(g + 1) * kDim + d];
+            }
+            if (segment_blocked(v, w, o, obstacles, n_obs))
+                ok[le] = 0;
+        }
+    }
+    __syncthreads();
+    for (int q = t; q < ne; q += kTile)
+        is_free[e0 + q] = (uint8_t)ok[q];
+}
+
+__global__ __launch_bounds__(256) void k_sample_free(const float *__restrict__ obstacles, int n_obs,
+                                                     const float *__restrict__ samples, int num,
+                                                     uint8_t *__restrict__ is_free)
+{
+    __shared__ ObsLds o;
+    stage_obstacles(o, obstacles, n_obs);
+    __syncthreads();
+    int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= num)
+        return;
+    float p[3] = {samples[(int64_t)n * kDim], samples[(int64_t)n * kDim + 1],
+                  samples[(int64_t)n * kDim + 2]};
+    bool free_ = true;
+    for (int b = 0; b < n_obs && free_; ++b) {
+        float lo[3], hi[3];
+        box_bounds(o, obstacles, b, lo, hi);
+        if (p[0] > lo[0] && p[0] < hi[0] && p[1] > lo[1] && p[1] < hi[1] && p[2] > lo[2] &&
+            p[2] < hi[2])
+            free_ = false;
+    }
+    is_free[n] = free_ ? 1 : 0;
+}
+
+} // namespace
+} // namespace pump
+
+using namespace pump;
+
+static int edge_blocks(int64_t n_edges, unsigned &blocks)
+{
+    int64_t b = (n_edges + kEdgesPerBlock - 1) / kEdgesPerBlock;
+    if (b > 0x7fffffff)
+        return PUMP_E_RANGE;
+    blocks = (unsigned)(b > 0 ? b : 1);
+    return PUMP_OK;
+}
+
+extern "C" int pump_discretize_edges(const float *d_samples, const int *d_edge_src,
+                                     const int *d_edge_dst, const float *d_topts_edge,
+                                     int64_t n_edges, const int64_t *d_wp_offsets,
+                                     int waypoints_count, float *d_disc_motions,
+                                     pump_stream_t stream)
+{
+    PUMP_REQUIRE(n_edges >= 0);
+    if (n_edges == 0)
+        return PUMP_OK;
+    PUMP_REQUIRE(d_samples && d_edge_src && d_edge_dst && d_topts_edge && d_disc_motions);
+    PUMP_REQUIRE(d_wp_offsets || waypoints_count >= 2);
+    unsigned blocks;
+    int r = edge_blocks(n_edges, blocks);
+    if (r)
+        return r;
+    hipLaunchKernelGGL((k_edges<false, true>), dim3(blocks), dim3(kTile), 0, as_stream(stream),
+                       d_samples, d_edge_src, d_edge_dst, d_topts_edge, n_edges, d_wp_offsets,
+                       waypoints_count, nullptr, 0, d_disc_motions, nullptr);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+
+extern "C" int pump_edges_discretize_check(const float *d_samples, const int *d_edge_src,
+                                           const int *d_edge_dst, const float *d_topts_edge,
+                                           int64_t n_edges, const int64_t *d_wp_offsets,
+                                           int waypoints_count, const float *d_obstacles,
+                                           int n_obs, float *d_disc_motions,
+                                           uint8_t *d_is_free_edges, pump_stream_t stream)
+{
+    PUMP_REQUIRE(n_edges >= 0 && n_obs >= 0);
+    if (n_edges == 0)
+        return PUMP_OK;
+    PUMP_REQUIRE(d_samples && d_edge_src && d_edge_dst && d_topts_edge && d_is_free_edges);
+    PUMP_REQUIRE(n_obs == 0 || d_obstacles);
+    PUMP_REQUIRE(d_wp_offsets || waypoints_count >= 2);
+    unsigned blocks;
+    int r = edge_blocks(n_edges, blocks);
+    if (r)
+        return r;
+    if (d_disc_motions)
+        hipLaunchKernelGGL((k_edges<true, true>), dim3(blocks), dim3(kTile), 0, as_stream(stream),
+                           d_samples, d_edge_src, d_edge_dst, d_topts_edge, n_edges, d_wp_offsets,
+                           waypoints_count, d_obstacles, n_obs, d_disc_motions, d_is_free_edges);
+    else
+        hipLaunchKernelGGL((k_edges<true, false>), dim3(blocks), dim3(kTile), 0,
+                           as_stream(stream), d_samples, d_edge_src, d_edge_dst, d_topts_edge,
+                           n_edges, d_wp_offsets, waypoints_count, d_obstacles, n_obs, nullptr,
+                           d_is_free_edges);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+
+extern "C" int pump_free_edges(const float *d_obstacles, int n_obs, const float *d_samples,
+                               const uint8_t *d_is_free_samples, int waypoints_count,
+                               const int64_t *d_wp_offsets, const float *d_disc_motions,
+                               uint8_t *d_is_free_edges, int64_t n_edges, pump_stream_t stream)
+{
+    (void)d_samples;          // passed but unused, as in the reference
+    (void)d_is_free_samples;  // idem (collisionCheck.cu:5-37)
+    PUMP_REQUIRE(n_edges >= 0 && n_obs >= 0);
+    if (n_edges == 0)
+        return PUMP_OK;
+    PUMP_REQUIRE(d_disc_motions && d_is_free_edges && (n_obs == 0 || d_obstacles));
+    PUMP_REQUIRE(d_wp_offsets || waypoints_count >= 1);
+    unsigned blocks;
+    int r = edge_blocks(n_edges, blocks);
+    if (r)
+        return r;
+    hipLaunchKernelGGL(k_free_edges, dim3(blocks), dim3(kTile), 0, as_stream(stream), d_obstacles,
+                       n_obs, waypoints_count, d_wp_offsets, d_disc_motions, d_is_free_edges,
+                       n_edges);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+
+extern "C" int pump_sample_free(const float *d_obstacles, int n_obs, const float *d_samples,
+                                int num, uint8_t *d_is_free_samples, pump_stream_t stream)
+{
+    PUMP_REQUIRE(num >= 0 && n_obs >= 0);
+    if (num == 0)
+        return PUMP_OK;
+    PUMP_REQUIRE(d_samples && d_is_free_samples && (n_obs == 0 || d_obstacles));
+    hipLaunchKernelGGL(k_sample_free, dim3((num + 255) / 256), dim3(256), 0, as_stream(stream),
+                       d_obstacles, n_obs, d_samples, num, d_is_free_samples);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}

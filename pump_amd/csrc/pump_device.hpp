@@ -1,0 +1,348 @@
+// pump_device.hpp -- device arithmetic for PUMP's 2pBVP and collision tests
+// on gfx950.
+//
+// Results are bit-identical to the reference's host build (x86-64, no FP
+// contraction, pow(float,int) promoted to double):
+//   * contraction is disabled for every function in this header (and the
+//     library is compiled with -ffp-contract=off);
+//   * pow(x,2) of a float is formed as x*x in double (exact); pow(x,3) and
+//     pow(x,4) as (x*x)*x and (x*x)*(x*x), the correctly rounded powers.
+//     glibc's pow differs from the correctly rounded value by one double ulp
+//     for about 0.2% of arguments; after the final rounding to float the
+//     outputs agree except in rare last-place cases, which the parity tests
+//     count and bound;
+//   * per-pair quantities that do not depend on tau are formed once (same
+//     operations, same order) and reused by every cost evaluation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace pump {
+
+constexpr int kDim = 6;
+constexpr float kGravity = -9.81f;  // 2pBVP.cu:3
+
+// ---------------------------------------------------------------------------
+// cost(tau, x0, x1), 2pBVP.cu:97-107, split into the tau-independent part
+// (terms 1-7 of the numerator, summed in source order) and per-term factors.
+// ---------------------------------------------------------------------------
+struct CostTerms {
+    double s7;                                     // 12a0^2+12a2^2+12a1^2+12b0^2+12b2^2-24a1b1+12b1^2
+    float f8, f9, f10, f11, f12, f13, f14, f15;    // float products multiplied by tau
+    double d16, d17, d18, d20, d22, d24;           // 4*pow(v,2), times tau^2
+    float f19, f21, f23;                           // 4*v0*v1 (float), times tau^2
+    float f25, f26;                                // 2*g*vz (float), times tau^3
+    float f29a, f29b, f29c, f30a, f30b, f30c;      // 12*p0*(-2*p1 + (v0+v1)*tau)
+};
+
+__host__ __device__ __forceinline__ double sq(float x) { return (double)x * (double)x; }
+
+__host__ __device__ __forceinline__ void cost_terms(CostTerms &c, const float *a, const float *b)
+{
+    const float g = kGravity;
+    double s = 12.0 * sq(a[0]);
+    s = s + 12.0 * sq(a[2]);
+    s = s + 12.0 * sq(a[1]);
+    s = s + 12.0 * sq(b[0]);
+    s = s + 12.0 * sq(b[2]);
+    s = s - (double)(24.0f * a[1] * b[1]);
+    s = s + 12.0 * sq(b[1]);
+    c.s7 = s;
+    c.f8 = 12.0f * a[4] * a[1];
+    c.f9 = 12.0f * a[1] * b[4];
+    c.f10 = 12.0f * a[3] * b[0];
+    c.f11 = 12.0f * b[3] * b[0];
+    c.f12 = 12.0f * a[5] * b[2];
+    c.f13 = 12.0f * b[5] * b[2];
+    c.f14 = 12.0f * a[4] * b[1];
+    c.f15 = 12.0f * b[4] * b[1];
+    c.d16 = 4.0 * sq(a[4]);
+    c.d17 = 4.0 * sq(a[3]);
+    c.d18 = 4.0 * sq(a[5]);
+    c.f19 = 4.0f * a[4] * b[4];
+    c.d20 = 4.0 * sq(b[4]);
+    c.f21 = 4.0f * a[3] * b[3];
+    c.d22 = 4.0 * sq(b[3]);
+    c.f23 = 4.0f * a[5] * b[5];
+    c.d24 = 4.0 * sq(b[5]);
+    c.f25 = 2.0f * g * a[5];
+    c.f26 = 2.0f * g * b[5];
+    c.f29a = 12.0f * a[0];
+    c.f29b = -2.0f * b[0];
+    c.f29c = a[3] + b[3];
+    c.f30a = 12.0f * a[2];
+    c.f30b = -2.0f * b[2];
+    c.f30c = a[5] + b[5];
+}
+
+__host__ __device__ __forceinline__ float cost_eval(const CostTerms &c, float tau)
+{
+    const double g2 = (double)kGravity * (double)kGravity;
+    const double t = (double)tau;
+    const double t2 = t * t;   // exact
+    const double t3 = t2 * t;  // correctly rounded
+    const double t4 = t2 * t2; // correctly rounded
+    double s = c.s7;
+    s = s + (double)(c.f8 * tau);
+    s = s + (double)(c.f9 * tau);
+    s = s - (double)(c.f10 * tau);
+    s = s - (double)(c.f11 * tau);
+    s = s - (double)(c.f12 * tau);
+    s = s - (double)(c.f13 * tau);
+    s = s - (double)(c.f14 * tau);
+    s = s - (double)(c.f15 * tau);
+    s = s + c.d16 * t2;
+    s = s + c.d17 * t2;
+    s = s + c.d18 * t2;
+    s = s + (double)c.f19 * t2;
+    s = s + c.d20 * t2;
+    s = s + (double)c.f21 * t2;
+    s = s + c.d22 * t2;
+    s = s + (double)c.f23 * t2;
+    s = s + c.d24 * t2;
+    s = s + (double)c.f25 * t3;
+    s = s - (double)c.f26 * t3;
+    s = s + t4;
+    s = s + g2 * t4;
+    s = s + (double)(c.f29a * (c.f29b + c.f29c * tau));
+    s = s + (double)(c.f30a * (c.f30b + c.f30c * tau));
+    return (float)((1.0 / t3) * s);
+}
+
+// dcost, 2pBVP.cu:109-115: float central difference, dtau = 1e-6f
+__host__ __device__ __forceinline__ float dcost_eval(const CostTerms &c, float tau)
+{
+    const float h = 0.000001f;
+    float up = cost_eval(c, tau + h / 2.0f);
+    float dn = cost_eval(c, tau - h / 2.0f);
+    return (up - dn) / h;
+}
+
+// toptBisection, 2pBVP.cu:71-95 (halving loop capped at 400: the reference
+// would not terminate once tl underflows to zero).
+__host__ __device__ __forceinline__ float topt_bisection(const CostTerms &c, float tmax)
+{
+    float tu = tmax;
+    if (dcost_eval(c, tmax) < 0.0f)
+        return tmax;
+    float tl = 0.01f;
+    for (int k = 0; k < 400 && dcost_eval(c, tl) > 0.0f; ++k)
+        tl = tl / 2.0f;
+    float topt = 0.0f;
+    float dcval = 1.0f;
+    int itr = 0;
+    while ((double)fabsf(dcval) > 0.0001 && itr < 50) {
+        ++itr;
+        topt = (tu + tl) / 2.0f;
+        dcval = dcost_eval(c, topt);
+        if (dcval > 0.0f)
+            tu = topt;
+        else
+            tl = topt;
+    }
+    return topt;
+}
+
+// ---------------------------------------------------------------------------
+// pathPoint(t, tau, x0, x1), 2pBVP.cu:117-131
+// ---------------------------------------------------------------------------
+struct TauPowers {
+    float tau;
+    double t2, t3; // tau^2 (exact), tau^3
+};
+
+__host__ __device__ __forceinline__ TauPowers tau_powers(float tau)
+{
+    TauPowers p;
+    p.tau = tau;
+    p.t2 = (double)tau * (double)tau;
+    p.t3 = p.t2 * (double)tau;
+    return p;
+}
+
+__host__ __device__ __forceinline__ float pp_pos_xy(float t, double s2, double s3, const TauPowers &P,
+                                           float p0, float v0, float p1, float v1)
+{
+    const float tau = P.tau;
+    float lin = t * v0 + p0;
+    float c3 = 2.0f * p0 - 2.0f * p1 + (v0 + v1) * tau;
+    float c2 = 3.0f * p0 - 3.0f * p1 + (2.0f * v0 + v1) * tau;
+    return (float)((double)lin + (s3 * (double)c3) / P.t3 - (s2 * (double)c2) / P.t2);
+}
+
+__host__ __device__ __forceinline__ float pp_vel_xy(float t, double s2, const TauPowers &P, float p0,
+                                           float v0, float p1, float v1)
+{
+    const float tau = P.tau;
+    float e = 3.0f * p0 - 3.0f * p1 + v1 * tau;
+    float f = -(2.0f * t * e);
+    float c = 2.0f * p0 - 2.0f * p1 + v1 * tau;
+    double q = 3.0 * s2 - (double)(4.0f * t * tau) + P.t2;
+    return (float)((double)f / P.t2 + (3.0 * s2 * (double)c) / P.t3 + ((double)v0 * q) / P.t2);
+}
+
+__host__ __device__ __forceinline__ void path_point(float t, const TauPowers &P, const float *x0,
+                                           const float *x1, float *x)
+{
+    const float tau = P.tau;
+    const double s2 = (double)t * (double)t;
+    const double s3 = s2 * (double)t;
+    x[0] = pp_pos_xy(t, s2, s3, P, x0[0], x0[3], x1[0], x1[3]);
+    x[1] = pp_pos_xy(t, s2, s3, P, x0[1], x0[4], x1[1], x1[4]);
+    {
+        double a = (double)(t * x0[5]) * P.t3;
+        double b = (double)x0[2] * P.t3;
+        double c = s2 * (double)tau *
+                   (-3.0 * (double)x0[2] + 3.0 * (double)x1[2] - 2.0 * (double)x0[5] * (double)tau -
+                    1.0 * (double)x1[5] * (double)tau);
+        double d = s3 * (2.0 * (double)x0[2] - 2.0 * (double)x1[2] + (double)((x0[5] + x1[5]) * tau));
+        x[2] = (float)((a + b + c + d) / P.t3);
+    }
+    x[3] = pp_vel_xy(t, s2, P, x0[0], x0[3], x1[0], x1[3]);
+    x[4] = pp_vel_xy(t, s2, P, x0[1], x0[4], x1[1], x1[4]);
+    {
+        double a = (double)x0[5] * P.t3;
+        double b = (double)(t * tau) *
+                   (-6.0 * (double)x0[2] + 6.0 * (double)x1[2] - 4.0 * (double)x0[5] * (double)tau -
+                    2.0 * (double)x1[5] * (double)tau);
+        double c = s2 * (6.0 * (double)x0[2] - 6.0 * (double)x1[2] + 3.0 * (double)x0[5] * (double)tau +
+                         3.0 * (double)x1[5] * (double)tau);
+        x[5] = (float)((a + b + c) / P.t3);
+    }
+}
+
+// Waypoint k of an edge with W waypoints (findDiscretizedPath,
+// 2pBVP.cu:53-69): endpoints are copies, interior points at (tau/(W-1))*k.
+__host__ __device__ __forceinline__ void waypoint(int k, int W, const TauPowers &P, const float *x0,
+                                         const float *x1, float *x)
+{
+    if (k == 0) {
+        for (int d = 0; d < kDim; ++d)
+            x[d] = x0[d];
+    } else if (k == W - 1) {
+        for (int d = 0; d < kDim; ++d)
+            x[d] = x1[d];
+    } else {
+        float dt = P.tau / (float)(W - 1);
+        path_point(dt * (float)k, P, x0, x1, x);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Segment vs axis-aligned box, collisionCheck.cu:112-190.
+// Boxes are given by their position bounds lo[3], hi[3].
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ bool face_contains(const float *v, const float *w, float lambda, int j,
+                                              const float *lo, const float *hi)
+{
+    for (int d = 0; d < 3; ++d) {
+        float proj = v[d] + (w[d] - v[d]) * lambda;
+        if (d != j && !(lo[d] <= proj && proj <= hi[d]))
+            return false;
+    }
+    return true;
+}
+
+// true if the segment v->w collides with the box (broad phase overlap and a
+// face projection inside the box)
+__host__ __device__ __forceinline__ bool segment_hits_box(const float *v, const float *w,
+                                                 const float *bmin, const float *bmax,
+                                                 const float *lo, const float *hi)
+{
+    for (int d = 0; d < 3; ++d)
+        if (bmax[d] <= lo[d] || hi[d] <= bmin[d])
+            return false;
+    for (int d = 0; d < 3; ++d) {
+        float vw = w[d] - v[d];
+        float lambda = (v[d] < lo[d]) ? (lo[d] - v[d]) / vw : (hi[d] - v[d]) / vw;
+        if (face_contains(v, w, lambda, d, lo, hi))
+            return true;
+    }
+    return false;
+}
+
+__host__ __device__ __forceinline__ void seg_bbox(const float *v, const float *w, float *bmin, float *bmax)
+{
+    for (int d = 0; d < 3; ++d) {
+        if (v[d] > w[d]) {
+            bmin[d] = w[d];
+            bmax[d] = v[d];
+        } else {
+            bmin[d] = v[d];
+            bmax[d] = w[d];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Obstacle boxes staged in LDS (position bounds only, structure of arrays);
+// boxes past the capacity are read from global memory by the callers.
+// ---------------------------------------------------------------------------
+constexpr int kMaxObsLds = 512;
+
+struct ObsLds {
+    float lo[3][kMaxObsLds];
+    float hi[3][kMaxObsLds];
+};
+
+__device__ __forceinline__ void stage_obstacles(ObsLds &o, const float *__restrict__ obs, int n)
+{
+    const int m = n < kMaxObsLds ? n : kMaxObsLds;
+    for (int q = threadIdx.x; q < m * 3; q += blockDim.x) {
+        int b = q / 3, d = q - 3 * (q / 3);
+        o.lo[d][b] = obs[b * 12 + d];
+        o.hi[d][b] = obs[b * 12 + 6 + d];
+    }
+}
+
+__device__ __forceinline__ void box_bounds(const ObsLds &o, const float *__restrict__ obs, int b,
+                                           float *lo, float *hi)
+{
+    if (b < kMaxObsLds) {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = o.lo[d][b];
+            hi[d] = o.hi[d][b];
+        }
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = obs[b * 12 + d];
+            hi[d] = obs[b * 12 + 6 + d];
+        }
+    }
+}
+
+// true if segment v->w hits any of the n_obs boxes
+__device__ __forceinline__ bool segment_blocked(const float *v, const float *w, const ObsLds &o,
+                                                const float *__restrict__ obs, int n_obs)
+{
+    float bmin[3], bmax[3];
+    seg_bbox(v, w, bmin, bmax);
+    for (int b = 0; b < n_obs; ++b) {
+        float lo[3], hi[3];
+        box_bounds(o, obs, b, lo, hi);
+        if (segment_hits_box(v, w, bmin, bmax, lo, hi))
+            return true;
+    }
+    return false;
+}
+
+// index of the edge (within a block's edge run) holding global waypoint g,
+// off[0] <= g < off[ne]
+__device__ __forceinline__ int find_edge(const int64_t *off, int ne, int64_t g)
+{
+    int lo = 0, hi = ne;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+} // namespace pump

@@ -1,0 +1,145 @@
+"""CPU-side checks of the product library: it loads, exports every symbol the
+headers declare, and its host logic (Philox, problem and noise generation,
+the host 2pBVP arithmetic used by findOptimalPath) is right.  No GPU calls."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pump_amd as P
+from pump_amd import configs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for h in ("pump_abi.h", "pump_host.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(pump_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) > 40
+    missing = [n for n in sorted(names) if not hasattr(P.lib, n)]
+    assert not missing, missing
+
+
+def test_status_strings_and_version():
+    assert P.lib.pump_abi_version() == 1
+    assert P.lib.pump_status_string(0) == b"ok"
+    assert P.lib.pump_status_string(-5).startswith(b"argument")
+
+
+def test_invalid_arguments_rejected_without_gpu():
+    # argument validation happens before any device call
+    assert P.lib.pump_fill_copts_topts(None, 100, 5.0, None, None, None) == -1
+    assert P.lib.pump_select_rn(None, 10, 3, None, None, 0, None) == -1
+    assert P.lib.pump_propagate_cp(-1, 9, None, None, None, None, None, None, None, 0.05, 5, None,
+                                   10, 1.0, 64, None, None, 0, 64, None, 0, None, None) == -1
+
+
+def test_philox_known_answers():
+    # Random123 known-answer vectors for philox4x32-10
+    assert P.philox4x32([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert P.philox4x32([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E,
+                                                              0xA20BC7C6, 0x6D5451FD]
+    assert P.philox4x32([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                        [0xA4093822, 0x299F31D0]) == [0xD16CFE09, 0x94FDCCEB, 0x5001E420,
+                                                      0x24126EA1]
+
+
+def test_problem_generator_deterministic_and_bounded():
+    cfg = configs.config(0)
+    s1, o1 = P.gen_problem(cfg["spec"])
+    s2, o2 = P.gen_problem(cfg["spec"])
+    assert np.array_equal(s1, s2) and np.array_equal(o1, o2)
+    assert s1.shape == (4000, 6) and o1.shape == (10, 12)
+    assert np.array_equal(s1[0], np.float32([0.1, 0.1, 0.1, 0, 0, 0]))
+    assert np.array_equal(s1[-1], np.float32([0.9, 0.9, 0.9, 0, 0, 0]))
+    assert np.all((s1[1:-1, :3] >= 0) & (s1[1:-1, :3] < 1))
+    assert np.all((s1[1:-1, 3:] >= -0.5) & (s1[1:-1, 3:] < 0.5))
+    side = o1[:, 6:9] - o1[:, :3]
+    centre = (o1[:, 6:9] + o1[:, :3]) / 2
+    assert np.all((side > 0.0499) & (side < 0.2001))
+    assert np.all((centre > 0.1999) & (centre < 0.8001))
+    assert np.all(o1[:, 3:6] == np.float32(-3.1)) and np.all(o1[:, 9:12] == np.float32(3.1))
+    # uniformity of the position draws
+    m = s1[1:-1, :3].mean(0)
+    assert np.all(np.abs(m - 0.5) < 0.02)
+
+
+def test_walls_generator():
+    cfg = configs.config(3, num=1000)
+    s, o = P.gen_problem(cfg["spec"])
+    side = np.sort(o[:, 6:9] - o[:, :3], axis=1)
+    assert np.allclose(side[:, 0], 0.02, atol=1e-6)
+    assert np.all((side[:, 1] >= 0.05 - 1e-6) & (side[:, 2] <= 0.8 + 1e-6))
+
+
+def test_noise_tables_layout_and_covariance():
+    ns = P.noise_spec(n_mc=4096, tmax_steps=120)
+    ot, xc, sig = P.make_noise(ns, sigma=True)
+    assert np.array_equal(ot.transpose(1, 0, 2), xc)
+    assert np.all(ot[0] == 0)  # particles start on the nominal path
+    # sample covariance of the offsets matches the FP32-propagated Sigma_t
+    for t in (5, 20, 60, 119):
+        emp = np.cov(ot[t].astype(np.float64).T)
+        ref = sig[t].astype(np.float64)
+        se = np.sqrt((ref ** 2 + np.outer(np.diag(ref), np.diag(ref))) / ns.n_mc)
+        assert np.all(np.abs(emp - ref) <= 5 * se + 1e-9), t
+    sd = np.sqrt(sig[119][0, 0])
+    assert 0.03 < sd < 0.08
+    # independent particles: lag-0 correlation across particles ~ 0
+    c = np.corrcoef(ot[60, :-1, 0], ot[60, 1:, 0])[0, 1]
+    assert abs(c) < 0.06
+
+
+def test_host_bvp_arithmetic_matches_oracle():
+    """The library's 2pBVP arithmetic (used on the host by findOptimalPath and,
+    identically, by the kernels) agrees with the oracle's glibc-pow
+    restatement: bit-exact except for rare last-place double differences."""
+    import oracle
+    O = oracle.load()
+    rng = np.random.default_rng(5)
+    n, diff_t, diff_c = 20000, 0, 0
+    for _ in range(n):
+        a = np.concatenate([rng.uniform(0, 1, 3), rng.uniform(-0.5, 0.5, 3)]).astype(np.float32)
+        b = np.concatenate([rng.uniform(0, 1, 3), rng.uniform(-0.5, 0.5, 3)]).astype(np.float32)
+        t1 = P.lib.pump_host_topt(P.fp(a), P.fp(b), 5.0)
+        t2 = O.po_topt_bisection(oracle.fp(a), oracle.fp(b), 5.0)
+        diff_t += np.float32(t1).tobytes() != np.float32(t2).tobytes()
+        c1 = P.lib.pump_host_cost(np.float32(t2), P.fp(a), P.fp(b))
+        c2 = O.po_cost(np.float32(t2), oracle.fp(a), oracle.fp(b))
+        diff_c += np.float32(c1).tobytes() != np.float32(c2).tobytes()
+    assert diff_t <= n * 1e-3 and diff_c <= n * 1e-3, (diff_t, diff_c)
+
+
+def test_host_find_optimal_path_matches_golden(golden_dir):
+    g = np.load(os.path.join(golden_dir, "bvp_optimal_path.npz"))
+    exact = 0
+    for k in range(len(g["totals"])):
+        nwp, plen = g["lens"][k]
+        path, tot = P.find_optimal_path(g["chains"][k][:nwp], dt=0.05)
+        assert path.shape[0] == plen
+        exact += path.tobytes() == g["paths"][k][:plen].tobytes()
+        assert np.allclose(path, g["paths"][k][:plen], rtol=1e-5, atol=1e-6)
+    assert exact >= len(g["totals"]) - 2
+
+
+def test_configs_map_reference_parameters():
+    c0 = configs.config(0)
+    assert c0["params"].w_fixed == 20 and c0["params"].n_p == 1000
+    assert abs(c0["params"].offset_mult - 0.02) < 1e-9
+    rn_idx = P.lib.pump_rn_index(4000, c0["params"].rn_perc)
+    assert abs(rn_idx / 4000 - 50) < 0.01
+    c1 = configs.config(1)
+    assert c1["params"].w_fixed == 0 and c1["params"].w_cap == 64 and c1["params"].n_p == 2000
+    assert P.lib.pump_maxd2(np.float32(0.5)) == np.float32(0.1 * np.float32(0.5) ** 2)
+    b = configs.batch_config(16)
+    assert all(1000 <= x["spec"].num <= 8000 for x in b)

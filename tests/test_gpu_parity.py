@@ -1,0 +1,371 @@
+"""GPU parity: every entry point of include/pump_abi.h, called through the C
+ABI on an MI355X, against the CPU oracle on the same seeded inputs.
+
+Bar: bit-exact for flags, indices, neighbour sets, counts and the returned
+path; arrival times / costs / waypoints bit-exact in practice, with a
+tolerance of 1e-5 relative written into each assertion for the rare
+last-place double rounding of pow() (see pump_device.hpp)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+import oracle  # noqa: E402
+import oracle.pipeline as OP  # noqa: E402
+import pump_amd as P  # noqa: E402
+from pump_amd import configs  # noqa: E402
+
+O = oracle.load()
+REL = 1e-5
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def assert_close_f32(got, ref, rel=REL, max_mismatch_frac=1e-5):
+    got = np.asarray(got, np.float32)
+    ref = np.asarray(ref, np.float32)
+    assert got.shape == ref.shape
+    same = got.view(np.uint32) == ref.view(np.uint32)
+    frac = 1 - same.mean() if same.size else 0.0
+    assert frac <= max_mismatch_frac, f"{(~same).sum()} of {same.size} differ"
+    if not same.all():
+        d = np.abs(got[~same].astype(np.float64) - ref[~same])
+        assert np.all(d <= rel * np.abs(ref[~same]) + 1e-30)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    print("gpu:", torch.cuda.get_device_name(0), flush=True)
+    assert torch.cuda.is_available()
+
+
+# ------------------------------------------------------------ pipeline ----
+CASES = [
+    # (num, n_obs, w_fixed, offset_mult, n_p, k, seed)
+    (192, 10, 20, 0.5, 256, 30, 0),
+    (256, 12, 0, 1.0, 300, 25, 1),     # variable waypoints, ragged n_p
+    (300, 0, 9, 0.5, 128, 30, 2),      # no obstacles
+    (200, 40, 0, 0.7, 200, 40, 3),     # many boxes
+]
+
+
+def build_pair(case, **over):
+    num, n_obs, w_fixed, om, n_p, k, seed = case
+    cfg = configs.small(num=num, n_obs=n_obs, seed=seed, w_fixed=w_fixed, n_p=n_p,
+                        offset_mult=om, k=k, n_mc=512, tmax_steps=1200, keep_pairs=1,
+                        **over)
+    s, o = P.gen_problem(cfg["spec"])
+    ot, xc = P.make_noise(cfg["noise"])
+    print("case", case, flush=True)
+    rm = P.Roadmap(s, o, s[-1], ot, xc, cfg["params"])
+    orc = OP.roadmap(s, o, s[-1], cfg["params"], nthreads=8)
+    return cfg, s, o, ot, xc, rm, orc
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_roadmap_parity(case):
+    cfg, s, o, ot, xc, rm, orc = build_pair(case)
+    c, t = rm.pairs()
+    assert_close_f32(t, orc["topts"])
+    assert_close_f32(c, orc["copts"])
+    assert rm.info.rn_idx == orc["rn_idx"]
+    assert np.float32(rm.info.rn) == orc["rn"]
+    e = rm.edges()
+    assert rm.info.n_edges == orc["n_edges"]
+    assert np.array_equal(e["row_start"], orc["row_start"])
+    assert np.array_equal(e["edge_src"], orc["edge_src"])
+    assert np.array_equal(e["edge_dst"], orc["edge_dst"])
+    assert_close_f32(e["copts_edge"], orc["copts_edge"])
+    assert_close_f32(e["topts_edge"], orc["topts_edge"])
+    assert_close_f32(e["h"], orc["h"])
+    assert rm.info.max_nn == orc["max_nn"]
+    f = rm.flags()
+    assert np.array_equal(f["is_free_samples"], orc["is_free_samples"])
+    assert np.array_equal(f["wp_offsets"], orc["wp_offsets"])
+    assert np.array_equal(f["is_free_edges"], orc["is_free_edges"])
+    assert_close_f32(rm.waypoints(), orc["disc"])
+    hs = rm.halfspaces()
+    assert np.array_equal(hs["count_stored"], orc["count_stored"])
+    assert_close_f32(hs["bs"], orc["bs"])
+    assert_close_f32(hs["as_"], orc["as_"])
+    rm.close()
+
+
+def test_nn_tables_parity():
+    cfg, s, o, ot, xc, rm, orc = build_pair(CASES[0])
+    nn = rm.nn_tables(dense=True)
+    num, m = s.shape[0], rm.info.max_nn
+    go = np.empty((num, m), np.int32)
+    come = np.empty((num, m), np.int32)
+    dist = np.empty((num, m), np.float32)
+    O.po_nn_tables(oracle.lp(orc["row_start"]), oracle.ip(orc["edge_dst"]),
+                   oracle.fp(orc["copts_edge"]), num, m, oracle.ip(go), oracle.ip(come),
+                   oracle.fp(dist))
+    assert np.array_equal(nn["nn_go_edges"], go)
+    assert np.array_equal(nn["nn_come_edges"], come)
+    assert np.array_equal(nn["distances_come"], dist)
+    dense = np.full((num, num), -3, np.int32)
+    dense[orc["edge_dst"], orc["edge_src"]] = np.arange(orc["n_edges"], dtype=np.int32)
+    assert np.array_equal(nn["nn_idxs"], dense)
+    rm.close()
+
+
+@pytest.mark.parametrize("case", CASES[:2] + CASES[3:])
+@pytest.mark.parametrize("bits", [0, 1])
+def test_search_parity(case, bits):
+    cfg, s, o, ot, xc, rm, orc = build_pair(case, validity_bits=bits)
+    prm = cfg["params"]
+    got = P.search_result_dict(rm.search())
+    ref = OP.search(orc, s, s[-1], o, ot, xc, prm, nthreads=8, path_cap=prm.path_cap)
+    print("gpu", got, "\noracle", ref, flush=True)
+    for k in ("status", "itrs", "num_paths", "num_considered", "num_goal_paths", "soln_path_idx",
+              "soln_nodes", "particle_steps", "clamped_time_idx"):
+        assert got[k] == ref[k], k
+    for k in ("soln_cost", "soln_cp", "soln_time", "soln_hsmc_cp"):
+        assert np.float32(got[k]) == np.float32(ref[k]), k
+    rm.close()
+
+
+# ------------------------------------------- entry points called directly ----
+def random_hsmc_inputs(seed, n_edges=60, W=7, max_hs=4, n_p=333, n_mc=512, T=80, n_new=50,
+                       n_parent=9, variable=False):
+    rng = np.random.default_rng(seed)
+    if variable:
+        w = rng.integers(2, 40, n_edges)
+    else:
+        w = np.full(n_edges, W)
+    off = np.zeros(n_edges + 1, np.int64)
+    np.cumsum(w, out=off[1:])
+    n_wp = int(off[-1])
+    hs = n_wp * max_hs
+    as_ = rng.normal(0, 0.05, (hs, 3)).astype(np.float32)
+    bs = (rng.normal(0, 0.05, hs) ** 2 * 0.5).astype(np.float32)
+    bs[rng.uniform(size=hs) < 0.6] = 1111
+    bs[rng.uniform(size=hs) < 0.02] = 0.0
+    offs = rng.normal(0, 0.05, (T, n_mc, 3)).astype(np.float32)
+    topts = rng.uniform(0.2, 2.0, n_edges).astype(np.float32)
+    valid = np.zeros((n_parent + n_new, n_p), np.uint8)
+    valid[:n_parent] = rng.uniform(size=(n_parent, n_p)) < 0.85
+    times = np.zeros(n_parent + n_new, np.float32)
+    times[:n_parent] = rng.uniform(0, 2.0, n_parent)
+    times[n_parent - 1] = 3.9  # pushes time indices past the table end
+    wf_edge = rng.integers(0, n_edges, n_new).astype(np.int32)
+    wf_prev = rng.integers(0, n_parent, n_new).astype(np.int32)
+    return dict(off=off, w=w, as_=as_, bs=bs, offs=offs, topts=topts, valid=valid, times=times,
+                wf_edge=wf_edge, wf_prev=wf_prev, max_hs=max_hs, n_p=n_p, n_mc=n_mc, T=T,
+                n_new=n_new, n_parent=n_parent, om=np.float32(0.8), dt=np.float32(0.05))
+
+
+def oracle_propagate(d):
+    v = d["valid"].copy()
+    cl = O.po_propagate_valid_particles(
+        d["n_new"], oracle.lp(d["off"]), oracle.ip(d["wf_edge"]), oracle.ip(d["wf_prev"]),
+        d["n_parent"], oracle.fp(d["as_"]), oracle.fp(d["bs"]), oracle.fp(d["times"]),
+        oracle.fp(d["topts"]), d["dt"], d["max_hs"], oracle.fp(d["offs"]), d["T"], d["om"],
+        d["n_mc"], oracle.bp(v), d["n_p"], 1)
+    cp = np.ones(d["n_parent"] + d["n_new"], np.float32)
+    O.po_calculate_cp_hsmc(d["n_new"], d["n_parent"], oracle.fp(cp), oracle.bp(v), d["n_p"])
+    return v, cp, cl
+
+
+@pytest.mark.parametrize("variable", [False, True])
+def test_propagate_entry_points(variable):
+    d = random_hsmc_inputs(7 + variable, variable=variable)
+    ref_v, ref_cp, ref_cl = oracle_propagate(d)
+    assert ref_cl > 0
+    g = {k: dev(d[k]) for k in ("off", "as_", "bs", "offs", "topts", "times", "wf_edge",
+                                 "wf_prev")}
+    W = int(d["w"][0])
+    off_ptr = ptr(g["off"]) if variable else None
+    # reference-shaped pair: in-place propagate, then calculateCP_HSMC
+    valid = dev(d["valid"])
+    cp = dev(np.ones(d["n_parent"] + d["n_new"], np.float32))
+    clamp = dev(np.zeros(2, np.uint64))
+    P.check("propagate", P.lib.pump_propagate_valid_particles(
+        d["n_new"], W, off_ptr, ptr(g["wf_edge"]), ptr(g["wf_prev"]), d["n_parent"],
+        ptr(g["as_"]), ptr(g["bs"]), None, ptr(g["times"]), ptr(g["topts"]), d["dt"],
+        d["max_hs"], ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(valid), d["n_p"],
+        ptr(clamp), None))
+    P.check("cp", P.lib.pump_calculate_cp_hsmc(d["n_new"], d["n_parent"], ptr(cp), ptr(valid),
+                                               d["n_p"], None))
+    assert np.array_equal(host(valid), ref_v)
+    assert np.array_equal(host(cp)[d["n_parent"]:], ref_cp[d["n_parent"]:])
+    assert int(host(clamp)[0]) == ref_cl
+    # fused propagate + CP with separate parent / child arrays
+    src = dev(d["valid"][: d["n_parent"]])
+    dst = dev(np.zeros((d["n_new"], d["n_p"]), np.uint8))
+    cp2 = dev(np.zeros(d["n_new"], np.float32))
+    P.check("propagate_cp", P.lib.pump_propagate_cp(
+        d["n_new"], W, off_ptr, ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
+        ptr(g["bs"]), ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"], ptr(g["offs"]),
+        d["T"], d["om"], d["n_mc"], ptr(src), ptr(dst), 0, d["n_p"], ptr(cp2), 0, None, None))
+    assert np.array_equal(host(dst), ref_v[d["n_parent"]:])
+    assert np.array_equal(host(cp2), ref_cp[d["n_parent"]:])
+    # bit masks
+    nw = (d["n_p"] + 63) // 64
+    bits_src = np.zeros((d["n_parent"], nw), np.uint64)
+    for r in range(d["n_parent"]):
+        packed = np.packbits(d["valid"][r].astype(np.uint8), bitorder="little")
+        packed = np.pad(packed, (0, nw * 8 - packed.size))
+        bits_src[r] = packed.view(np.uint64)
+    bsrc = dev(bits_src)
+    bdst = dev(np.zeros((d["n_new"], nw), np.uint64))
+    cp3 = dev(np.zeros(d["n_new"], np.float32))
+    P.check("propagate_cp_bits", P.lib.pump_propagate_cp_bits(
+        d["n_new"], W, off_ptr, ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
+        ptr(g["bs"]), ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"], ptr(g["offs"]),
+        d["T"], d["om"], d["n_mc"], ptr(bsrc), ptr(bdst), 0, d["n_p"], ptr(cp3), 0, 0, None,
+        None, None))
+    out = host(bdst)
+    for r in range(d["n_new"]):
+        unpacked = np.unpackbits(out[r].view(np.uint8), bitorder="little")[: d["n_p"]]
+        assert np.array_equal(unpacked, ref_v[d["n_parent"] + r])
+    assert np.array_equal(host(cp3), ref_cp[d["n_parent"]:])
+
+
+def test_gather_rows():
+    rng = np.random.default_rng(3)
+    src = rng.integers(0, 255, (40, 100), dtype=np.uint8)
+    rows = np.array([3, 7, 7, 0, 39], np.int32)
+    dst = np.zeros((12, 100), np.uint8)
+    gs, gr, gd = dev(src), dev(rows), dev(dst)
+    P.check("gather", P.lib.pump_gather_rows(ptr(gs), ptr(gr), 5, ptr(gd), 4, 100, None))
+    out = host(gd)
+    assert np.array_equal(out[4:9], src[rows]) and not out[:4].any() and not out[9:].any()
+
+
+def test_collision_probability_entry_points():
+    rng = np.random.default_rng(11)
+    n_obs = 15
+    obs = np.zeros((n_obs, 12), np.float32)
+    c = rng.uniform(0.2, 0.8, (n_obs, 3))
+    sd = rng.uniform(0.05, 0.2, (n_obs, 3))
+    obs[:, :3], obs[:, 6:9] = c - sd / 2, c + sd / 2
+    obs[:, 3:6], obs[:, 9:12] = -3.1, 3.1
+    n_mc, tsteps = 700, 300
+    xc = rng.normal(0, 0.05, (n_mc, tsteps, 3)).astype(np.float32)
+    n_paths, cap = 6, 260
+    paths = np.zeros((n_paths, cap, 6), np.float32)
+    Ts = rng.integers(3, cap, n_paths).astype(np.int32)
+    Ts[0] = 2  # no segment at all
+    Ts[1] = 130
+    for k in range(n_paths):
+        a, b = rng.uniform(0, 1, 3), rng.uniform(0, 1, 3)
+        lam = np.linspace(0, 1, cap)[:, None]
+        paths[k, :, :3] = a + lam * (b - a)
+    om = np.float32(1.0)
+    ref = [O.po_mccp_hits(oracle.fp(obs), n_obs, oracle.fp(xc), tsteps, n_mc, om,
+                          oracle.fp(np.ascontiguousarray(paths[k])), int(Ts[k]), None)
+           for k in range(n_paths)]
+    assert max(ref) > 0 and min(ref) == 0
+    gobs, gxc, gp, gT = dev(obs), dev(xc), dev(paths), dev(Ts)
+    hits = dev(np.full(n_paths, 77, np.int32))
+    P.check("mccp_many", P.lib.pump_collision_probability_many(
+        ptr(gobs), n_obs, ptr(gxc), tsteps, n_mc, om, ptr(gp), cap, ptr(gT), n_paths, ptr(hits), 1,
+        None))
+    assert host(hits).tolist() == ref
+    cp = ctypes.c_float(0)
+    gp1 = dev(paths[1])
+    P.check("mccp", P.lib.pump_collision_probability(ptr(gobs), n_obs, ptr(gxc), tsteps, n_mc, om,
+                                                     ptr(gp1), int(Ts[1]), ctypes.byref(cp), None))
+    assert np.float32(cp.value) == np.float32(np.float32(ref[1]) / np.float32(n_mc))
+
+
+def test_free_edges_and_discretize_entry_points():
+    cfg, s, o, ot, xc, rm, orc = build_pair(CASES[1])
+    E, n_wp = orc["n_edges"], orc["n_wp"]
+    gs, go = dev(s), dev(o)
+    gsrc, gdst = dev(orc["edge_src"]), dev(orc["edge_dst"])
+    gte, goff = dev(orc["topts_edge"]), dev(orc["wp_offsets"])
+    disc = dev(np.zeros((n_wp, 6), np.float32))
+    P.check("discretize", P.lib.pump_discretize_edges(ptr(gs), ptr(gsrc), ptr(gdst), ptr(gte), E,
+                                                      ptr(goff), 0, ptr(disc), None))
+    assert_close_f32(host(disc), orc["disc"])
+    gdisc = dev(orc["disc"])
+    flags = dev(np.full(E, 9, np.uint8))
+    P.check("free_edges", P.lib.pump_free_edges(ptr(go), o.shape[0], ptr(gs), None, 0, ptr(goff),
+                                                ptr(gdisc), ptr(flags), E, None))
+    assert np.array_equal(host(flags), orc["is_free_edges"])
+    sf = dev(np.zeros(s.shape[0], np.uint8))
+    P.check("sample_free", P.lib.pump_sample_free(ptr(go), o.shape[0], ptr(gs), s.shape[0],
+                                                  ptr(sf), None))
+    assert np.array_equal(host(sf), orc["is_free_samples"])
+    # the reference pair setupAsBs + cacheHalfspaces (no sentinel pass)
+    max_hs = orc["max_hs"]
+    hs = n_wp * max_hs
+    as_, bs = dev(np.zeros((hs, 3), np.float32)), dev(np.zeros(hs, np.float32))
+    cnt = dev(np.full(n_wp, -7, np.int32))
+    P.check("setup_as_bs", P.lib.pump_setup_as_bs(hs, ptr(bs), ptr(as_), None))
+    P.check("cache_hs", P.lib.pump_cache_halfspaces(
+        E, ptr(gdisc), ptr(dev(orc["is_free_edges"])), o.shape[0], max_hs, ptr(go), 0, ptr(goff),
+        ptr(as_), ptr(bs), ptr(cnt), orc["maxd2"], 0, None))
+    assert np.array_equal(host(cnt), orc["count_stored"])
+    assert np.array_equal(host(bs), orc["bs"][:hs])
+    assert np.array_equal(host(as_), orc["as_"][:hs])
+    rm.close()
+
+
+def test_setup_valid_particles():
+    v = dev(np.zeros((37, 101), np.uint8))
+    cp = dev(np.zeros(37, np.float32))
+    tt = dev(np.ones(37, np.float32))
+    P.check("setup_valid", P.lib.pump_setup_valid_particles(ptr(v), 101, 37, ptr(cp), ptr(tt),
+                                                            None))
+    assert host(v).all() and np.all(host(cp) == 1) and np.all(host(tt) == 0)
+    b = dev(np.zeros((5, 3), np.uint64))
+    P.check("setup_bits", P.lib.pump_setup_valid_bits(ptr(b), 150, 5, None))
+    hb = host(b)
+    assert np.all(hb[:, :2] == np.uint64(0xFFFFFFFFFFFFFFFF))
+    assert np.all(hb[:, 2] == np.uint64((1 << 22) - 1))
+
+
+def test_select_rn_order_statistic():
+    rng = np.random.default_rng(17)
+    n = 3_000_017
+    keys = rng.normal(10, 30, n).astype(np.float32)
+    keys[:1000] = -0.0
+    keys[1000:2000] = 0.0
+    keys[2000:2100] = keys[5000]  # ties
+    g = dev(keys)
+    ws = dev(np.zeros(P.lib.pump_select_rn_workspace(n), np.uint8))
+    srt = np.sort(keys)
+    for k in (0, 1, 999, 1500, n // 3, 2222222, n - 1):
+        rn = ctypes.c_float(0)
+        P.check("select", P.lib.pump_select_rn(ptr(g), n, k, ctypes.byref(rn), ptr(ws),
+                                               ws.numel(), None))
+        assert np.float32(rn.value) == srt[k] or (rn.value == 0 and srt[k] == 0)
+
+
+def test_polish_improves_arrival_time():
+    cfg, s, o, ot, xc, rm, orc = build_pair(CASES[0], polish=1)
+    t64 = rm.polished()
+    _, t32 = rm.pairs()
+    num = s.shape[0]
+    rng = np.random.default_rng(0)
+    idx = rng.integers(0, num * (num - 1), 3000)
+    e32 = e64 = 0.0
+    for q in idx:
+        i = q // (num - 1)
+        kk = q % (num - 1)
+        j = kk + (kk >= i)
+        ref = O.po64_topt(oracle.fp(s[i].copy()), oracle.fp(s[j].copy()), 5.0)
+        e32 = max(e32, abs(t32[q] - ref) / ref)
+        e64 = max(e64, abs(t64[q] - ref) / ref)
+    print("max rel err: bisection", e32, "polished", e64, flush=True)
+    assert e64 <= 1e-9 and e64 < e32
+    rm.close()
