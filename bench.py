@@ -1,0 +1,288 @@
+"""Benchmark of the planning hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
+
+Workload: the 1-GPU configuration (config 1: NUM=16000 samples, 30 boxes,
+~80 neighbours per sample, W_e = clamp(ceil(tau/0.05), 2, 64) waypoints per
+edge, 2000 particles, noise multiplier 0.03), synthetic seeded inputs.
+
+Setup (untimed): problem + noise tables, then the roadmap precompute on the
+device (all-pairs 2pBVP, rn selection, edge numbering, heuristic,
+sampleFree, waypoint offsets, fused edge pass, half-space cache).
+
+Step (timed K times, after W warmups): one edge pass -- kernel (b), waypoint
+generation fused with the segment/AABB test -- over every candidate edge.
+    value = edge-checks/s = E / (seconds per step)
+Also measured in the same run, each over its own timed repetitions:
+    * kernel (c): one HSMC propagate + CP pass in which every free edge is
+      expanded once from a synthetic parent path (particle-steps/s), in the
+      byte-validity form (plain FP32, the reference layout) and the bit-mask
+      form;
+    * kernel (a): the all-pairs 2pBVP solve (pair-solves/s);
+    * cpu_baseline: the CPU oracle's edge pass on a bounded sample of the
+      same edges with all host cores.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK = 8.0e12          # B/s, MI355X
+FP32_PEAK = 157.3e12       # FLOP/s, MI355X vector FP32
+METRIC = "edge-checks/s + MC particle-steps/s, batched split-BF16 vs plain FP32, 1 MI355X"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--num", type=int, default=0, help="override NUM (smaller runs)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--hsmc-reps", type=int, default=3)
+    ap.add_argument("--skip-hsmc", action="store_true")
+    return ap.parse_args()
+
+
+def edge_pass(P, d, n_obs, E, stream):
+    P.check("pump_edges_discretize_check", P.lib.pump_edges_discretize_check(
+        d.samples, d.edge_src, d.edge_dst, d.topts_edge, E, d.wp_offsets, 0, d.obstacles,
+        n_obs, d.disc, d.is_free_edges, stream.h))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the data-parallel "
+              "split across GPUs is described in DESIGN.md and not run here.", file=sys.stderr)
+        sys.exit(2)
+    import pump_amd as P
+    from pump_amd import configs
+
+    assert P.lib.pump_dev_count() >= 1, "no GPU visible"
+    log("device:", P.device_name())
+    over = {"num": args.num} if args.num else {}
+    cfg = configs.config(args.config, **over)
+    prm = cfg["params"]
+    t0 = time.perf_counter()
+    samples, obstacles = P.gen_problem(cfg["spec"])
+    ot, xc = P.make_noise(cfg["noise"])
+    log(f"problem + noise tables: {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    rm = P.Roadmap(samples, obstacles, samples[-1], ot, xc, prm)
+    info = rm.info
+    log(f"roadmap build {time.perf_counter() - t0:.1f} s: pairs {info.n_pairs} "
+        f"({info.ms_pairs:.1f} ms), E={info.n_edges}, waypoints={info.n_waypoints}, "
+        f"free={info.n_free_edges}, rn={info.rn:.4f}")
+    d = rm.device()
+    n_obs = int(obstacles.shape[0])
+    E = int(info.n_edges)
+    n_wp = int(info.n_waypoints)
+    stream = P.Stream()
+    timer = P.Timer(stream)
+
+    # ---------------- step: kernel (b) edge pass over all candidate edges --
+    for _ in range(args.warmup):
+        edge_pass(P, d, n_obs, E, stream)
+    stream.sync()
+    P.sync()
+    wall0 = time.perf_counter()
+    timer.start()
+    for _ in range(args.steps):
+        edge_pass(P, d, n_obs, E, stream)
+    ms_total = timer.stop()
+    P.sync()
+    wall = time.perf_counter() - wall0
+    ms_step = ms_total / args.steps
+    value = E / (ms_step / 1e3)
+    alg_bytes = 24 * n_wp + 33 * E          # waypoints + per-edge states/tau/index/flag
+    achieved = alg_bytes / (ms_step / 1e3)
+    log(f"edge pass: {ms_step:.3f} ms/step, {value:.3e} edge-checks/s, "
+        f"{achieved / 1e9:.1f} GB/s algorithmic, wall {wall:.3f} s for {args.steps} steps")
+
+    # ---------------- kernel (c): HSMC pass over every free edge -----------
+    hsmc = None
+    if not args.skip_hsmc:
+        hsmc = hsmc_pass(P, rm, d, prm, info, stream, args.hsmc_reps)
+
+    # ---------------- kernel (a): all-pairs 2pBVP --------------------------
+    pairs = {"pair_solves_per_s": info.n_pairs / (info.ms_pairs / 1e3),
+             "ms": info.ms_pairs, "n_pairs": int(info.n_pairs)}
+
+    # ---------------- CPU baseline (oracle, bounded sample) ----------------
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(rm, samples, obstacles, prm, args.cpu_seconds)
+
+    traffic = None
+    tpath = os.path.join(HERE, "profiles", "edge_pass_c1_traffic.json")
+    if os.path.exists(tpath) and not args.num and args.config == 1:
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "edge-checks/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (2pBVP cost/waypoints in f64 as the reference's host build)",
+        "data": "synthetic (seeded Philox draws)",
+        "config": {"workload": cfg["name"], "num": int(info.num), "n_obs": int(info.n_obs),
+                   "n_edges": E, "n_waypoints": n_wp, "n_free_edges": int(info.n_free_edges),
+                   "n_p": int(prm.n_p), "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                     "kernel": "k_edges<true,true> (pump_edges_discretize_check)",
+                     "alg_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+        "hsmc": hsmc,
+        "pairs": pairs,
+        "pipeline_ms": {k: getattr(info, k) for k in (
+            "ms_pairs", "ms_select", "ms_edges_build", "ms_nn", "ms_sample_free",
+            "ms_edge_pass", "ms_hs", "ms_total")},
+    }
+    print(json.dumps(line), flush=True)
+    rm.close()
+
+
+def hsmc_pass(P, rm, d, prm, info, stream, reps):
+    """Every free edge expanded once from one of 4096 synthetic parent paths
+    (times uniform in [0, 2] s, all particles valid)."""
+    n_p, n_mc, T = prm.n_p, info.n_mc, info.tmax_steps
+    fl = rm.flags()
+    free = np.nonzero(fl["is_free_edges"])[0].astype(np.int32)
+    n_new = int(free.shape[0])
+    cap = int(os.environ.get("PUMP_BENCH_MAX_NEW", "1000000"))
+    if n_new > cap:
+        free = free[:cap]
+        n_new = cap
+    n_par = 4096
+    rng = np.random.default_rng(0)
+    times = rng.uniform(0, 2.0, n_par).astype(np.float32)
+    prev = (np.arange(n_new) % n_par).astype(np.int32)
+    wo = fl["wp_offsets"]
+    W = (wo[free + 1] - wo[free]).astype(np.int64)
+    steps = int(n_p) * int(W.sum())
+    counts = rm.halfspaces()["count_stored"]
+    # stored half-spaces over the expanded waypoints: 3 MACs = 6 flops each
+    cs = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
+    hs_total = int((cs[wo[free + 1]] - cs[wo[free]]).sum())
+    flops = 6.0 * n_p * hs_total
+    g_edge, g_prev = P.DeviceArray(free), P.DeviceArray(prev)
+    g_times = P.DeviceArray(times)
+    g_cp = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
+    out = {"n_new_paths": n_new, "particle_steps": steps, "stored_halfspace_tests": n_p * hs_total}
+    timer = P.Timer(stream)
+    # plain: byte validity, FP32 (reference layout)
+    src = P.DeviceArray(shape=(n_par, n_p), dtype=np.uint8, fill=1)
+    dst = P.DeviceArray(shape=(n_new, n_p), dtype=np.uint8, fill=0)
+
+    def run_plain():
+        P.check("pump_propagate_cp", P.lib.pump_propagate_cp(
+            n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, g_times.p, d.topts_edge,
+            prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, src.p, dst.p, 0, n_p,
+            g_cp.p, 0, None, stream.h))
+
+    nw = (n_p + 63) // 64
+    bsrc = P.DeviceArray(shape=(n_par, nw), dtype=np.uint64, fill=0)
+    P.check("pump_setup_valid_bits", P.lib.pump_setup_valid_bits(bsrc.p, n_p, n_par, None))
+    bdst = P.DeviceArray(shape=(n_new, nw), dtype=np.uint64, fill=0)
+
+    def run_bits():
+        P.check("pump_propagate_cp_bits", P.lib.pump_propagate_cp_bits(
+            n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, g_times.p, d.topts_edge,
+            prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, bsrc.p, bdst.p, 0, n_p,
+            g_cp.p, 0, 0, None, None, stream.h))
+
+    for name, fn in (("plain_fp32_bytes", run_plain), ("fp32_bits", run_bits)):
+        fn()
+        stream.sync()
+        timer.start()
+        for _ in range(reps):
+            fn()
+        ms = timer.stop() / reps
+        out[name] = {"ms": ms, "particle_steps_per_s": steps / (ms / 1e3),
+                     "achieved_tflops": flops / (ms / 1e3) / 1e12,
+                     "frac_fp32_peak": flops / (ms / 1e3) / FP32_PEAK}
+        log(f"hsmc {name}: {ms:.3f} ms, {steps / (ms / 1e3):.3e} particle-steps/s")
+    # both forms must agree
+    cp_plain = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
+    P.check("pump_propagate_cp", P.lib.pump_propagate_cp(
+        n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, g_times.p, d.topts_edge,
+        prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, src.p, dst.p, 0, n_p,
+        cp_plain.p, 0, None, stream.h))
+    stream.sync()
+    run_bits()
+    stream.sync()
+    out["forms_agree"] = bool(np.array_equal(cp_plain.numpy(), g_cp.numpy()))
+    out["mean_cp"] = float(cp_plain.numpy().mean())
+    out["value"] = out["fp32_bits"]["particle_steps_per_s"]
+    out["unit"] = "particle-steps/s"
+    return out
+
+
+def cpu_baseline(rm, samples, obstacles, prm, seconds):
+    """The CPU oracle's edge pass (findDiscretizedPath + freeEdges) over the
+    first R candidate edges, OpenMP over all host cores; R is sized so that
+    the run takes about `seconds`."""
+    import oracle
+    from oracle import bp, fp, ip, lp
+    O = oracle.load()
+    e = rm.edges()
+    fl = rm.flags()
+    cores = len(os.sched_getaffinity(0))
+    obs = np.ascontiguousarray(obstacles, np.float32)
+    num = samples.shape[0]
+
+    def run(R):
+        # rows covering the first R edges
+        rs = e["row_start"]
+        last_row = int(np.searchsorted(rs, R, side="left"))
+        rs_cut = np.minimum(rs[: last_row + 1], R).astype(np.int64)
+        rs_cut = np.concatenate([rs_cut, np.full(num + 1 - rs_cut.shape[0], R, np.int64)])
+        wo = np.ascontiguousarray(fl["wp_offsets"][: R + 1])
+        disc = np.zeros((int(wo[-1]) + 1, 6), np.float32)
+        flags = np.zeros(R + 1, np.uint8)
+        dst = np.ascontiguousarray(e["edge_dst"][:R])
+        t0 = time.perf_counter()
+        O.po_discretize_edges(fp(samples), lp(rs_cut), ip(dst), num, lp(wo), fp(disc), cores)
+        O.po_free_edges(fp(obs), obs.shape[0], fp(disc), lp(wo), R, bp(flags), cores)
+        dt = time.perf_counter() - t0
+        agree = bool(np.array_equal(flags[:R], fl["is_free_edges"][:R]))
+        return dt, agree
+
+    E = int(rm.info.n_edges)
+    R = min(E, 20000)
+    dt, agree = run(R)
+    if dt < seconds * 0.5 and R < E:
+        R = int(min(E, R * max(1.0, seconds / max(dt, 1e-3))))
+        dt, agree = run(R)
+    log(f"cpu baseline: {R} edges in {dt:.2f} s on {cores} threads, flags agree: {agree}")
+    return {"value": R / dt, "unit": "edge-checks/s", "cores": cores, "kind": "port",
+            "sample": f"first {R} of {E} candidate edges (CPU oracle findDiscretizedPath + "
+                      f"freeEdges, OpenMP)", "seconds": dt, "flags_match_gpu": agree}
+
+
+if __name__ == "__main__":
+    main()

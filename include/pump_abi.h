@@ -49,6 +49,8 @@ enum pump_status {
 
 const char *pump_status_string(int status);
 int pump_abi_version(void);
+/* The HIP error and call site behind the last PUMP_E_HIP on this thread. */
+const char *pump_last_error(void);
 
 /* ---------------------------------------------------------------------------
  * Roadmap precompute ("roadMap/precomp")

@@ -17,6 +17,26 @@
 extern "C" {
 #endif
 
+/* ---- device memory, streams and timers ----------------------------------
+ * Minimal helpers so that a host program can drive the C ABI with only this
+ * library's HIP runtime in the process. */
+void *pump_dev_alloc(size_t bytes); /* NULL on failure (see pump_last_error) */
+int pump_dev_free(void *p);
+int pump_dev_upload(void *dst, const void *src, size_t bytes);
+int pump_dev_download(void *dst, const void *src, size_t bytes);
+int pump_dev_memset(void *dst, int value, size_t bytes);
+int pump_dev_sync(void);
+int pump_dev_count(void);
+int pump_dev_name(char *buf, int len);
+int pump_stream_create(pump_stream_t *s);
+int pump_stream_destroy(pump_stream_t s);
+int pump_stream_sync(pump_stream_t s);
+typedef struct pump_timer pump_timer; /* hipEvent pair on one stream */
+pump_timer *pump_timer_create(pump_stream_t s);
+int pump_timer_start(pump_timer *t);
+float pump_timer_stop(pump_timer *t); /* ms between start and stop, -1 on error */
+void pump_timer_destroy(pump_timer *t);
+
 /* ---- synthetic problems (counter-based Philox4x32-10 draws) ------------- */
 enum pump_obstacle_kind { PUMP_OBS_BOXES = 0, PUMP_OBS_WALLS = 1 };
 

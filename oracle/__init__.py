@@ -85,6 +85,7 @@ _SIGS = {
     "po_search": (c_int, [ctypes.POINTER(SearchIn), ctypes.POINTER(SearchOut)]),
     "po64_cost": (c_double, [c_double, f32p, f32p]),
     "po64_topt": (c_double, [f32p, f32p, c_double]),
+    "po64_topt_near": (c_double, [f32p, f32p, c_double, c_double]),
     "po64_mccp_hits": (c_int, [f32p, c_int, f32p, c_int, c_int, c_double, f64p, c_int]),
     "po64_find_optimal_path": (c_int, [c_double, f64p, f32p, c_int, c_int]),
 }

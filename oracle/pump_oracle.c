@@ -1097,6 +1097,33 @@ double po64_topt(const float *x0, const float *x1, double tmax)
     return 0.5 * (tl + tu);
 }
 
+/* The root of q nearest to t_guess (the FP64 stationary point at the same
+ * local minimum a bisection landed on): widen a bracket around t_guess until
+ * q changes sign from - to +, then bisect to the last bit. */
+double po64_topt_near(const float *x0, const float *x1, double t_guess, double tmax)
+{
+    double D, S, R;
+    po64_drs(x0, x1, &D, &S, &R);
+    if (t_guess >= tmax && po64_q(tmax, D, S, R) < 0)
+        return tmax;
+    double lo = t_guess, hi = t_guess;
+    for (int k = 0; k < 200; ++k) {
+        double w = t_guess * 1e-6 * (double)(1ull << (k < 60 ? k : 60));
+        lo = t_guess - w > 1e-9 ? t_guess - w : 1e-9;
+        hi = t_guess + w < tmax ? t_guess + w : tmax;
+        if (po64_q(lo, D, S, R) <= 0 && po64_q(hi, D, S, R) >= 0)
+            break;
+    }
+    for (int k = 0; k < 200 && hi - lo > 1e-15 * hi; ++k) {
+        double m = 0.5 * (lo + hi);
+        if (po64_q(m, D, S, R) > 0)
+            hi = m;
+        else
+            lo = m;
+    }
+    return 0.5 * (lo + hi);
+}
+
 static int po64_face(const double *v, const double *w, double lambda, int j, const float *lo,
                      const float *hi)
 {

@@ -147,6 +147,8 @@ double po64_cost(double tau, const float *x0, const float *x1);
 /* tau* by exact-arithmetic restatement of the bisection bracket in FP64 on
  * the quartic (1+g^2)tau^4 - 4R tau^2 + 24 S tau - 36 D = 0 */
 double po64_topt(const float *x0, const float *x1, double tmax);
+/* the FP64 root of the same quartic nearest to t_guess */
+double po64_topt_near(const float *x0, const float *x1, double t_guess, double tmax);
 int    po64_mccp_hits(const float *obstacles, int n_obs, const float *xcomb, int tmax_steps,
                       int n_mc, double offset_mult, const double *path, int T);
 /* path sampled in FP64 at the same dt schedule as findOptimalPath */

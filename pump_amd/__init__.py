@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libpump_amd.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
-        f"{LIB_PATH} is not built; run `python -m pump_amd.build` (hipcc, gfx950)")
+        f"{LIB_PATH} is not built; run `python pump_amd/build.py` (hipcc, gfx950)")
 
 lib = ctypes.CDLL(LIB_PATH)
 
@@ -42,7 +42,10 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "buffer 
 
 class PumpError(RuntimeError):
     def __init__(self, fn, code):
-        super().__init__(f"{fn} failed: {code} ({STATUS.get(code, 'unknown')})")
+        detail = ""
+        if code == -2:
+            detail = " -- " + lib.pump_last_error().decode(errors="replace")
+        super().__init__(f"{fn} failed: {code} ({STATUS.get(code, 'unknown')}){detail}")
         self.code = code
 
 
@@ -107,6 +110,7 @@ _SIGS = {
     # pump_abi.h
     "pump_status_string": (ctypes.c_char_p, [c_int]),
     "pump_abi_version": (c_int, []),
+    "pump_last_error": (ctypes.c_char_p, []),
     "pump_fill_copts_topts": (c_int, [S, c_int, c_float, S, S, S]),
     "pump_polish_topts": (c_int, [S, c_int, c_float, S, S, S]),
     "pump_select_rn_workspace": (c_size_t, [c_int64]),
@@ -162,12 +166,30 @@ _SIGS = {
     "pump_roadmap_device": (c_int, [c_void_p, ctypes.POINTER(RoadmapDev)]),
     "pump_search_run": (c_int, [c_void_p, ctypes.POINTER(Params), ctypes.POINTER(SearchResult)]),
     "pump_find_optimal_path": (c_int, [c_float, f32p, c_int, f32p, c_int, f32p]),
+    "pump_dev_alloc": (c_void_p, [c_size_t]),
+    "pump_dev_free": (c_int, [c_void_p]),
+    "pump_dev_upload": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "pump_dev_download": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "pump_dev_memset": (c_int, [c_void_p, c_int, c_size_t]),
+    "pump_dev_sync": (c_int, []),
+    "pump_dev_count": (c_int, []),
+    "pump_dev_name": (c_int, [ctypes.c_char_p, c_int]),
+    "pump_stream_create": (c_int, [ctypes.POINTER(c_void_p)]),
+    "pump_stream_destroy": (c_int, [c_void_p]),
+    "pump_stream_sync": (c_int, [c_void_p]),
+    "pump_timer_create": (c_void_p, [c_void_p]),
+    "pump_timer_start": (c_int, [c_void_p]),
+    "pump_timer_stop": (c_float, [c_void_p]),
+    "pump_timer_destroy": (None, [c_void_p]),
     "pump_host_topt": (c_float, [f32p, f32p, c_float]),
     "pump_host_cost": (c_float, [c_float, f32p, f32p]),
 }
 
 for _name, (_res, _args) in _SIGS.items():
-    _fn = getattr(lib, _name)
+    try:
+        _fn = getattr(lib, _name)
+    except AttributeError as exc:
+        raise ImportError(f"{LIB_PATH} lacks {_name}; rebuild with `python pump_amd/build.py`") from exc
     _fn.restype = _res
     _fn.argtypes = _args
 
@@ -346,6 +368,108 @@ class Roadmap:
         check("pump_search_run",
               lib.pump_search_run(self._h, ctypes.byref(prm or self.prm), ctypes.byref(r)))
         return r
+
+
+# ---------------------------------------------------------------------------
+# device buffers / streams / timers (this library's HIP runtime only)
+# ---------------------------------------------------------------------------
+class DeviceArray:
+    """A device allocation holding a copy of a numpy array's bytes."""
+
+    def __init__(self, host=None, shape=None, dtype=None, fill=None):
+        if host is not None:
+            host = np.ascontiguousarray(host)
+            shape, dtype = host.shape, host.dtype
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self.ptr = lib.pump_dev_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise PumpError("pump_dev_alloc", -4)
+        if host is not None and self.nbytes:
+            check("pump_dev_upload", lib.pump_dev_upload(self.ptr, host.ctypes.data, self.nbytes))
+        elif fill is not None and self.nbytes:
+            check("pump_dev_memset", lib.pump_dev_memset(self.ptr, int(fill), self.nbytes))
+
+    @property
+    def p(self):
+        return c_void_p(self.ptr)
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check("pump_dev_download", lib.pump_dev_download(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, self.dtype)
+        assert host.nbytes == self.nbytes
+        check("pump_dev_upload", lib.pump_dev_upload(self.ptr, host.ctypes.data, self.nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib.pump_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    check("pump_dev_name", lib.pump_dev_name(buf, 256))
+    return buf.value.decode()
+
+
+def sync():
+    check("pump_dev_sync", lib.pump_dev_sync())
+
+
+class Stream:
+    def __init__(self):
+        self.h = c_void_p()
+        check("pump_stream_create", lib.pump_stream_create(ctypes.byref(self.h)))
+
+    def sync(self):
+        check("pump_stream_sync", lib.pump_stream_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib.pump_stream_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Timer:
+    """hipEvent pair recorded on a stream (None = default stream)."""
+
+    def __init__(self, stream=None):
+        self.h = lib.pump_timer_create(stream.h if stream is not None else None)
+        if not self.h:
+            raise PumpError("pump_timer_create", -2)
+
+    def start(self):
+        check("pump_timer_start", lib.pump_timer_start(self.h))
+
+    def stop(self):
+        ms = lib.pump_timer_stop(self.h)
+        if ms < 0:
+            raise PumpError("pump_timer_stop", -2)
+        return ms
+
+    def __del__(self):
+        try:
+            lib.pump_timer_destroy(self.h)
+        except Exception:
+            pass
 
 
 def search_result_dict(r):

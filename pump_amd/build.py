@@ -1,6 +1,9 @@
 """Builds pump_amd/libpump_amd.so (all HIP sources, gfx950) in-tree.
 
-    python -m pump_amd.build            # or pump_amd.build.build()
+    python pump_amd/build.py
+
+(Run it by path: importing the pump_amd package loads the built library,
+which the build replaces.)
 
 Every source is compiled with hipcc for gfx950 with FP contraction off (the
 kernels reproduce the reference's uncontracted float/double arithmetic) and
@@ -26,6 +29,7 @@ SOURCES = [
     "mccp.hip",
     "host_gen.hip",
     "host_pipeline.hip",
+    "runtime.hip",
 ]
 
 ARCH = os.environ.get("PUMP_OFFLOAD_ARCH", "gfx950")

@@ -188,12 +188,25 @@ __global__ __launch_bounds__(256) void k_heuristic(const float *__restrict__ sam
     h[i] = cost_eval(c, tau);
 }
 
-// FP64 Newton polish on q(t) = t^4 c'(t) = (1+g^2)t^4 - 4R t^2 + 24 S t - 36 D.
+// FP64 polish of a bisected arrival time: the root of
+//   q(t) = t^4 c'(t) = (1+g^2) t^4 - 4R t^2 + 24 S t - 36 D
+// at the same local minimum.  A bracket [lo, hi] with q(lo) <= 0 <= q(hi) is
+// grown geometrically around the bisection result (the float finite
+// difference can stop anywhere within its noise, and at tmax it can read
+// negative while c'(tmax) > 0), then safeguarded Newton steps converge to the
+// last bit.  tmax is kept when c'(tmax) < 0 exactly.
+__device__ __forceinline__ double q_of(double t, double k4, double D, double S, double R)
+{
+    return ((k4 * t * t - 4.0 * R) * t + 24.0 * S) * t - 36.0 * D;
+}
+
 __global__ __launch_bounds__(256) void k_polish(const float *__restrict__ samples, int num,
                                                 float tmax, const float *__restrict__ topts,
                                                 double *__restrict__ out, int64_t n_pairs)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const double k4 = 1.0 + (double)kGravity * (double)kGravity;
+    const double tm = (double)tmax;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n_pairs;
          idx += stride) {
         int i, j;
@@ -209,17 +222,41 @@ __global__ __launch_bounds__(256) void k_polish(const float *__restrict__ sample
             S += dp * (v0 + v1);
             R += v0 * v0 + v0 * v1 + v1 * v1;
         }
-        const double k4 = 1.0 + (double)kGravity * (double)kGravity;
-        double t = (double)topts[idx];
-        if (t < (double)tmax) {
-            for (int it = 0; it < 4; ++it) {
-                double q = ((k4 * t * t - 4.0 * R) * t + 24.0 * S) * t - 36.0 * D;
+        const double t0 = (double)topts[idx];
+        double t = t0;
+        if (t0 >= tm && q_of(tm, k4, D, S, R) < 0.0) {
+            out[idx] = tm;
+            continue;
+        }
+        double lo = t0, hi = t0;
+        bool ok = false;
+        for (int k = 0; k < 64; ++k) {
+            double w = t0 * 1e-6 * (double)(1ull << (k < 60 ? k : 60));
+            lo = fmax(t0 - w, 1e-9);
+            hi = fmin(t0 + w, tm);
+            if (q_of(lo, k4, D, S, R) <= 0.0 && q_of(hi, k4, D, S, R) >= 0.0) {
+                ok = true;
+                break;
+            }
+        }
+        if (ok) {
+            t = fmin(fmax(t0, lo), hi);
+            for (int it = 0; it < 80; ++it) {
+                double q = q_of(t, k4, D, S, R);
+                if (q == 0.0)
+                    break;
+                if (q > 0.0)
+                    hi = t;
+                else
+                    lo = t;
                 double dq = (4.0 * k4 * t * t - 8.0 * R) * t + 24.0 * S;
-                if (dq <= 0.0)
-                    break;
                 double tn = t - q / dq;
-                if (!(tn > 0.0) || tn > (double)tmax)
+                if (!(tn > lo && tn < hi))
+                    tn = 0.5 * (lo + hi);
+                if (fabs(tn - t) <= 1e-16 * t) {
+                    t = tn;
                     break;
+                }
                 t = tn;
             }
         }
@@ -245,8 +282,12 @@ extern "C" int pump_fill_copts_topts(const float *d_samples, int num, float tmax
         return PUMP_OK;
     }
     unsigned long long *counter = nullptr;
-    PUMP_HIP(hipMallocAsync((void **)&counter, sizeof(unsigned long long), s));
-    PUMP_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned long long), s));
+    PUMP_HIP(hipMalloc((void **)&counter, sizeof(unsigned long long)));
+    hipError_t me = hipMemsetAsync(counter, 0, sizeof(unsigned long long), s);
+    if (me != hipSuccess) {
+        hipFree(counter);
+        PUMP_HIP(me);
+    }
     int dev = 0, cus = 256;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -258,9 +299,10 @@ extern "C" int pump_fill_copts_topts(const float *d_samples, int num, float tmax
     hipLaunchKernelGGL(k_fill_pairs_persistent, dim3((unsigned)blocks), dim3(256), 0, s,
                        d_samples, num, tmax, d_copts, d_topts, n_pairs, counter);
     hipError_t le = hipGetLastError();
-    PUMP_HIP(hipFreeAsync(counter, s));
-    if (le != hipSuccess)
-        return PUMP_E_HIP;
+    hipError_t se = hipStreamSynchronize(s);
+    hipFree(counter);
+    PUMP_HIP(le);
+    PUMP_HIP(se);
     return PUMP_OK;
 }
 

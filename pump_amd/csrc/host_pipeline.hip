@@ -12,6 +12,7 @@
 // CSR rows of the edge numbering and per-node lists with the same order:
 // out-edge nn of node i is edge row_start[i] + nn, destinations ascending.
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -124,8 +125,11 @@ struct EventTimer {
 
 #define TRYH(expr)                                                                                 \
     do {                                                                                           \
-        if ((expr) != hipSuccess)                                                                  \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            ::pump::note_hip_error(e_, __FILE__, __LINE__);                                        \
             return PUMP_E_HIP;                                                                     \
+        }                                                                                          \
     } while (0)
 
 int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
@@ -273,6 +277,21 @@ int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
 
 } // namespace
 
+namespace pump {
+static thread_local char g_last_error[256] = "";
+void note_hip_error(hipError_t e, const char *where, int line)
+{
+    const char *base = where;
+    for (const char *q = where; *q; ++q)
+        if (*q == '/')
+            base = q + 1;
+    snprintf(g_last_error, sizeof(g_last_error), "%s (%d) at %s:%d", hipGetErrorString(e), (int)e,
+             base, line);
+}
+} // namespace pump
+
+extern "C" const char *pump_last_error(void) { return pump::g_last_error; }
+
 extern "C" const char *pump_status_string(int status)
 {
     switch (status) {
@@ -333,7 +352,9 @@ extern "C" int pump_roadmap_build(const float *samples_h, int num, const float *
     rm->prm = *prm;
     std::memcpy(rm->goal, goal_h, sizeof(rm->goal));
     rm->samples_h.assign(samples_h, samples_h + (size_t)num * 6);
-    if (hipStreamCreate(&rm->stream) != hipSuccess) {
+    hipError_t ce = hipStreamCreate(&rm->stream);
+    if (ce != hipSuccess) {
+        note_hip_error(ce, __FILE__, __LINE__);
         delete rm;
         return PUMP_E_HIP;
     }
@@ -394,7 +415,8 @@ static int d2h(void *dst, const void *src, size_t bytes)
         return PUMP_OK;
     if (!src)
         return PUMP_E_INVALID;
-    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? PUMP_OK : PUMP_E_HIP;
+    PUMP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PUMP_OK;
 }
 
 extern "C" int pump_roadmap_copy_pairs(const pump_roadmap *rm, float *copts, float *topts)

@@ -104,7 +104,7 @@ extern "C" int pump_collision_probability(const float *d_obstacles, int n_obs,
         return PUMP_E_RANGE;
     hipStream_t s = as_stream(stream);
     int *scratch = nullptr; // [0] = T, [1] = hits
-    PUMP_HIP(hipMallocAsync((void **)&scratch, 2 * sizeof(int), s));
+    PUMP_HIP(hipMalloc((void **)&scratch, 2 * sizeof(int)));
     int hostT[2] = {T, 0};
     hipError_t ce = hipMemcpyAsync(scratch, hostT, 2 * sizeof(int), hipMemcpyHostToDevice, s);
     int r = PUMP_OK;
@@ -114,10 +114,11 @@ extern "C" int pump_collision_probability(const float *d_obstacles, int n_obs,
                                             scratch + 1, 0, stream);
     int hits = 0;
     hipError_t de = hipMemcpyAsync(&hits, scratch + 1, sizeof(int), hipMemcpyDeviceToHost, s);
-    hipError_t fe = hipFreeAsync(scratch, s);
     hipError_t se = hipStreamSynchronize(s);
-    if (ce != hipSuccess || de != hipSuccess || fe != hipSuccess || se != hipSuccess)
-        return PUMP_E_HIP;
+    hipFree(scratch);
+    PUMP_HIP(ce);
+    PUMP_HIP(de);
+    PUMP_HIP(se);
     if (r != PUMP_OK)
         return r;
     *cp_host = ((float)hits) / ((float)n_mc);

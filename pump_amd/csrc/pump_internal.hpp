@@ -10,20 +10,27 @@ namespace pump {
 
 inline hipStream_t as_stream(pump_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Records the failing HIP call for pump_last_error() (thread-local).
+void note_hip_error(hipError_t e, const char *where, int line);
+
 // Returns PUMP_E_HIP from the enclosing entry point on a HIP error.
 #define PUMP_HIP(expr)                                                                             \
     do {                                                                                           \
         hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess)                                                                      \
+        if (e_ != hipSuccess) {                                                                    \
+            ::pump::note_hip_error(e_, __FILE__, __LINE__);                                        \
             return PUMP_E_HIP;                                                                     \
+        }                                                                                          \
     } while (0)
 
 // Launch check: catches configuration errors of the preceding launch.
 #define PUMP_LAUNCHED()                                                                            \
     do {                                                                                           \
         hipError_t e_ = hipGetLastError();                                                         \
-        if (e_ != hipSuccess)                                                                      \
+        if (e_ != hipSuccess) {                                                                    \
+            ::pump::note_hip_error(e_, __FILE__, __LINE__);                                        \
             return PUMP_E_HIP;                                                                     \
+        }                                                                                          \
     } while (0)
 
 #define PUMP_REQUIRE(cond)                                                                         \

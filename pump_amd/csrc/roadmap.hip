@@ -524,7 +524,7 @@ extern "C" int pump_max_nn_size(const int64_t *d_row_start, const int *d_edge_ds
     PUMP_REQUIRE(n_edges == 0 || d_edge_dst);
     hipStream_t s = as_stream(stream);
     int *d_max = nullptr;
-    PUMP_HIP(hipMallocAsync((void **)&d_max, sizeof(int), s));
+    PUMP_HIP(hipMalloc((void **)&d_max, sizeof(int)));
     hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(num + 1, 256)), dim3(256), 0, s, d_in_degree,
                        (int64_t)num);
     hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, d_max, (int64_t)1);
@@ -536,10 +536,11 @@ extern "C" int pump_max_nn_size(const int64_t *d_row_start, const int *d_edge_ds
     hipError_t le = hipGetLastError();
     int m = 0;
     hipError_t ce = hipMemcpyAsync(&m, d_max, sizeof(int), hipMemcpyDeviceToHost, s);
-    hipError_t fe = hipFreeAsync(d_max, s);
     hipError_t se = hipStreamSynchronize(s);
-    if (le != hipSuccess || ce != hipSuccess || fe != hipSuccess || se != hipSuccess)
-        return PUMP_E_HIP;
+    hipFree(d_max);
+    PUMP_HIP(le);
+    PUMP_HIP(ce);
+    PUMP_HIP(se);
     *max_nn_host = m;
     return PUMP_OK;
 }
@@ -559,7 +560,7 @@ extern "C" int pump_nn_tables(const int64_t *d_row_start, const int *d_edge_src,
     }
     if (d_nn_come_edges && d_distances_come && tab > 0) {
         int *fill = nullptr;
-        PUMP_HIP(hipMallocAsync((void **)&fill, sizeof(int) * (size_t)num, s));
+        PUMP_HIP(hipMalloc((void **)&fill, sizeof(int) * (size_t)num));
         hipLaunchKernelGGL(k_zero_i32, dim3(grid_for(num, 256)), dim3(256), 0, s, fill,
                            (int64_t)num);
         hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(tab, 256)), dim3(256), 0, s,
@@ -575,9 +576,10 @@ extern "C" int pump_nn_tables(const int64_t *d_row_start, const int *d_edge_src,
                                d_nn_come_edges, d_distances_come);
         }
         hipError_t le = hipGetLastError();
-        PUMP_HIP(hipFreeAsync(fill, s));
-        if (le != hipSuccess)
-            return PUMP_E_HIP;
+        hipError_t se = hipStreamSynchronize(s);
+        hipFree(fill);
+        PUMP_HIP(le);
+        PUMP_HIP(se);
     }
     if (d_nn_idxs) {
         const int64_t nn2 = (int64_t)num * num;
