@@ -113,6 +113,13 @@ def main():
     log(f"edge pass: {ms_step:.3f} ms/step, {value:.3e} edge-checks/s, "
         f"{achieved / 1e9:.1f} GB/s algorithmic, wall {wall:.3f} s for {args.steps} steps")
 
+    # a plain device copy of known size: calibrates the HBM byte counters of a
+    # profiled run (FETCH_SIZE / WRITE_SIZE of the copy vs its 2 x 256 MiB)
+    calib = P.DeviceArray(shape=(2, 256 << 20), dtype=np.uint8, fill=7)
+    P.check("pump_dev_copy", P.lib.pump_dev_copy(calib.ptr + (256 << 20), calib.ptr, 256 << 20))
+    P.sync()
+    calib.free()
+
     # ---------------- kernel (c): HSMC pass over every free edge -----------
     hsmc = None
     if not args.skip_hsmc:

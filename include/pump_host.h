@@ -25,6 +25,7 @@ int pump_dev_free(void *p);
 int pump_dev_upload(void *dst, const void *src, size_t bytes);
 int pump_dev_download(void *dst, const void *src, size_t bytes);
 int pump_dev_memset(void *dst, int value, size_t bytes);
+int pump_dev_copy(void *dst, const void *src, size_t bytes); /* device to device */
 int pump_dev_sync(void);
 int pump_dev_count(void);
 int pump_dev_name(char *buf, int len);

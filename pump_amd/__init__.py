@@ -171,6 +171,7 @@ _SIGS = {
     "pump_dev_upload": (c_int, [c_void_p, c_void_p, c_size_t]),
     "pump_dev_download": (c_int, [c_void_p, c_void_p, c_size_t]),
     "pump_dev_memset": (c_int, [c_void_p, c_int, c_size_t]),
+    "pump_dev_copy": (c_int, [c_void_p, c_void_p, c_size_t]),
     "pump_dev_sync": (c_int, []),
     "pump_dev_count": (c_int, []),
     "pump_dev_name": (c_int, [ctypes.c_char_p, c_int]),

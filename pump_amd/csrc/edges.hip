@@ -23,6 +23,53 @@ namespace {
 constexpr int kEdgesPerBlock = 32;
 constexpr int kTile = 256;
 
+// Conservative position bounds of an edge: the waypoints lie on the cubic
+//   p(t) = p0 + v0 t - c2 t^2 + c3 t^3,  t in [0, tau]
+// (pathPoint's polynomial), so its extremes are at the endpoints or at zeros
+// of p'(t).  The bounds are inflated by a margin far above the rounding of a
+// waypoint (computed in double, rounded to float).  A box separated from
+// these bounds in some axis is separated from every segment's bounding box,
+// i.e. the reference's broad phase (collisionCheck.cu:104-112) rejects it for
+// every segment, so skipping it leaves every result unchanged.
+__device__ __forceinline__ void edge_bounds(const float *x0, const float *x1, float tau,
+                                            float *lo, float *hi)
+{
+    const double T = (double)tau;
+    for (int d = 0; d < 3; ++d) {
+        const double p0 = x0[d], v0 = x0[d + 3], p1 = x1[d], v1 = x1[d + 3];
+        const double c3 = (2.0 * p0 - 2.0 * p1 + (v0 + v1) * T) / (T * T * T);
+        const double c2 = (3.0 * p0 - 3.0 * p1 + (2.0 * v0 + v1) * T) / (T * T);
+        double mn = fmin(p0, p1), mx = fmax(p0, p1);
+        // p'(t) = v0 - 2 c2 t + 3 c3 t^2
+        const double A = 3.0 * c3, B = -2.0 * c2, C = v0;
+        double r[2];
+        int nr = 0;
+        if (fabs(A) > 1e-300) {
+            const double disc = B * B - 4.0 * A * C;
+            if (disc >= 0.0) {
+                const double sq = sqrt(disc);
+                r[nr++] = (-B - sq) / (2.0 * A);
+                r[nr++] = (-B + sq) / (2.0 * A);
+            }
+        } else if (fabs(B) > 1e-300) {
+            r[nr++] = -C / B;
+        }
+        for (int k = 0; k < nr; ++k) {
+            const double t = r[k];
+            if (t > 0.0 && t < T) {
+                const double v = p0 + v0 * t - c2 * t * t + c3 * t * t * t;
+                mn = fmin(mn, v);
+                mx = fmax(mx, v);
+            }
+        }
+        const double m = 1e-5 + 1e-5 * fmax(fabs(mn), fabs(mx));
+        lo[d] = (float)(mn - m);
+        hi[d] = (float)(mx + m);
+    }
+}
+
+constexpr int kMaskWords = kMaxObsLds / 64;
+
 template <bool kCheck, bool kWrite>
 __global__ __launch_bounds__(kTile) void k_edges(
     const float *__restrict__ samples, const int *__restrict__ edge_src,
@@ -33,6 +80,8 @@ __global__ __launch_bounds__(kTile) void k_edges(
     __shared__ ObsLds o;
     __shared__ float x0s[kEdgesPerBlock][kDim], x1s[kEdgesPerBlock][kDim];
     __shared__ float taus[kEdgesPerBlock];
+    __shared__ float eb_lo[kEdgesPerBlock][3], eb_hi[kEdgesPerBlock][3];
+    __shared__ unsigned long long cand[kEdgesPerBlock][kMaskWords];
     __shared__ int64_t off[kEdgesPerBlock + 1];
     __shared__ int ok[kEdgesPerBlock];
     __shared__ float wp[kTile][kDim];
@@ -40,8 +89,14 @@ __global__ __launch_bounds__(kTile) void k_edges(
     const int64_t e0 = (int64_t)blockIdx.x * kEdgesPerBlock;
     const int ne = (int)((n_edges - e0) < kEdgesPerBlock ? (n_edges - e0) : kEdgesPerBlock);
     const int t = threadIdx.x;
-    if (kCheck)
+    // boxes past the LDS capacity are tested for every segment (no mask)
+    const int n_masked = n_obs < kMaxObsLds ? n_obs : kMaxObsLds;
+    const int n_words = (n_masked + 63) >> 6;
+    if (kCheck) {
         stage_obstacles(o, obstacles, n_obs);
+        for (int q = t; q < ne * kMaskWords; q += kTile)
+            cand[q / kMaskWords][q % kMaskWords] = 0ull;
+    }
     for (int q = t; q < ne * kDim; q += kTile) {
         int le = q / kDim, d = q - kDim * (q / kDim);
         x0s[le][d] = samples[(int64_t)edge_src[e0 + le] * kDim + d];
@@ -54,6 +109,21 @@ __global__ __launch_bounds__(kTile) void k_edges(
         ok[q] = 1;
     }
     __syncthreads();
+    if (kCheck) {
+        for (int q = t; q < ne; q += kTile)
+            edge_bounds(x0s[q], x1s[q], taus[q], eb_lo[q], eb_hi[q]);
+        __syncthreads();
+        // candidate boxes per edge: not separated from the edge bounds
+        for (int q = t; q < ne * n_masked; q += kTile) {
+            const int le = q / n_masked, b = q - le * n_masked;
+            bool sep = false;
+            for (int d = 0; d < 3; ++d)
+                sep = sep || eb_hi[le][d] <= o.lo[d][b] || o.hi[d][b] <= eb_lo[le][d];
+            if (!sep)
+                atomicOr(&cand[le][b >> 6], 1ull << (b & 63));
+        }
+        __syncthreads();
+    }
 
     const int64_t g_begin = off[0], g_end = off[ne];
     for (int64_t base = g_begin; base < g_end; base += kTile - 1) {
@@ -78,7 +148,26 @@ __global__ __launch_bounds__(kTile) void k_edges(
         __syncthreads();
         bool test = (g < g_end) && (t < kTile - 1) && (k < W - 1);
         if (test && ok[le]) {
-            if (segment_blocked(wp[t], wp[t + 1], o, obstacles, n_obs))
+            const float *v = wp[t], *w = wp[t + 1];
+            float bmin[3], bmax[3];
+            seg_bbox(v, w, bmin, bmax);
+            bool hit = false;
+            for (int wd = 0; wd < n_words && !hit; ++wd) {
+                unsigned long long m = cand[le][wd];
+                while (m && !hit) {
+                    const int b = wd * 64 + __builtin_ctzll(m);
+                    m &= m - 1;
+                    float lo[3] = {o.lo[0][b], o.lo[1][b], o.lo[2][b]};
+                    float hi[3] = {o.hi[0][b], o.hi[1][b], o.hi[2][b]};
+                    hit = segment_hits_box(v, w, bmin, bmax, lo, hi);
+                }
+            }
+            for (int b = n_masked; b < n_obs && !hit; ++b) {
+                float lo[3], hi[3];
+                box_bounds(o, obstacles, b, lo, hi);
+                hit = segment_hits_box(v, w, bmin, bmax, lo, hi);
+            }
+            if (hit)
                 ok[le] = 0;
         }
         __syncthreads();

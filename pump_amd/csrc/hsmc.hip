@@ -17,6 +17,8 @@
 //   LHS = ((0 + o0*a0) + o1*a1) + o2*a2;  valid &= LHS*offsetMult < b
 // with no contraction, so they are bit-identical.  The AND over
 // (waypoint, half-space) pairs is order-independent.
+#include <cstdlib>
+
 #include "pump_device.hpp"
 #include "pump_internal.hpp"
 
@@ -352,6 +354,211 @@ __global__ __launch_bounds__(256) void k_propagate_bits(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-per-path form of propagate + CP.  One wave
+// owns one new path: a first pass ballots, per chunk of 64 waypoints, which
+// waypoints hold at least one stored half-space (b < 1000); the particles are
+// then streamed 64 at a time (lane = particle) and tested only against those
+// waypoints' half-spaces, whose data are wave-uniform loads.  A path whose
+// edge stores no half-space (the common case) is a straight copy of the
+// parent row plus a count.  No LDS, no workgroup barriers.
+struct PathInfo {
+    int64_t g0;
+    int W;
+    float dtl, tbase;
+};
+
+__device__ __forceinline__ PathInfo path_info(int e, int prev, const int64_t *wp_offsets,
+                                              int w_fixed, const float *topts,
+                                              const float *times)
+{
+    PathInfo pi;
+    pi.g0 = wp_offsets ? wp_offsets[e] : (int64_t)e * w_fixed;
+    pi.W = (int)((wp_offsets ? wp_offsets[e + 1] : pi.g0 + w_fixed) - pi.g0);
+    pi.dtl = topts[e] / (float)(pi.W - 1);
+    pi.tbase = times[prev];
+    return pi;
+}
+
+// masks[c] bit l: waypoint i0+c*64+l holds a stored half-space (i0..i0+255);
+// counts clamped time indices of those waypoints
+__device__ __forceinline__ uint64_t hs_masks(uint64_t *masks, const PathInfo &pi, int i0,
+                                             int max_hs, const float *__restrict__ bs, float dt,
+                                             int tmax_steps, unsigned long long *clamped)
+{
+    const int lane = threadIdx.x & 63;
+    uint64_t any = 0;
+    for (int c = 0; c < 4; ++c) {
+        const int i = i0 + c * 64 + lane;
+        bool has = false;
+        if (i < pi.W) {
+            const int64_t head = (pi.g0 + i) * (int64_t)max_hs;
+            for (int h = 0; h < max_hs; ++h)
+                has |= bs[head + h] < 1000.0f;
+            if (has && clamped) {
+                float time = pi.tbase + (float)i * pi.dtl;
+                int ti = (int)(time / dt);
+                if (ti < 0 || ti >= tmax_steps)
+                    atomicAdd(clamped, 1ull);
+            }
+        }
+        const uint64_t m = __ballot(has);
+        masks[c] = m;
+        any |= m;
+    }
+    return any;
+}
+
+// AND of one particle's tests over every stored half-space of the marked
+// waypoints (wave-uniform control flow over the masks)
+__device__ __forceinline__ bool particle_walk(bool v, int p, const uint64_t *masks, int i0,
+                                              const PathInfo &pi, int max_hs,
+                                              const float *__restrict__ as,
+                                              const float *__restrict__ bs,
+                                              const float *__restrict__ off, int n_mc,
+                                              int tmax_steps, float dt, float om)
+{
+    for (int c = 0; c < 4; ++c) {
+        uint64_t m = masks[c];
+        while (m) {
+            const int l = __builtin_ctzll(m);
+            m &= m - 1;
+            const int i = i0 + c * 64 + l;
+            float time = pi.tbase + (float)i * pi.dtl;
+            int ti = (int)(time / dt);
+            ti = ti < 0 ? 0 : (ti >= tmax_steps ? tmax_steps - 1 : ti);
+            const int64_t head = (pi.g0 + i) * (int64_t)max_hs;
+            const float *o = off + ((int64_t)ti * n_mc + p) * 3;
+            float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+            if (v) {
+                o0 = o[0];
+                o1 = o[1];
+                o2 = o[2];
+            }
+            for (int h = 0; h < max_hs; ++h) {
+                const float b = bs[head + h];
+                if (b >= 1000.0f)
+                    continue;
+                const float a0 = as[(head + h) * 3 + 0];
+                const float a1 = as[(head + h) * 3 + 1];
+                const float a2 = as[(head + h) * 3 + 2];
+                float lhs = 0.0f;
+                lhs = lhs + o0 * a0;
+                lhs = lhs + o1 * a1;
+                lhs = lhs + o2 * a2;
+                v = v && (lhs * om < b);
+            }
+        }
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum(int x)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        x += __shfl_down(x, off, 64);
+    return x;
+}
+
+// 0x01 in every byte of x that is nonzero, 0x00 elsewhere
+__device__ __forceinline__ uint32_t nonzero_bytes01(uint32_t x)
+{
+    return ((((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u) >> 7;
+}
+
+template <bool kBits>
+__global__ __launch_bounds__(256) void k_propagate_wave(
+    int n_new, const int64_t *__restrict__ wp_offsets, int w_fixed,
+    const int *__restrict__ wf_edge, const int *__restrict__ wf_prev,
+    const float *__restrict__ as, const float *__restrict__ bs, const float *__restrict__ times,
+    const float *__restrict__ topts, float dt, int max_hs, const float *__restrict__ offsets_time,
+    int tmax_steps, float om, int n_mc, const void *__restrict__ valid_src,
+    void *__restrict__ valid_dst, int64_t dst_base, int n_p, float *__restrict__ path_cp,
+    int64_t cp_base, unsigned long long *__restrict__ clamped)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t n_waves = (int64_t)gridDim.x * 4;
+    for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < n_new; w += n_waves) {
+        const int e = wf_edge[w];
+        const int prev = wf_prev[w];
+        const PathInfo pi = path_info(e, prev, wp_offsets, w_fixed, topts, times);
+        int count = 0; // valid particles (bits) or invalid particles (bytes), per lane
+        // waypoints in groups of 256; later groups continue from the child row
+        for (int i0 = 0; i0 < pi.W; i0 += 256) {
+            uint64_t masks[4];
+            const uint64_t any = hs_masks(masks, pi, i0, max_hs, bs, dt, tmax_steps, clamped);
+            const bool last = i0 + 256 >= pi.W;
+            if (kBits) {
+                const int nwords = (n_p + 63) >> 6;
+                const uint64_t *src =
+                    i0 == 0 ? (const uint64_t *)valid_src + (int64_t)prev * nwords
+                            : (const uint64_t *)valid_dst + (dst_base + w) * (int64_t)nwords;
+                uint64_t *dst = (uint64_t *)valid_dst + (dst_base + w) * (int64_t)nwords;
+                if (!any) {
+                    for (int wd = lane; wd < nwords; wd += 64) {
+                        const uint64_t x = src[wd];
+                        if (i0 == 0)
+                            dst[wd] = x;
+                        if (last)
+                            count += __popcll(x);
+                    }
+                } else {
+                    for (int wd = 0; wd < nwords; ++wd) {
+                        const uint64_t word = src[wd];
+                        const int p = wd * 64 + lane;
+                        bool v = (p < n_p) && ((word >> lane) & 1ull);
+                        if (__ballot(v) != 0ull)
+                            v = particle_walk(v, p, masks, i0, pi, max_hs, as, bs, offsets_time,
+                                              n_mc, tmax_steps, dt, om);
+                        const uint64_t out = __ballot(v);
+                        if (lane == 0) {
+                            dst[wd] = out;
+                            if (last)
+                                count += __popcll(out);
+                        }
+                    }
+                }
+            } else {
+                const uint8_t *src = i0 == 0
+                                         ? (const uint8_t *)valid_src + (int64_t)prev * n_p
+                                         : (const uint8_t *)valid_dst + (dst_base + w) * (int64_t)n_p;
+                uint8_t *dst = (uint8_t *)valid_dst + (dst_base + w) * (int64_t)n_p;
+                const bool words = ((n_p & 3) == 0) &&
+                                   ((((uintptr_t)src) | ((uintptr_t)dst)) & 3) == 0;
+                if (!any && words) {
+                    const uint32_t *s4 = (const uint32_t *)src;
+                    uint32_t *d4 = (uint32_t *)dst;
+                    const int nq = n_p >> 2;
+                    for (int q = lane; q < nq; q += 64) {
+                        const uint32_t x = nonzero_bytes01(s4[q]);
+                        d4[q] = x;
+                        if (last)
+                            count += 4 - __popc(x);
+                    }
+                } else {
+                    for (int p0 = 0; p0 < n_p; p0 += 64) {
+                        const int p = p0 + lane;
+                        bool v = (p < n_p) && src[p] != 0;
+                        if (any && __ballot(v) != 0ull)
+                            v = particle_walk(v, p, masks, i0, pi, max_hs, as, bs, offsets_time,
+                                              n_mc, tmax_steps, dt, om);
+                        if (p < n_p) {
+                            dst[p] = v ? 1 : 0;
+                            if (last)
+                                count += v ? 0 : 1;
+                        }
+                    }
+                }
+            }
+        }
+        count = wave_sum(count);
+        if (path_cp && lane == 0) {
+            const int invalid = kBits ? n_p - count : count;
+            path_cp[cp_base + w] = ((float)invalid) / (float)n_p;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_setup_bits(uint64_t *__restrict__ v, int64_t n_paths,
                                                     int n_p)
 {
@@ -429,6 +636,29 @@ extern "C" int pump_setup_valid_particles(uint8_t *d_valid, int n_p, int64_t max
     return PUMP_OK;
 }
 
+// Propagate kernel selection: the wave-per-path kernel (default) or the
+// workgroup-per-path kernel with an LDS list (PUMP_HSMC_KERNEL=block), which
+// suits edges that store many half-spaces.
+static bool use_block_kernel()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("PUMP_HSMC_KERNEL");
+        v = (e && e[0] == 'b') ? 1 : 0;
+    }
+    return v == 1;
+}
+
+static unsigned wave_grid(int n_new)
+{
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    int64_t blocks = ((int64_t)n_new + 3) / 4;
+    const int64_t cap = (int64_t)cus * 32;
+    return (unsigned)(blocks < cap ? (blocks > 0 ? blocks : 1) : cap);
+}
+
 static int check_prop_args(int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets,
                            int max_hs, int tmax_steps, int n_p, int n_mc)
 {
@@ -457,11 +687,19 @@ extern "C" int pump_propagate_valid_particles(
         return PUMP_OK;
     PUMP_REQUIRE(d_wf_edge && d_wf_prev && d_as && d_bs && d_times && d_topts && d_offsets_time &&
                  d_valid);
-    hipLaunchKernelGGL(k_propagate, dim3(num_new_paths), dim3(256), 0, as_stream(stream),
-                       num_new_paths, d_wp_offsets, waypoints_count, d_wf_edge, d_wf_prev, d_as,
-                       d_bs, d_times, d_topts, dt, max_hs, d_offsets_time, tmax_steps, offset_mult,
-                       num_mc_particles, d_valid, d_valid, path_base, n_p, (float *)nullptr,
-                       (int64_t)0, d_clamped);
+    if (use_block_kernel())
+        hipLaunchKernelGGL(k_propagate, dim3(num_new_paths), dim3(256), 0, as_stream(stream),
+                           num_new_paths, d_wp_offsets, waypoints_count, d_wf_edge, d_wf_prev,
+                           d_as, d_bs, d_times, d_topts, dt, max_hs, d_offsets_time, tmax_steps,
+                           offset_mult, num_mc_particles, d_valid, d_valid, path_base, n_p,
+                           (float *)nullptr, (int64_t)0, d_clamped);
+    else
+        hipLaunchKernelGGL(k_propagate_wave<false>, dim3(wave_grid(num_new_paths)), dim3(256), 0,
+                           as_stream(stream), num_new_paths, d_wp_offsets, waypoints_count,
+                           d_wf_edge, d_wf_prev, d_as, d_bs, d_times, d_topts, dt, max_hs,
+                           d_offsets_time, tmax_steps, offset_mult, num_mc_particles,
+                           (const void *)d_valid, (void *)d_valid, path_base, n_p,
+                           (float *)nullptr, (int64_t)0, d_clamped);
     PUMP_LAUNCHED();
     return PUMP_OK;
 }
@@ -497,11 +735,19 @@ extern "C" int pump_propagate_cp(int num_new_paths, int waypoints_count,
         return PUMP_OK;
     PUMP_REQUIRE(d_wf_edge && d_wf_prev && d_as && d_bs && d_times && d_topts && d_offsets_time &&
                  d_valid_src && d_valid_dst);
-    hipLaunchKernelGGL(k_propagate, dim3(num_new_paths), dim3(256), 0, as_stream(stream),
-                       num_new_paths, d_wp_offsets, waypoints_count, d_wf_edge, d_wf_prev, d_as,
-                       d_bs, d_times, d_topts, dt, max_hs, d_offsets_time, tmax_steps, offset_mult,
-                       num_mc_particles, d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp,
-                       cp_base, d_clamped);
+    if (use_block_kernel())
+        hipLaunchKernelGGL(k_propagate, dim3(num_new_paths), dim3(256), 0, as_stream(stream),
+                           num_new_paths, d_wp_offsets, waypoints_count, d_wf_edge, d_wf_prev,
+                           d_as, d_bs, d_times, d_topts, dt, max_hs, d_offsets_time, tmax_steps,
+                           offset_mult, num_mc_particles, d_valid_src, d_valid_dst, dst_base, n_p,
+                           d_path_cp, cp_base, d_clamped);
+    else
+        hipLaunchKernelGGL(k_propagate_wave<false>, dim3(wave_grid(num_new_paths)), dim3(256), 0,
+                           as_stream(stream), num_new_paths, d_wp_offsets, waypoints_count,
+                           d_wf_edge, d_wf_prev, d_as, d_bs, d_times, d_topts, dt, max_hs,
+                           d_offsets_time, tmax_steps, offset_mult, num_mc_particles,
+                           (const void *)d_valid_src, (void *)d_valid_dst, dst_base, n_p,
+                           d_path_cp, cp_base, d_clamped);
     PUMP_LAUNCHED();
     return PUMP_OK;
 }
@@ -558,11 +804,19 @@ extern "C" int pump_propagate_cp_bits(int num_new_paths, int waypoints_count,
                                    d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp, cp_base,
                                    d_band_count, d_clamped, as_stream(stream));
     PUMP_REQUIRE(precision == 0);
-    hipLaunchKernelGGL(k_propagate_bits, dim3(num_new_paths), dim3(256), 0, as_stream(stream),
-                       num_new_paths, d_wp_offsets, waypoints_count, d_wf_edge, d_wf_prev, d_as,
-                       d_bs, d_times, d_topts, dt, max_hs, d_offsets_time, tmax_steps, offset_mult,
-                       num_mc_particles, d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp,
-                       cp_base, d_clamped);
+    if (use_block_kernel())
+        hipLaunchKernelGGL(k_propagate_bits, dim3(num_new_paths), dim3(256), 0,
+                           as_stream(stream), num_new_paths, d_wp_offsets, waypoints_count,
+                           d_wf_edge, d_wf_prev, d_as, d_bs, d_times, d_topts, dt, max_hs,
+                           d_offsets_time, tmax_steps, offset_mult, num_mc_particles, d_valid_src,
+                           d_valid_dst, dst_base, n_p, d_path_cp, cp_base, d_clamped);
+    else
+        hipLaunchKernelGGL(k_propagate_wave<true>, dim3(wave_grid(num_new_paths)), dim3(256), 0,
+                           as_stream(stream), num_new_paths, d_wp_offsets, waypoints_count,
+                           d_wf_edge, d_wf_prev, d_as, d_bs, d_times, d_topts, dt, max_hs,
+                           d_offsets_time, tmax_steps, offset_mult, num_mc_particles,
+                           (const void *)d_valid_src, (void *)d_valid_dst, dst_base, n_p,
+                           d_path_cp, cp_base, d_clamped);
     PUMP_LAUNCHED();
     return PUMP_OK;
 }

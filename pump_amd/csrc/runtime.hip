@@ -43,6 +43,14 @@ extern "C" int pump_dev_download(void *dst, const void *src, size_t bytes)
     return PUMP_OK;
 }
 
+extern "C" int pump_dev_copy(void *dst, const void *src, size_t bytes)
+{
+    PUMP_REQUIRE(dst && src);
+    if (bytes)
+        PUMP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return PUMP_OK;
+}
+
 extern "C" int pump_dev_memset(void *dst, int value, size_t bytes)
 {
     PUMP_REQUIRE(dst);
