@@ -196,10 +196,20 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
     cs = np.concatenate([[0], np.cumsum(counts.astype(np.int64))])
     hs_total = int((cs[wo[free + 1]] - cs[wo[free]]).sum())
     flops = 6.0 * n_p * hs_total
+    # waypoints that hold a half-space, and the paths whose edge has one
+    marked = np.concatenate([[0], np.cumsum((counts > 0).astype(np.int64))])
+    marked_per_path = marked[wo[free + 1]] - marked[wo[free]]
+    n_marked = int(marked_per_path.sum())
     g_edge, g_prev = P.DeviceArray(free), P.DeviceArray(prev)
     g_times = P.DeviceArray(times)
     g_cp = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
-    out = {"n_new_paths": n_new, "particle_steps": steps, "stored_halfspace_tests": n_p * hs_total}
+    ws_bytes = int(P.lib.pump_propagate_workspace(n_new))
+    g_ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
+    out = {"n_new_paths": n_new, "particle_steps": steps, "stored_halfspace_tests": n_p * hs_total,
+           "paths_with_halfspaces": int((marked_per_path > 0).sum()),
+           "marked_waypoints": n_marked,
+           # offsets read by the half-space tests: 12 B per (particle, marked waypoint)
+           "offset_bytes": 12 * n_p * n_marked}
     timer = P.Timer(stream)
     # plain: byte validity, FP32 (reference layout)
     src = P.DeviceArray(shape=(n_par, n_p), dtype=np.uint8, fill=1)
@@ -207,9 +217,9 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
 
     def run_plain():
         P.check("pump_propagate_cp", P.lib.pump_propagate_cp(
-            n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, g_times.p, d.topts_edge,
-            prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, src.p, dst.p, 0, n_p,
-            g_cp.p, 0, None, stream.h))
+            n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, d.edge_hs_mask, g_times.p,
+            d.topts_edge, prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, src.p,
+            dst.p, 0, n_p, g_cp.p, 0, None, g_ws.p, ws_bytes, stream.h))
 
     nw = (n_p + 63) // 64
     bsrc = P.DeviceArray(shape=(n_par, nw), dtype=np.uint64, fill=0)
@@ -218,9 +228,9 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
 
     def run_bits():
         P.check("pump_propagate_cp_bits", P.lib.pump_propagate_cp_bits(
-            n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, g_times.p, d.topts_edge,
-            prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, bsrc.p, bdst.p, 0, n_p,
-            g_cp.p, 0, 0, None, None, stream.h))
+            n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, d.edge_hs_mask, g_times.p,
+            d.topts_edge, prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, bsrc.p,
+            bdst.p, 0, n_p, g_cp.p, 0, 0, None, None, g_ws.p, ws_bytes, stream.h))
 
     for name, fn in (("plain_fp32_bytes", run_plain), ("fp32_bits", run_bits)):
         fn()
@@ -236,9 +246,9 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
     # both forms must agree
     cp_plain = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
     P.check("pump_propagate_cp", P.lib.pump_propagate_cp(
-        n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, g_times.p, d.topts_edge,
-        prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, src.p, dst.p, 0, n_p,
-        cp_plain.p, 0, None, stream.h))
+        n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, d.edge_hs_mask, g_times.p,
+        d.topts_edge, prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, src.p,
+        dst.p, 0, n_p, cp_plain.p, 0, None, g_ws.p, ws_bytes, stream.h))
     stream.sync()
     run_bits()
     stream.sync()

@@ -164,13 +164,17 @@ float pump_maxd2(float offset_mult);
  * countStoredHSs of waypoints on blocked edges is written as 0 (the
  * reference leaves it unwritten).  write_sentinels != 0 additionally writes
  * the empty sentinel into every unused slot, which makes a preceding
- * pump_setup_as_bs unnecessary (same final contents). */
+ * pump_setup_as_bs unnecessary (same final contents).  d_edge_hs_mask, if
+ * non-NULL (n_edges words), receives per edge a waypoint summary: bit k
+ * (k < 63) set iff waypoint k stores a half-space with b not >= 1000, bit 63
+ * iff some waypoint >= 63 does.  It describes d_bs only when the unused
+ * slots hold the empty sentinel (pump_setup_as_bs or write_sentinels). */
 int pump_cache_halfspaces(int64_t n_edges, const float *d_disc_motions,
                           const uint8_t *d_is_free_edges, int n_obs, int max_hs,
                           const float *d_obstacles, int waypoints_count,
                           const int64_t *d_wp_offsets, float *d_as, float *d_bs,
                           int *d_count_stored, float maxd2, int write_sentinels,
-                          pump_stream_t stream);
+                          uint64_t *d_edge_hs_mask, pump_stream_t stream);
 
 /* setupValidParticles -- PUMP.cu:766-777, launched at mainPUMP.cu:663-668. */
 int pump_setup_valid_particles(uint8_t *d_valid, int n_p, int64_t max_path_count,
@@ -195,17 +199,24 @@ int pump_propagate_valid_particles(int num_new_paths, int waypoints_count,
 int pump_calculate_cp_hsmc(int num_new_paths, int64_t path_base, float *d_path_cp,
                            const uint8_t *d_valid, int n_p, pump_stream_t stream);
 
-/* Propagate + CP in one launch with separate parent/child validity arrays
- * (child rows dst_base+w of d_valid_dst).  Same results as the two calls
- * above; used by the search consumer so that compaction needs no aliasing
- * copies (PUMP.cu:236-237 issues one D2D memcpy per surviving path). */
+/* Propagate + CP with separate parent/child validity arrays (child rows
+ * dst_base+w of d_valid_dst).  Same results as the two calls above; used by
+ * the search consumer so that compaction needs no aliasing copies
+ * (PUMP.cu:236-237 issues one D2D memcpy per surviving path).  With a
+ * workspace of pump_propagate_workspace(num_new_paths) bytes, large
+ * wavefronts run in a batched form (classify / copy / half-space paths);
+ * pass NULL, 0 for a single launch.  d_edge_hs_mask: the per-edge waypoint
+ * summary written by pump_cache_halfspaces for these d_bs, or NULL (then
+ * every waypoint's slots are scanned, as the reference does). */
+size_t pump_propagate_workspace(int num_new_paths);
 int pump_propagate_cp(int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets,
                       const int *d_wf_edge, const int *d_wf_prev, const float *d_as,
-                      const float *d_bs, const float *d_times, const float *d_topts, float dt,
-                      int max_hs, const float *d_offsets_time, int tmax_steps, float offset_mult,
+                      const float *d_bs, const uint64_t *d_edge_hs_mask, const float *d_times,
+                      const float *d_topts, float dt, int max_hs, const float *d_offsets_time, int tmax_steps, float offset_mult,
                       int num_mc_particles, const uint8_t *d_valid_src, uint8_t *d_valid_dst,
                       int64_t dst_base, int n_p, float *d_path_cp, int64_t cp_base,
-                      unsigned long long *d_clamped, pump_stream_t stream);
+                      unsigned long long *d_clamped, void *d_workspace, size_t workspace_bytes,
+                      pump_stream_t stream);
 
 /* Compaction of surviving paths -- the per-path D2D copies of
  * PUMP.cu:236-237 as one gather: dst row dst_base+k = src row src_rows[k]. */
@@ -220,13 +231,13 @@ int pump_gather_rows(const uint8_t *d_src, const int *d_src_rows, int n_rows, ui
  * |offset.a*offsetMult - b| lies within the stated margin band. */
 int pump_propagate_cp_bits(int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets,
                            const int *d_wf_edge, const int *d_wf_prev, const float *d_as,
-                           const float *d_bs, const float *d_times, const float *d_topts, float dt,
-                           int max_hs, const float *d_offsets_time, int tmax_steps,
+                           const float *d_bs, const uint64_t *d_edge_hs_mask,
+                           const float *d_times, const float *d_topts, float dt, int max_hs, const float *d_offsets_time, int tmax_steps,
                            float offset_mult, int num_mc_particles,
                            const uint64_t *d_valid_src, uint64_t *d_valid_dst, int64_t dst_base,
                            int n_p, float *d_path_cp, int64_t cp_base, int precision,
                            unsigned long long *d_band_count, unsigned long long *d_clamped,
-                           pump_stream_t stream);
+                           void *d_workspace, size_t workspace_bytes, pump_stream_t stream);
 
 /* Initial bit masks for pump_propagate_cp_bits: all n_p particles valid in
  * each of n_paths rows (the bit form of setupValidParticles). */

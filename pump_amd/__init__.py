@@ -91,7 +91,7 @@ class RoadmapDev(ctypes.Structure):
     _fields_ = [(n, c_void_p) for n in (
         "samples", "obstacles", "copts", "topts", "copts_edge", "topts_edge", "h", "disc", "as_",
         "bs", "offsets_time", "xcomb", "row_start", "wp_offsets", "edge_src", "edge_dst",
-        "count_stored", "is_free_edges", "is_free_samples")]
+        "count_stored", "is_free_edges", "is_free_samples", "edge_hs_mask")]
 
 
 class SearchResult(ctypes.Structure):
@@ -130,18 +130,20 @@ _SIGS = {
     "pump_setup_as_bs": (c_int, [c_int64, S, S, S]),
     "pump_maxd2": (c_float, [c_float]),
     "pump_cache_halfspaces": (c_int, [c_int64, S, S, c_int, c_int, S, c_int, S, S, S, S, c_float,
-                                      c_int, S]),
+                                      c_int, S, S]),
     "pump_setup_valid_particles": (c_int, [S, c_int, c_int64, S, S, S]),
     "pump_propagate_valid_particles": (c_int, [c_int, c_int, S, S, S, c_int64, S, S, S, S, S,
                                                c_float, c_int, S, c_int, c_float, c_int, S, c_int,
                                                S, S]),
     "pump_calculate_cp_hsmc": (c_int, [c_int, c_int64, S, S, c_int, S]),
-    "pump_propagate_cp": (c_int, [c_int, c_int, S, S, S, S, S, S, S, c_float, c_int, S, c_int,
-                                  c_float, c_int, S, S, c_int64, c_int, S, c_int64, S, S]),
+    "pump_propagate_workspace": (c_size_t, [c_int]),
+    "pump_propagate_cp": (c_int, [c_int, c_int, S, S, S, S, S, S, S, S, c_float, c_int, S, c_int,
+                                  c_float, c_int, S, S, c_int64, c_int, S, c_int64, S, S, c_size_t,
+                                  S]),
     "pump_gather_rows": (c_int, [S, S, c_int, S, c_int64, c_int64, S]),
-    "pump_propagate_cp_bits": (c_int, [c_int, c_int, S, S, S, S, S, S, S, c_float, c_int, S,
+    "pump_propagate_cp_bits": (c_int, [c_int, c_int, S, S, S, S, S, S, S, S, c_float, c_int, S,
                                        c_int, c_float, c_int, S, S, c_int64, c_int, S, c_int64,
-                                       c_int, S, S, S]),
+                                       c_int, S, S, S, c_size_t, S]),
     "pump_setup_valid_bits": (c_int, [S, c_int, c_int64, S]),
     "pump_collision_probability": (c_int, [S, c_int, S, c_int, c_int, c_float, S, c_int, f32p, S]),
     "pump_collision_probability_many": (c_int, [S, c_int, S, c_int, c_int, c_float, S, c_int64, S,

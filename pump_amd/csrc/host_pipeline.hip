@@ -50,6 +50,7 @@ struct pump_roadmap {
     uint8_t *is_free_edges = nullptr, *is_free_samples = nullptr;
     float *as = nullptr, *bs = nullptr;
     int *count = nullptr;
+    uint64_t *edge_hs_mask = nullptr;
     int max_hs = 0;
     float maxd2 = 0;
     float *offsets_time = nullptr, *xcomb = nullptr;
@@ -238,13 +239,14 @@ int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
     TRY(dalloc(&rm->as, hs * 3));
     TRY(dalloc(&rm->bs, hs));
     TRY(dalloc(&rm->count, (size_t)rm->n_wp));
+    TRY(dalloc(&rm->edge_hs_mask, (size_t)rm->n_edges));
     tm.start();
     if (rm->max_hs == 0) {
         TRY(pump_setup_as_bs((int64_t)hs, rm->bs, rm->as, s));
     }
     TRY(pump_cache_halfspaces(rm->n_edges, rm->disc, rm->is_free_edges, rm->n_obs, rm->max_hs,
                               rm->obstacles, 0, rm->wp_offsets, rm->as, rm->bs, rm->count,
-                              rm->maxd2, 1, s));
+                              rm->maxd2, 1, rm->edge_hs_mask, s));
     rm->info.ms_hs = tm.stop();
     TRYH(hipStreamSynchronize(s));
 
@@ -401,6 +403,7 @@ extern "C" void pump_roadmap_free(pump_roadmap *rm)
     dfree(rm->as);
     dfree(rm->bs);
     dfree(rm->count);
+    dfree(rm->edge_hs_mask);
     dfree(rm->offsets_time);
     dfree(rm->xcomb);
     dfree(rm->ws);
@@ -541,6 +544,7 @@ extern "C" int pump_roadmap_device(const pump_roadmap *rm, pump_roadmap_dev *d)
     d->count_stored = rm->count;
     d->is_free_edges = rm->is_free_edges;
     d->is_free_samples = rm->is_free_samples;
+    d->edge_hs_mask = rm->edge_hs_mask;
     return PUMP_OK;
 }
 
@@ -661,7 +665,7 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
 
     // device search state: validity rows of stored paths, their times, and
     // staging for the current wavefront
-    DevBuf valid, stage, d_time, d_cp, d_prev, d_edge, d_rows, d_clamp, d_band;
+    DevBuf valid, stage, d_time, d_cp, d_prev, d_edge, d_rows, d_clamp, d_band, d_ws;
     TRY(valid.ensure((size_t)maxp * row_bytes));
     TRY(d_time.ensure(sizeof(float) * (size_t)maxp));
     TRY(d_clamp.ensure(2 * sizeof(unsigned long long)));
@@ -747,6 +751,7 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
 
         // HSMC step on the device (PUMP.cu:80-160)
         TRY(stage.ensure((size_t)n_new * row_bytes));
+        TRY(d_ws.ensure(pump_propagate_workspace(n_new)));
         TRY(d_cp.ensure(sizeof(float) * n_new));
         TRY(d_prev.ensure(sizeof(int) * n_new));
         TRY(d_edge.ensure(sizeof(int) * n_new));
@@ -759,17 +764,19 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
         if (bits) {
             TRY(pump_propagate_cp_bits(
                 n_new, 0, rm->wp_offsets, (const int *)d_edge.p, (const int *)d_prev.p, rm->as,
-                rm->bs, (const float *)d_time.p, rm->topts_edge, P.dt, rm->max_hs,
+                rm->bs, rm->edge_hs_mask, (const float *)d_time.p, rm->topts_edge, P.dt,
+                rm->max_hs,
                 rm->offsets_time, rm->tmax_steps, P.offset_mult, rm->n_mc,
                 (const uint64_t *)valid.p, (uint64_t *)stage.p, 0, n_p, (float *)d_cp.p, 0,
-                P.precision, clamp + 1, clamp, s));
+                P.precision, clamp + 1, clamp, d_ws.p, d_ws.bytes, s));
         } else {
             TRY(pump_propagate_cp(n_new, 0, rm->wp_offsets, (const int *)d_edge.p,
-                                  (const int *)d_prev.p, rm->as, rm->bs, (const float *)d_time.p,
+                                  (const int *)d_prev.p, rm->as, rm->bs, rm->edge_hs_mask,
+                                  (const float *)d_time.p,
                                   rm->topts_edge, P.dt, rm->max_hs, rm->offsets_time,
                                   rm->tmax_steps, P.offset_mult, rm->n_mc,
                                   (const uint8_t *)valid.p, (uint8_t *)stage.p, 0, n_p,
-                                  (float *)d_cp.p, 0, clamp, s));
+                                  (float *)d_cp.p, 0, clamp, d_ws.p, d_ws.bytes, s));
         }
         ms_prop += tm.stop();
         TRYH(hipMemcpyAsync(&path_cp[num_paths], d_cp.p, sizeof(float) * n_new,

@@ -49,25 +49,29 @@ __global__ __launch_bounds__(kHsBlock) void k_cache_hs(
     int64_t n_edges, const float *__restrict__ disc, const int64_t *__restrict__ wp_offsets,
     int w_fixed, const uint8_t *__restrict__ is_free, int n_obs, int max_hs,
     const float *__restrict__ obstacles, float *__restrict__ as, float *__restrict__ bs,
-    int *__restrict__ count_stored, float maxd2)
+    int *__restrict__ count_stored, float maxd2, uint64_t *__restrict__ edge_mask)
 {
     __shared__ ObsLds o;
     __shared__ int64_t off[kEdgesPerBlock + 1];
     __shared__ uint8_t fr[kEdgesPerBlock];
+    __shared__ unsigned long long em[kEdgesPerBlock];
     const int64_t e0 = (int64_t)blockIdx.x * kEdgesPerBlock;
     const int ne = (int)((n_edges - e0) < kEdgesPerBlock ? (n_edges - e0) : kEdgesPerBlock);
     const int t = threadIdx.x;
     stage_obstacles(o, obstacles, n_obs);
     for (int q = t; q <= ne; q += kHsBlock)
         off[q] = wp_offsets ? wp_offsets[e0 + q] : (e0 + q) * (int64_t)w_fixed;
-    for (int q = t; q < ne; q += kHsBlock)
+    for (int q = t; q < ne; q += kHsBlock) {
         fr[q] = is_free[e0 + q];
+        em[q] = 0ull;
+    }
     __syncthreads();
     const int64_t g_begin = off[0], g_end = off[ne];
     for (int64_t g = g_begin + t; g < g_end; g += kHsBlock) {
         const int le = find_edge(off, ne, g);
         int64_t hs = g * max_hs;
         int stored = 0;
+        bool live = false; // some stored b passes the sentinel test of PUMP.cu:642
         if (fr[le]) {
             float pos[3], vel[3];
             for (int d = 0; d < 3; ++d) {
@@ -102,11 +106,16 @@ __global__ __launch_bounds__(kHsBlock) void k_cache_hs(
                     xx = xx + x[d] * x[d];
                 }
                 bs[hs] = xx;
+                live = live || !(xx >= 1000.0f);
                 ++stored;
                 ++hs;
             }
         }
         count_stored[g] = stored;
+        if (live && edge_mask) {
+            const int k = (int)(g - off[le]);
+            atomicOr(&em[le], 1ull << (k < 63 ? k : 63));
+        }
         if (kSentinels) {
             for (int h = stored; h < max_hs; ++h, ++hs) {
                 as[hs * 3 + 0] = -1.0f;
@@ -115,6 +124,11 @@ __global__ __launch_bounds__(kHsBlock) void k_cache_hs(
                 bs[hs] = 1111.0f;
             }
         }
+    }
+    if (edge_mask) {
+        __syncthreads();
+        for (int q = t; q < ne; q += kHsBlock)
+            edge_mask[e0 + q] = em[q];
     }
 }
 
@@ -355,13 +369,13 @@ __global__ __launch_bounds__(256) void k_propagate_bits(
 }
 
 // ---------------------------------------------------------------------------
-// Wave-per-path form of propagate + CP.  One wave
-// owns one new path: a first pass ballots, per chunk of 64 waypoints, which
-// waypoints hold at least one stored half-space (b < 1000); the particles are
-// then streamed 64 at a time (lane = particle) and tested only against those
-// waypoints' half-spaces, whose data are wave-uniform loads.  A path whose
-// edge stores no half-space (the common case) is a straight copy of the
-// parent row plus a count.  No LDS, no workgroup barriers.
+// Wave-level forms of propagate + CP.  A wave first marks, per group of 64
+// waypoints, the waypoints that hold at least one stored half-space (b <
+// 1000): from the edge's waypoint mask when one is given (bit k: waypoint k,
+// bit 63: some waypoint >= 63), else by ballots over bs.  Particles are then
+// streamed 64 at a time (lane = particle) and tested only against those
+// waypoints' half-spaces, whose data are wave-uniform loads.  Decisions use
+// the reference's float order.  No LDS, no workgroup barriers.
 struct PathInfo {
     int64_t g0;
     int W;
@@ -380,27 +394,64 @@ __device__ __forceinline__ PathInfo path_info(int e, int prev, const int64_t *wp
     return pi;
 }
 
-// masks[c] bit l: waypoint i0+c*64+l holds a stored half-space (i0..i0+255);
-// counts clamped time indices of those waypoints
+__device__ __forceinline__ bool time_clamped(const PathInfo &pi, int i, float dt, int tmax_steps)
+{
+    float time = pi.tbase + (float)i * pi.dtl;
+    int ti = (int)(time / dt);
+    return ti < 0 || ti >= tmax_steps;
+}
+
+// An edge's waypoint mask (bit k: waypoint k holds a stored half-space, bit
+// 63: waypoint 63 or a later one does) is exact when the edge has at most 64
+// waypoints or bit 63 is clear.
+__device__ __forceinline__ bool mask_exact(uint64_t emask, int W)
+{
+    return W <= 64 || !(emask >> 63);
+}
+
+// the summary's bits that name waypoints of a W-waypoint edge
+__device__ __forceinline__ uint64_t mask_bits(uint64_t emask, int W)
+{
+    return W >= 64 ? emask : emask & ((1ull << W) - 1ull);
+}
+
+// A stored half-space: the reference skips entries with b >= 1000
+// (PUMP.cu:642), so "holds a half-space" means some b that is not >= 1000.
+__device__ __forceinline__ bool waypoint_has_hs(const float *__restrict__ bs, int64_t head,
+                                                int max_hs)
+{
+    bool has = false;
+    for (int h = 0; h < max_hs; ++h)
+        has |= !(bs[head + h] >= 1000.0f);
+    return has;
+}
+
+// masks[c] bit l: waypoint i0+c*64+l holds a stored half-space (i0..i0+255),
+// from the edge's waypoint mask when it is known and exact, else by ballots
+// over bs.  Counts clamped time indices of marked waypoints into *clamped
+// when non-null.
 __device__ __forceinline__ uint64_t hs_masks(uint64_t *masks, const PathInfo &pi, int i0,
-                                             int max_hs, const float *__restrict__ bs, float dt,
+                                             bool known, uint64_t emask, int max_hs,
+                                             const float *__restrict__ bs, float dt,
                                              int tmax_steps, unsigned long long *clamped)
 {
     const int lane = threadIdx.x & 63;
+    if (known && mask_exact(emask, pi.W)) {
+        const uint64_t m = i0 == 0 ? mask_bits(emask, pi.W) : 0ull;
+        masks[0] = m;
+        masks[1] = masks[2] = masks[3] = 0;
+        if (clamped && ((m >> lane) & 1ull) && time_clamped(pi, lane, dt, tmax_steps))
+            atomicAdd(clamped, 1ull);
+        return m;
+    }
     uint64_t any = 0;
     for (int c = 0; c < 4; ++c) {
         const int i = i0 + c * 64 + lane;
         bool has = false;
         if (i < pi.W) {
-            const int64_t head = (pi.g0 + i) * (int64_t)max_hs;
-            for (int h = 0; h < max_hs; ++h)
-                has |= bs[head + h] < 1000.0f;
-            if (has && clamped) {
-                float time = pi.tbase + (float)i * pi.dtl;
-                int ti = (int)(time / dt);
-                if (ti < 0 || ti >= tmax_steps)
-                    atomicAdd(clamped, 1ull);
-            }
+            has = waypoint_has_hs(bs, (pi.g0 + i) * (int64_t)max_hs, max_hs);
+            if (has && clamped && time_clamped(pi, i, dt, tmax_steps))
+                atomicAdd(clamped, 1ull);
         }
         const uint64_t m = __ballot(has);
         masks[c] = m;
@@ -409,15 +460,28 @@ __device__ __forceinline__ uint64_t hs_masks(uint64_t *masks, const PathInfo &pi
     return any;
 }
 
-// AND of one particle's tests over every stored half-space of the marked
-// waypoints (wave-uniform control flow over the masks)
-__device__ __forceinline__ bool particle_walk(bool v, int p, const uint64_t *masks, int i0,
-                                              const PathInfo &pi, int max_hs,
-                                              const float *__restrict__ as,
+__device__ __forceinline__ float lane_f(float x, int s)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), s));
+}
+
+// Tests NQ particle groups (lane = particle of each group) against every
+// stored half-space of the marked waypoints; v[q] becomes the AND of the
+// particle's tests (PUMP.cu:636-656, same float order).  Per waypoint, the
+// slots are loaded lane-parallel (lane h: slot h), the live ones found by a
+// ballot and broadcast by readlane, and the offsets of the NQ particles are
+// loaded together: one round of memory latency per marked waypoint.
+template <int NQ>
+__device__ __forceinline__ void particle_walk(bool (&v)[NQ], const int (&p)[NQ], int n_p,
+                                              const uint64_t *masks, int i0, const PathInfo &pi,
+                                              int max_hs, const float *__restrict__ as,
                                               const float *__restrict__ bs,
                                               const float *__restrict__ off, int n_mc,
                                               int tmax_steps, float dt, float om)
 {
+    const int lane = threadIdx.x & 63;
+    float o[NQ][3];
+    int last_ti = -1; // consecutive waypoints often share a time index
     for (int c = 0; c < 4; ++c) {
         uint64_t m = masks[c];
         while (m) {
@@ -428,29 +492,46 @@ __device__ __forceinline__ bool particle_walk(bool v, int p, const uint64_t *mas
             int ti = (int)(time / dt);
             ti = ti < 0 ? 0 : (ti >= tmax_steps ? tmax_steps - 1 : ti);
             const int64_t head = (pi.g0 + i) * (int64_t)max_hs;
-            const float *o = off + ((int64_t)ti * n_mc + p) * 3;
-            float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
-            if (v) {
-                o0 = o[0];
-                o1 = o[1];
-                o2 = o[2];
+            if (ti != last_ti) {
+                last_ti = ti;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    // p < n_p <= n_mc: in bounds whatever the particle's validity
+                    const float *op = off + ((int64_t)ti * n_mc + p[q]) * 3;
+                    const bool in = p[q] < n_p;
+                    o[q][0] = in ? op[0] : 0.0f;
+                    o[q][1] = in ? op[1] : 0.0f;
+                    o[q][2] = in ? op[2] : 0.0f;
+                }
             }
-            for (int h = 0; h < max_hs; ++h) {
-                const float b = bs[head + h];
-                if (b >= 1000.0f)
-                    continue;
-                const float a0 = as[(head + h) * 3 + 0];
-                const float a1 = as[(head + h) * 3 + 1];
-                const float a2 = as[(head + h) * 3 + 2];
-                float lhs = 0.0f;
-                lhs = lhs + o0 * a0;
-                lhs = lhs + o1 * a1;
-                lhs = lhs + o2 * a2;
-                v = v && (lhs * om < b);
+            for (int h0 = 0; h0 < max_hs; h0 += 64) {
+                const int h = h0 + lane;
+                float b = 1111.0f, a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+                if (h < max_hs) {
+                    b = bs[head + h];
+                    a0 = as[(head + h) * 3 + 0];
+                    a1 = as[(head + h) * 3 + 1];
+                    a2 = as[(head + h) * 3 + 2];
+                }
+                // the reference skips slots with b >= 1000 (PUMP.cu:642)
+                uint64_t sm = __ballot(!(b >= 1000.0f));
+                while (sm) {
+                    const int sl = __builtin_ctzll(sm);
+                    sm &= sm - 1;
+                    const float B = lane_f(b, sl), A0 = lane_f(a0, sl), A1 = lane_f(a1, sl),
+                                A2 = lane_f(a2, sl);
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        float lhs = 0.0f;
+                        lhs = lhs + o[q][0] * A0;
+                        lhs = lhs + o[q][1] * A1;
+                        lhs = lhs + o[q][2] * A2;
+                        v[q] = v[q] && (lhs * om < B);
+                    }
+                }
             }
         }
     }
-    return v;
 }
 
 __device__ __forceinline__ int wave_sum(int x)
@@ -466,27 +547,42 @@ __device__ __forceinline__ uint32_t nonzero_bytes01(uint32_t x)
     return ((((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u) >> 7;
 }
 
+// index of this wave within the grid, as a wave-uniform (scalar) value
+__device__ __forceinline__ int64_t wave_index()
+{
+    return (int64_t)blockIdx.x * (blockDim.x >> 6) +
+           __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
+constexpr int kWalkGroups = 4; // particle groups of 64 per walk, wave-per-path kernel
+constexpr int kHsGroups = 8;   // the same for the batched half-space kernel (swept: 2, 4, 8)
+
+// One wave per new path (single launch; small wavefronts).
 template <bool kBits>
 __global__ __launch_bounds__(256) void k_propagate_wave(
     int n_new, const int64_t *__restrict__ wp_offsets, int w_fixed,
     const int *__restrict__ wf_edge, const int *__restrict__ wf_prev,
-    const float *__restrict__ as, const float *__restrict__ bs, const float *__restrict__ times,
+    const float *__restrict__ as, const float *__restrict__ bs,
+    const uint64_t *__restrict__ edge_mask, const float *__restrict__ times,
     const float *__restrict__ topts, float dt, int max_hs, const float *__restrict__ offsets_time,
     int tmax_steps, float om, int n_mc, const void *__restrict__ valid_src,
     void *__restrict__ valid_dst, int64_t dst_base, int n_p, float *__restrict__ path_cp,
     int64_t cp_base, unsigned long long *__restrict__ clamped)
 {
+    constexpr int NQ = kWalkGroups;
     const int lane = threadIdx.x & 63;
-    const int64_t n_waves = (int64_t)gridDim.x * 4;
-    for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < n_new; w += n_waves) {
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t w = wave_index(); w < n_new; w += n_waves) {
         const int e = wf_edge[w];
         const int prev = wf_prev[w];
         const PathInfo pi = path_info(e, prev, wp_offsets, w_fixed, topts, times);
+        const uint64_t emask = edge_mask ? edge_mask[e] : 0ull;
         int count = 0; // valid particles (bits) or invalid particles (bytes), per lane
         // waypoints in groups of 256; later groups continue from the child row
         for (int i0 = 0; i0 < pi.W; i0 += 256) {
             uint64_t masks[4];
-            const uint64_t any = hs_masks(masks, pi, i0, max_hs, bs, dt, tmax_steps, clamped);
+            const uint64_t any = hs_masks(masks, pi, i0, edge_mask != nullptr, emask, max_hs, bs,
+                                          dt, tmax_steps, clamped);
             const bool last = i0 + 256 >= pi.W;
             if (kBits) {
                 const int nwords = (n_p + 63) >> 6;
@@ -503,18 +599,29 @@ __global__ __launch_bounds__(256) void k_propagate_wave(
                             count += __popcll(x);
                     }
                 } else {
-                    for (int wd = 0; wd < nwords; ++wd) {
-                        const uint64_t word = src[wd];
-                        const int p = wd * 64 + lane;
-                        bool v = (p < n_p) && ((word >> lane) & 1ull);
-                        if (__ballot(v) != 0ull)
-                            v = particle_walk(v, p, masks, i0, pi, max_hs, as, bs, offsets_time,
-                                              n_mc, tmax_steps, dt, om);
-                        const uint64_t out = __ballot(v);
-                        if (lane == 0) {
-                            dst[wd] = out;
-                            if (last)
-                                count += __popcll(out);
+                    for (int wd0 = 0; wd0 < nwords; wd0 += NQ) {
+                        bool v[NQ];
+                        int p[NQ];
+                        uint64_t alive = 0;
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            const int wd = wd0 + q;
+                            p[q] = wd * 64 + lane;
+                            const uint64_t word = wd < nwords ? src[wd] : 0ull;
+                            v[q] = (p[q] < n_p) && ((word >> lane) & 1ull);
+                            alive |= __ballot(v[q]);
+                        }
+                        if (alive)
+                            particle_walk<NQ>(v, p, n_p, masks, i0, pi, max_hs, as, bs,
+                                              offsets_time, n_mc, tmax_steps, dt, om);
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            const uint64_t out = __ballot(v[q]);
+                            if (lane == q && wd0 + q < nwords) {
+                                dst[wd0 + q] = out;
+                                if (last)
+                                    count += __popcll(out);
+                            }
                         }
                     }
                 }
@@ -536,16 +643,26 @@ __global__ __launch_bounds__(256) void k_propagate_wave(
                             count += 4 - __popc(x);
                     }
                 } else {
-                    for (int p0 = 0; p0 < n_p; p0 += 64) {
-                        const int p = p0 + lane;
-                        bool v = (p < n_p) && src[p] != 0;
-                        if (any && __ballot(v) != 0ull)
-                            v = particle_walk(v, p, masks, i0, pi, max_hs, as, bs, offsets_time,
-                                              n_mc, tmax_steps, dt, om);
-                        if (p < n_p) {
-                            dst[p] = v ? 1 : 0;
-                            if (last)
-                                count += v ? 0 : 1;
+                    for (int p0 = 0; p0 < n_p; p0 += 64 * NQ) {
+                        bool v[NQ];
+                        int p[NQ];
+                        uint64_t alive = 0;
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            p[q] = p0 + q * 64 + lane;
+                            v[q] = (p[q] < n_p) && src[p[q]] != 0;
+                            alive |= __ballot(v[q]);
+                        }
+                        if (any && alive)
+                            particle_walk<NQ>(v, p, n_p, masks, i0, pi, max_hs, as, bs,
+                                              offsets_time, n_mc, tmax_steps, dt, om);
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            if (p[q] < n_p) {
+                                dst[p[q]] = v[q] ? 1 : 0;
+                                if (last)
+                                    count += v[q] ? 0 : 1;
+                            }
                         }
                     }
                 }
@@ -555,6 +672,230 @@ __global__ __launch_bounds__(256) void k_propagate_wave(
         if (path_cp && lane == 0) {
             const int invalid = kBits ? n_p - count : count;
             path_cp[cp_base + w] = ((float)invalid) / (float)n_p;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batched form for large wavefronts:
+//  (1) k_classify: lane = path; a path needs half-space tests iff its edge
+//      has a marked waypoint; those paths are appended to a list of path
+//      records (one atomic per wave) and their CP accumulators cleared;
+//  (2) k_copy_rows: the other paths' parent rows copied by a flat, fully
+//      parallel kernel that also counts them;
+//  (3) k_hs_groups: one wave per (listed path, 256 particles); the path's CP
+//      is finished by the last of its waves (a 64-bit atomic ticket packing
+//      the finished-wave count and the particle count).
+struct HsPath {
+    int64_t g0;     // first waypoint of the edge
+    uint64_t emask; // the edge's waypoint summary (when known)
+    float dtl, tbase;
+    int W, prev, w, pad;
+};
+
+__global__ __launch_bounds__(256) void k_classify(
+    int n_new, const int64_t *__restrict__ wp_offsets, int w_fixed,
+    const int *__restrict__ wf_edge, const int *__restrict__ wf_prev,
+    const float *__restrict__ bs, const uint64_t *__restrict__ edge_mask, int max_hs,
+    const float *__restrict__ times, const float *__restrict__ topts, float dt, int tmax_steps,
+    uint8_t *__restrict__ has_hs, HsPath *__restrict__ recs, int *__restrict__ hs_count,
+    unsigned long long *__restrict__ acc, unsigned long long *__restrict__ clamped)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // wave-uniform trip count: every lane reaches the ballots below
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n_new;
+         base += stride) {
+        const int64_t w = base + lane;
+        bool has = false;
+        int n_clamped = 0;
+        HsPath r;
+        if (w < n_new) {
+            const int e = wf_edge[w];
+            // an all-clear summary settles the path with one load
+            const uint64_t em0 = edge_mask ? edge_mask[e] : ~0ull;
+            if (em0 != 0ull) {
+                r.prev = wf_prev[w];
+                const PathInfo pi = path_info(e, r.prev, wp_offsets, w_fixed, topts, times);
+                int scan_from = 0;
+                r.emask = 0ull;
+                if (edge_mask) {
+                    const uint64_t em = mask_bits(em0, pi.W);
+                    const bool exact = mask_exact(em, pi.W);
+                    has = em != 0ull;
+                    r.emask = em;
+                    // waypoints the summary names exactly: bits 0..62, and bit 63 when exact
+                    for (uint64_t m = exact ? em : em & ~(1ull << 63); m; m &= m - 1)
+                        n_clamped += time_clamped(pi, __builtin_ctzll(m), dt, tmax_steps) ? 1 : 0;
+                    scan_from = exact ? pi.W : 63;
+                }
+                for (int i = scan_from; i < pi.W; ++i) {
+                    if (waypoint_has_hs(bs, (pi.g0 + i) * (int64_t)max_hs, max_hs)) {
+                        has = true;
+                        n_clamped += time_clamped(pi, i, dt, tmax_steps) ? 1 : 0;
+                    }
+                }
+                r.g0 = pi.g0;
+                r.W = pi.W;
+                r.dtl = pi.dtl;
+                r.tbase = pi.tbase;
+                r.w = (int)w;
+                r.pad = 0;
+            }
+            has_hs[w] = has ? 1 : 0;
+        }
+        if (n_clamped && clamped)
+            atomicAdd(clamped, (unsigned long long)n_clamped);
+        const uint64_t m = __ballot(has);
+        if (m) {
+            int slot0 = 0;
+            if (lane == 0)
+                slot0 = atomicAdd(hs_count, __popcll(m));
+            slot0 = __builtin_amdgcn_readfirstlane(slot0);
+            if (has) {
+                const int slot = slot0 + __popcll(m & ((1ull << lane) - 1ull));
+                recs[slot] = r;
+                acc[slot] = 0ull;
+            }
+        }
+    }
+}
+
+// Copy unit per lane: 8-byte words (bit masks) or 16/4/1-byte chunks of a
+// byte row.  Each path is served by a segment of S lanes (S a power of two,
+// at most 64); the segment sums its count with shuffles of width S.
+template <int kUnit, bool kBits>
+__global__ __launch_bounds__(256) void k_copy_rows(int n_new, const int *__restrict__ wf_prev,
+                                                   const uint8_t *__restrict__ has_hs,
+                                                   const uint8_t *__restrict__ src,
+                                                   uint8_t *__restrict__ dst, int64_t dst_base,
+                                                   int64_t row_bytes, int n_p, int seg,
+                                                   float *__restrict__ path_cp, int64_t cp_base)
+{
+    const int lane = threadIdx.x & 63;
+    const int units = (int)(row_bytes / kUnit);
+    const int per_wave = 64 / seg;
+    const int64_t n_waves = (int64_t)gridDim.x * 4;
+    const int sub = lane / seg, j0 = lane % seg;
+    for (int64_t wb = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * per_wave; wb < n_new;
+         wb += n_waves * per_wave) {
+        const int64_t w = wb + sub;
+        int count = 0;
+        const bool mine = (w < n_new) && !has_hs[w];
+        if (mine) {
+            const uint8_t *s = src + (int64_t)wf_prev[w] * row_bytes;
+            uint8_t *d = dst + (dst_base + w) * row_bytes;
+            for (int j = j0; j < units; j += seg) {
+                if (kUnit == 16) {
+                    uint4 x = reinterpret_cast<const uint4 *>(s)[j];
+                    x.x = nonzero_bytes01(x.x);
+                    x.y = nonzero_bytes01(x.y);
+                    x.z = nonzero_bytes01(x.z);
+                    x.w = nonzero_bytes01(x.w);
+                    reinterpret_cast<uint4 *>(d)[j] = x;
+                    count += 16 - __popc(x.x) - __popc(x.y) - __popc(x.z) - __popc(x.w);
+                } else if (kUnit == 8) {
+                    const uint64_t x = reinterpret_cast<const uint64_t *>(s)[j];
+                    reinterpret_cast<uint64_t *>(d)[j] = x;
+                    count += __popcll(x);
+                } else if (kUnit == 4) {
+                    const uint32_t x = nonzero_bytes01(reinterpret_cast<const uint32_t *>(s)[j]);
+                    reinterpret_cast<uint32_t *>(d)[j] = x;
+                    count += 4 - __popc(x);
+                } else {
+                    const uint8_t x = s[j] ? 1 : 0;
+                    d[j] = x;
+                    count += 1 - x;
+                }
+            }
+        }
+        for (int off = seg >> 1; off > 0; off >>= 1)
+            count += __shfl_down(count, off, seg);
+        if (mine && j0 == 0 && path_cp) {
+            const int invalid = kBits ? n_p - count : count;
+            path_cp[cp_base + w] = ((float)invalid) / (float)n_p;
+        }
+    }
+}
+
+template <bool kBits, int NQ>
+__global__ __launch_bounds__(256) void k_hs_groups(
+    const HsPath *__restrict__ recs, const int *__restrict__ hs_count, int known,
+    const float *__restrict__ as, const float *__restrict__ bs, float dt, int max_hs,
+    const float *__restrict__ offsets_time, int tmax_steps, float om, int n_mc,
+    const void *__restrict__ valid_src, void *__restrict__ valid_dst, int64_t dst_base, int n_p,
+    float *__restrict__ path_cp, int64_t cp_base, unsigned long long *__restrict__ acc)
+{
+    const int lane = threadIdx.x & 63;
+    const int n_groups = (n_p + 63) >> 6;
+    const int n_sg = (n_groups + NQ - 1) / NQ; // waves per listed path
+    const int64_t n_items = (int64_t)(*hs_count) * n_sg;
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t item = wave_index(); item < n_items; item += n_waves) {
+        const int slot = (int)(item / n_sg);
+        const int sg = (int)(item - (int64_t)slot * n_sg);
+        const HsPath r = recs[slot];
+        PathInfo pi;
+        pi.g0 = r.g0;
+        pi.W = r.W;
+        pi.dtl = r.dtl;
+        pi.tbase = r.tbase;
+        bool v[NQ];
+        int p[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int grp = sg * NQ + q;
+            p[q] = grp * 64 + lane;
+            if (kBits) {
+                const uint64_t word =
+                    grp < n_groups
+                        ? ((const uint64_t *)valid_src)[(int64_t)r.prev * n_groups + grp]
+                        : 0ull;
+                v[q] = (p[q] < n_p) && ((word >> lane) & 1ull);
+            } else {
+                v[q] = (p[q] < n_p) &&
+                       ((const uint8_t *)valid_src)[(int64_t)r.prev * n_p + p[q]] != 0;
+            }
+        }
+        for (int i0 = 0; i0 < pi.W; i0 += 256) {
+            uint64_t masks[4];
+            const uint64_t any = hs_masks(masks, pi, i0, known != 0, r.emask, max_hs, bs, dt,
+                                          tmax_steps, (unsigned long long *)nullptr);
+            uint64_t alive = 0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                alive |= __ballot(v[q]);
+            if (any && alive)
+                particle_walk<NQ>(v, p, n_p, masks, i0, pi, max_hs, as, bs, offsets_time, n_mc,
+                                  tmax_steps, dt, om);
+        }
+        int count = 0; // valid (bits) or invalid (bytes) particles of this wave
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int grp = sg * NQ + q;
+            const uint64_t out = __ballot(v[q]);
+            if (grp < n_groups) {
+                if (kBits) {
+                    if (lane == q)
+                        ((uint64_t *)valid_dst)[(dst_base + r.w) * (int64_t)n_groups + grp] = out;
+                    count += __popcll(out);
+                } else {
+                    if (p[q] < n_p)
+                        ((uint8_t *)valid_dst)[(dst_base + r.w) * (int64_t)n_p + p[q]] =
+                            v[q] ? 1 : 0;
+                    const int in_group = (n_p - grp * 64) < 64 ? (n_p - grp * 64) : 64;
+                    count += in_group - __popcll(out);
+                }
+            }
+        }
+        if (lane == 0) {
+            const unsigned long long old =
+                atomicAdd(&acc[slot], (1ull << 32) | (unsigned long long)count);
+            if ((int)(old >> 32) + 1 == n_sg && path_cp) {
+                const int total = (int)(old & 0xffffffffull) + count;
+                const int invalid = kBits ? n_p - total : total;
+                path_cp[cp_base + r.w] = ((float)invalid) / (float)n_p;
+            }
         }
     }
 }
@@ -599,7 +940,7 @@ extern "C" int pump_cache_halfspaces(int64_t n_edges, const float *d_disc_motion
                                      const float *d_obstacles, int waypoints_count,
                                      const int64_t *d_wp_offsets, float *d_as, float *d_bs,
                                      int *d_count_stored, float maxd2, int write_sentinels,
-                                     pump_stream_t stream)
+                                     uint64_t *d_edge_hs_mask, pump_stream_t stream)
 {
     PUMP_REQUIRE(n_edges >= 0 && n_obs >= 0 && max_hs >= 0);
     if (n_edges == 0)
@@ -614,12 +955,12 @@ extern "C" int pump_cache_halfspaces(int64_t n_edges, const float *d_disc_motion
         hipLaunchKernelGGL(k_cache_hs<true>, dim3((unsigned)blocks), dim3(kHsBlock), 0,
                            as_stream(stream), n_edges, d_disc_motions, d_wp_offsets,
                            waypoints_count, d_is_free_edges, n_obs, max_hs, d_obstacles, d_as,
-                           d_bs, d_count_stored, maxd2);
+                           d_bs, d_count_stored, maxd2, d_edge_hs_mask);
     else
         hipLaunchKernelGGL(k_cache_hs<false>, dim3((unsigned)blocks), dim3(kHsBlock), 0,
                            as_stream(stream), n_edges, d_disc_motions, d_wp_offsets,
                            waypoints_count, d_is_free_edges, n_obs, max_hs, d_obstacles, d_as,
-                           d_bs, d_count_stored, maxd2);
+                           d_bs, d_count_stored, maxd2, d_edge_hs_mask);
     PUMP_LAUNCHED();
     return PUMP_OK;
 }
@@ -649,7 +990,7 @@ static bool use_block_kernel()
     return v == 1;
 }
 
-static unsigned wave_grid(int n_new)
+static unsigned wave_grid(int64_t n_new)
 {
     int dev = 0, cus = 256;
     hipGetDevice(&dev);
@@ -696,10 +1037,10 @@ extern "C" int pump_propagate_valid_particles(
     else
         hipLaunchKernelGGL(k_propagate_wave<false>, dim3(wave_grid(num_new_paths)), dim3(256), 0,
                            as_stream(stream), num_new_paths, d_wp_offsets, waypoints_count,
-                           d_wf_edge, d_wf_prev, d_as, d_bs, d_times, d_topts, dt, max_hs,
-                           d_offsets_time, tmax_steps, offset_mult, num_mc_particles,
-                           (const void *)d_valid, (void *)d_valid, path_base, n_p,
-                           (float *)nullptr, (int64_t)0, d_clamped);
+                           d_wf_edge, d_wf_prev, d_as, d_bs, (const uint64_t *)nullptr, d_times,
+                           d_topts, dt, max_hs, d_offsets_time, tmax_steps, offset_mult,
+                           num_mc_particles, (const void *)d_valid, (void *)d_valid, path_base,
+                           n_p, (float *)nullptr, (int64_t)0, d_clamped);
     PUMP_LAUNCHED();
     return PUMP_OK;
 }
@@ -717,15 +1058,116 @@ extern "C" int pump_calculate_cp_hsmc(int num_new_paths, int64_t path_base, floa
     return PUMP_OK;
 }
 
+static size_t round256(size_t x)
+{
+    return (x + 255) & ~(size_t)255;
+}
+
+// Workspace of the batched form: the list length, per-path flags, the
+// records of the paths that meet a half-space and one CP accumulator per
+// listed path.
+extern "C" size_t pump_propagate_workspace(int num_new_paths)
+{
+    const size_t n = num_new_paths > 0 ? (size_t)num_new_paths : 1;
+    return 256 + round256(n) + round256(n * sizeof(HsPath)) + n * sizeof(uint64_t);
+}
+
+// Launches the propagate + CP step for a wavefront: batched three-kernel
+// form when a workspace is given and the wavefront is large, else one
+// wave-per-path (or, on request, workgroup-per-path) launch.
+template <bool kBits>
+static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, const int *wf_edge,
+                            const int *wf_prev, const float *as, const float *bs,
+                            const uint64_t *edge_mask, const float *times, const float *topts,
+                            float dt, int max_hs,
+                            const float *offsets_time, int tmax_steps, float om, int n_mc,
+                            const void *src, void *dst, int64_t dst_base, int n_p, float *path_cp,
+                            int64_t cp_base, unsigned long long *clamped, void *ws,
+                            size_t ws_bytes, hipStream_t s)
+{
+    const int64_t row_bytes = kBits ? (int64_t)((n_p + 63) / 64) * 8 : (int64_t)n_p;
+    if (ws && ws_bytes >= pump_propagate_workspace(n_new) && n_new >= 4096 && !use_block_kernel()) {
+        uint8_t *base = reinterpret_cast<uint8_t *>(ws);
+        int *count = reinterpret_cast<int *>(base);
+        uint8_t *has = base + 256;
+        HsPath *recs = reinterpret_cast<HsPath *>(has + round256((size_t)n_new));
+        unsigned long long *acc = reinterpret_cast<unsigned long long *>(
+            reinterpret_cast<uint8_t *>(recs) + round256((size_t)n_new * sizeof(HsPath)));
+        PUMP_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_classify, dim3(grid_for(n_new, 256, 65536)), dim3(256), 0, s, n_new,
+                           wp_offsets, W_fixed, wf_edge, wf_prev, bs, edge_mask, max_hs, times,
+                           topts, dt, tmax_steps, has, recs, count, acc, clamped);
+        PUMP_LAUNCHED();
+        // copy unit and segment width
+        const uintptr_t al = ((uintptr_t)src | (uintptr_t)dst);
+        int unit = 1;
+        if (kBits)
+            unit = 8;
+        else if (row_bytes % 16 == 0 && (al & 15) == 0)
+            unit = 16;
+        else if (row_bytes % 4 == 0 && (al & 3) == 0)
+            unit = 4;
+        const int units = (int)(row_bytes / unit);
+        int seg = 1;
+        while (seg < units && seg < 64)
+            seg <<= 1;
+        const int64_t waves = ((int64_t)n_new * seg + 63) / 64;
+        const unsigned grid = grid_for(waves * 64, 256, 65536);
+        const uint8_t *sb = (const uint8_t *)src;
+        uint8_t *db = (uint8_t *)dst;
+        if (unit == 8)
+            hipLaunchKernelGGL((k_copy_rows<8, true>), dim3(grid), dim3(256), 0, s, n_new, wf_prev,
+                               has, sb, db, dst_base, row_bytes, n_p, seg, path_cp, cp_base);
+        else if (unit == 16)
+            hipLaunchKernelGGL((k_copy_rows<16, false>), dim3(grid), dim3(256), 0, s, n_new,
+                               wf_prev, has, sb, db, dst_base, row_bytes, n_p, seg, path_cp, cp_base);
+        else if (unit == 4)
+            hipLaunchKernelGGL((k_copy_rows<4, false>), dim3(grid), dim3(256), 0, s, n_new,
+                               wf_prev, has, sb, db, dst_base, row_bytes, n_p, seg, path_cp, cp_base);
+        else
+            hipLaunchKernelGGL((k_copy_rows<1, false>), dim3(grid), dim3(256), 0, s, n_new,
+                               wf_prev, has, sb, db, dst_base, row_bytes, n_p, seg, path_cp, cp_base);
+        PUMP_LAUNCHED();
+        // (listed paths) x (64*kHsGroups particles), one wave each, grid-stride
+        const int64_t per_path = ((n_p + 63) / 64 + kHsGroups - 1) / kHsGroups;
+        hipLaunchKernelGGL((k_hs_groups<kBits, kHsGroups>), dim3(wave_grid(n_new * per_path)),
+                           dim3(256), 0, s, (const HsPath *)recs, (const int *)count,
+                           edge_mask ? 1 : 0, as, bs, dt, max_hs, offsets_time, tmax_steps, om,
+                           n_mc, src, dst, dst_base, n_p, path_cp, cp_base, acc);
+        PUMP_LAUNCHED();
+        return PUMP_OK;
+    }
+    if (use_block_kernel()) {
+        if (kBits)
+            hipLaunchKernelGGL(k_propagate_bits, dim3(n_new), dim3(256), 0, s, n_new, wp_offsets,
+                               W_fixed, wf_edge, wf_prev, as, bs, times, topts, dt, max_hs,
+                               offsets_time, tmax_steps, om, n_mc, (const uint64_t *)src,
+                               (uint64_t *)dst, dst_base, n_p, path_cp, cp_base, clamped);
+        else
+            hipLaunchKernelGGL(k_propagate, dim3(n_new), dim3(256), 0, s, n_new, wp_offsets,
+                               W_fixed, wf_edge, wf_prev, as, bs, times, topts, dt, max_hs,
+                               offsets_time, tmax_steps, om, n_mc, (const uint8_t *)src,
+                               (uint8_t *)dst, dst_base, n_p, path_cp, cp_base, clamped);
+    } else {
+        hipLaunchKernelGGL(k_propagate_wave<kBits>, dim3(wave_grid(n_new)), dim3(256), 0, s, n_new,
+                           wp_offsets, W_fixed, wf_edge, wf_prev, as, bs, edge_mask, times, topts,
+                           dt, max_hs, offsets_time, tmax_steps, om, n_mc, src, dst, dst_base, n_p,
+                           path_cp, cp_base, clamped);
+    }
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+
 extern "C" int pump_propagate_cp(int num_new_paths, int waypoints_count,
                                  const int64_t *d_wp_offsets, const int *d_wf_edge,
                                  const int *d_wf_prev, const float *d_as, const float *d_bs,
-                                 const float *d_times, const float *d_topts, float dt, int max_hs,
+                                 const uint64_t *d_edge_hs_mask, const float *d_times,
+                                 const float *d_topts, float dt, int max_hs,
                                  const float *d_offsets_time, int tmax_steps, float offset_mult,
                                  int num_mc_particles, const uint8_t *d_valid_src,
                                  uint8_t *d_valid_dst, int64_t dst_base, int n_p,
                                  float *d_path_cp, int64_t cp_base, unsigned long long *d_clamped,
-                                 pump_stream_t stream)
+                                 void *d_workspace, size_t workspace_bytes, pump_stream_t stream)
 {
     int r = check_prop_args(num_new_paths, waypoints_count, d_wp_offsets, max_hs, tmax_steps, n_p,
                             num_mc_particles);
@@ -735,21 +1177,12 @@ extern "C" int pump_propagate_cp(int num_new_paths, int waypoints_count,
         return PUMP_OK;
     PUMP_REQUIRE(d_wf_edge && d_wf_prev && d_as && d_bs && d_times && d_topts && d_offsets_time &&
                  d_valid_src && d_valid_dst);
-    if (use_block_kernel())
-        hipLaunchKernelGGL(k_propagate, dim3(num_new_paths), dim3(256), 0, as_stream(stream),
-                           num_new_paths, d_wp_offsets, waypoints_count, d_wf_edge, d_wf_prev,
-                           d_as, d_bs, d_times, d_topts, dt, max_hs, d_offsets_time, tmax_steps,
-                           offset_mult, num_mc_particles, d_valid_src, d_valid_dst, dst_base, n_p,
-                           d_path_cp, cp_base, d_clamped);
-    else
-        hipLaunchKernelGGL(k_propagate_wave<false>, dim3(wave_grid(num_new_paths)), dim3(256), 0,
-                           as_stream(stream), num_new_paths, d_wp_offsets, waypoints_count,
-                           d_wf_edge, d_wf_prev, d_as, d_bs, d_times, d_topts, dt, max_hs,
-                           d_offsets_time, tmax_steps, offset_mult, num_mc_particles,
-                           (const void *)d_valid_src, (void *)d_valid_dst, dst_base, n_p,
-                           d_path_cp, cp_base, d_clamped);
-    PUMP_LAUNCHED();
-    return PUMP_OK;
+    return launch_propagate<false>(num_new_paths, waypoints_count, d_wp_offsets, d_wf_edge,
+                                   d_wf_prev, d_as, d_bs, d_edge_hs_mask, d_times, d_topts, dt,
+                                   max_hs, d_offsets_time, tmax_steps, offset_mult,
+                                   num_mc_particles,
+                                   d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp, cp_base,
+                                   d_clamped, d_workspace, workspace_bytes, as_stream(stream));
 }
 
 extern "C" int pump_gather_rows(const uint8_t *d_src, const int *d_src_rows, int n_rows,
@@ -774,20 +1207,22 @@ int propagate_bits_mfma(int n_new, const int64_t *wp_offsets, int w_fixed, const
                         int tmax_steps, float om, int n_mc, const uint64_t *valid_src,
                         uint64_t *valid_dst, int64_t dst_base, int n_p, float *path_cp,
                         int64_t cp_base, unsigned long long *band, unsigned long long *clamped,
-                        hipStream_t s);
+                        void *ws, size_t ws_bytes, hipStream_t s);
 }
 
 extern "C" int pump_propagate_cp_bits(int num_new_paths, int waypoints_count,
                                       const int64_t *d_wp_offsets, const int *d_wf_edge,
                                       const int *d_wf_prev, const float *d_as, const float *d_bs,
-                                      const float *d_times, const float *d_topts, float dt,
-                                      int max_hs, const float *d_offsets_time, int tmax_steps,
+                                      const uint64_t *d_edge_hs_mask, const float *d_times,
+                                      const float *d_topts, float dt, int max_hs,
+                                      const float *d_offsets_time, int tmax_steps,
                                       float offset_mult, int num_mc_particles,
                                       const uint64_t *d_valid_src, uint64_t *d_valid_dst,
                                       int64_t dst_base, int n_p, float *d_path_cp,
                                       int64_t cp_base, int precision,
                                       unsigned long long *d_band_count,
-                                      unsigned long long *d_clamped, pump_stream_t stream)
+                                      unsigned long long *d_clamped, void *d_workspace,
+                                      size_t workspace_bytes, pump_stream_t stream)
 {
     int r = check_prop_args(num_new_paths, waypoints_count, d_wp_offsets, max_hs, tmax_steps, n_p,
                             num_mc_particles);
@@ -802,23 +1237,15 @@ extern "C" int pump_propagate_cp_bits(int num_new_paths, int waypoints_count,
                                    d_wf_prev, d_as, d_bs, d_times, d_topts, dt, max_hs,
                                    d_offsets_time, tmax_steps, offset_mult, num_mc_particles,
                                    d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp, cp_base,
-                                   d_band_count, d_clamped, as_stream(stream));
+                                   d_band_count, d_clamped, d_workspace, workspace_bytes,
+                                   as_stream(stream));
     PUMP_REQUIRE(precision == 0);
-    if (use_block_kernel())
-        hipLaunchKernelGGL(k_propagate_bits, dim3(num_new_paths), dim3(256), 0,
-                           as_stream(stream), num_new_paths, d_wp_offsets, waypoints_count,
-                           d_wf_edge, d_wf_prev, d_as, d_bs, d_times, d_topts, dt, max_hs,
-                           d_offsets_time, tmax_steps, offset_mult, num_mc_particles, d_valid_src,
-                           d_valid_dst, dst_base, n_p, d_path_cp, cp_base, d_clamped);
-    else
-        hipLaunchKernelGGL(k_propagate_wave<true>, dim3(wave_grid(num_new_paths)), dim3(256), 0,
-                           as_stream(stream), num_new_paths, d_wp_offsets, waypoints_count,
-                           d_wf_edge, d_wf_prev, d_as, d_bs, d_times, d_topts, dt, max_hs,
-                           d_offsets_time, tmax_steps, offset_mult, num_mc_particles,
-                           (const void *)d_valid_src, (void *)d_valid_dst, dst_base, n_p,
-                           d_path_cp, cp_base, d_clamped);
-    PUMP_LAUNCHED();
-    return PUMP_OK;
+    return launch_propagate<true>(num_new_paths, waypoints_count, d_wp_offsets, d_wf_edge,
+                                  d_wf_prev, d_as, d_bs, d_edge_hs_mask, d_times, d_topts, dt,
+                                  max_hs, d_offsets_time, tmax_steps, offset_mult,
+                                  num_mc_particles,
+                                  d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp, cp_base,
+                                  d_clamped, d_workspace, workspace_bytes, as_stream(stream));
 }
 
 // initial bit masks: all n_p particles valid

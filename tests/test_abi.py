@@ -40,8 +40,9 @@ def test_invalid_arguments_rejected_without_gpu():
     # argument validation happens before any device call
     assert P.lib.pump_fill_copts_topts(None, 100, 5.0, None, None, None) == -1
     assert P.lib.pump_select_rn(None, 10, 3, None, None, 0, None) == -1
-    assert P.lib.pump_propagate_cp(-1, 9, None, None, None, None, None, None, None, 0.05, 5, None,
-                                   10, 1.0, 64, None, None, 0, 64, None, 0, None, None) == -1
+    assert P.lib.pump_propagate_cp(-1, 9, None, None, None, None, None, None, None, None, 0.05, 5,
+                                   None, 10, 1.0, 64, None, None, 0, 64, None, 0, None, None, 0,
+                                   None) == -1
 
 
 def test_philox_known_answers():

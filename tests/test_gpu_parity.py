@@ -141,10 +141,10 @@ def test_search_parity(case, bits):
 
 # ------------------------------------------- entry points called directly ----
 def random_hsmc_inputs(seed, n_edges=60, W=7, max_hs=4, n_p=333, n_mc=512, T=80, n_new=50,
-                       n_parent=9, variable=False):
+                       n_parent=9, variable=False, w_range=(2, 40)):
     rng = np.random.default_rng(seed)
     if variable:
-        w = rng.integers(2, 40, n_edges)
+        w = rng.integers(w_range[0], w_range[1], n_edges)
     else:
         w = np.full(n_edges, W)
     off = np.zeros(n_edges + 1, np.int64)
@@ -167,6 +167,21 @@ def random_hsmc_inputs(seed, n_edges=60, W=7, max_hs=4, n_p=333, n_mc=512, T=80,
     return dict(off=off, w=w, as_=as_, bs=bs, offs=offs, topts=topts, valid=valid, times=times,
                 wf_edge=wf_edge, wf_prev=wf_prev, max_hs=max_hs, n_p=n_p, n_mc=n_mc, T=T,
                 n_new=n_new, n_parent=n_parent, om=np.float32(0.8), dt=np.float32(0.05))
+
+
+def edge_mask_host(off, bs, max_hs):
+    """Per-edge waypoint summary: bit min(k, 63) set iff waypoint k of the
+    edge holds a slot whose b is not >= 1000 (the sentinel test of the
+    propagate step)."""
+    has = ~(bs.reshape(-1, max_hs) >= 1000).all(axis=1)
+    n_edges = off.shape[0] - 1
+    edge_of = np.repeat(np.arange(n_edges), np.diff(off))
+    k = np.arange(off[-1]) - off[edge_of]
+    bits = np.where(has, np.left_shift(np.uint64(1), np.minimum(k, 63).astype(np.uint64)),
+                    np.uint64(0)).astype(np.uint64)
+    out = np.zeros(n_edges, np.uint64)
+    np.bitwise_or.at(out, edge_of, bits)
+    return out
 
 
 def oracle_propagate(d):
@@ -210,8 +225,9 @@ def test_propagate_entry_points(variable):
     cp2 = dev(np.zeros(d["n_new"], np.float32))
     P.check("propagate_cp", P.lib.pump_propagate_cp(
         d["n_new"], W, off_ptr, ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
-        ptr(g["bs"]), ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"], ptr(g["offs"]),
-        d["T"], d["om"], d["n_mc"], ptr(src), ptr(dst), 0, d["n_p"], ptr(cp2), 0, None, None))
+        ptr(g["bs"]), None, ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"],
+        ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(src), ptr(dst), 0, d["n_p"], ptr(cp2), 0,
+        None, None, 0, None))
     assert np.array_equal(host(dst), ref_v[d["n_parent"]:])
     assert np.array_equal(host(cp2), ref_cp[d["n_parent"]:])
     # bit masks
@@ -226,14 +242,88 @@ def test_propagate_entry_points(variable):
     cp3 = dev(np.zeros(d["n_new"], np.float32))
     P.check("propagate_cp_bits", P.lib.pump_propagate_cp_bits(
         d["n_new"], W, off_ptr, ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
-        ptr(g["bs"]), ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"], ptr(g["offs"]),
-        d["T"], d["om"], d["n_mc"], ptr(bsrc), ptr(bdst), 0, d["n_p"], ptr(cp3), 0, 0, None,
-        None, None))
+        ptr(g["bs"]), None, ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"],
+        ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(bsrc), ptr(bdst), 0, d["n_p"], ptr(cp3),
+        0, 0, None, None, None, 0, None))
     out = host(bdst)
     for r in range(d["n_new"]):
         unpacked = np.unpackbits(out[r].view(np.uint8), bitorder="little")[: d["n_p"]]
         assert np.array_equal(unpacked, ref_v[d["n_parent"] + r])
     assert np.array_equal(host(cp3), ref_cp[d["n_parent"]:])
+
+
+def pack_bits(rows, n_p):
+    nw = (n_p + 63) // 64
+    out = np.zeros((rows.shape[0], nw), np.uint64)
+    for r in range(rows.shape[0]):
+        packed = np.packbits(rows[r].astype(np.uint8), bitorder="little")
+        packed = np.pad(packed, (0, nw * 8 - packed.size))
+        out[r] = packed.view(np.uint64)
+    return out
+
+
+def unpack_bits(words, n_p):
+    return np.stack([np.unpackbits(w.view(np.uint8), bitorder="little")[:n_p] for w in words])
+
+
+@pytest.mark.parametrize("n_p,w_range", [(333, (2, 40)), (2000, (2, 40)), (512, (2, 40)),
+                                          (640, (40, 300))])
+def test_propagate_batched_form(n_p, w_range):
+    """Large wavefronts (>= 4096 paths) with a workspace take the batched
+    classify / copy / half-space group form; results must equal the oracle,
+    with and without the per-edge waypoint summary (edges up to 300
+    waypoints exercise the summary's inexact case and waypoint groups past
+    256)."""
+    d = random_hsmc_inputs(21 + n_p, n_edges=3000, W=9, max_hs=5, n_p=n_p, n_mc=2048, T=120,
+                           n_new=6000, n_parent=64, variable=True, w_range=w_range)
+    # most edges hold no half-space, as in the real workloads
+    keep = np.random.default_rng(1).uniform(size=d["off"].shape[0] - 1) < 0.15
+    for e in np.nonzero(~keep)[0]:
+        d["bs"][d["off"][e] * d["max_hs"]: d["off"][e + 1] * d["max_hs"]] = 1111
+    ref_v, ref_cp, ref_cl = oracle_propagate(d)
+    g = {k: dev(d[k]) for k in ("off", "as_", "bs", "offs", "topts", "times", "wf_edge",
+                                 "wf_prev")}
+    mask = edge_mask_host(d["off"], d["bs"], d["max_hs"])
+    assert (mask >> np.uint64(63)).any() == (w_range[1] > 64)
+    g_mask = dev(mask)
+    ws_bytes = P.lib.pump_propagate_workspace(d["n_new"])
+    ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
+    npar, nn = d["n_parent"], d["n_new"]
+    for m in (None, ptr(g_mask)):
+        src = dev(d["valid"][:npar])
+        dst = dev(np.zeros((nn, n_p), np.uint8))
+        cp = dev(np.zeros(nn, np.float32))
+        clamp = dev(np.zeros(2, np.uint64))
+        P.check("propagate_cp", P.lib.pump_propagate_cp(
+            nn, 0, ptr(g["off"]), ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
+            ptr(g["bs"]), m, ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"],
+            ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(src), ptr(dst), 0, n_p, ptr(cp), 0,
+            ptr(clamp), ws.p, ws_bytes, None))
+        assert np.array_equal(host(dst), ref_v[npar:])
+        assert np.array_equal(host(cp), ref_cp[npar:])
+        assert int(host(clamp)[0]) == ref_cl
+        bsrc = dev(pack_bits(d["valid"][:npar], n_p))
+        bdst = dev(np.zeros((nn, (n_p + 63) // 64), np.uint64))
+        cpb = dev(np.zeros(nn, np.float32))
+        clamp_b = dev(np.zeros(2, np.uint64))
+        P.check("propagate_cp_bits", P.lib.pump_propagate_cp_bits(
+            nn, 0, ptr(g["off"]), ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
+            ptr(g["bs"]), m, ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"],
+            ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(bsrc), ptr(bdst), 0, n_p, ptr(cpb), 0,
+            0, None, ptr(clamp_b), ws.p, ws_bytes, None))
+        assert np.array_equal(unpack_bits(host(bdst), n_p), ref_v[npar:])
+        assert np.array_equal(host(cpb), ref_cp[npar:])
+        assert int(host(clamp_b)[0]) == ref_cl
+        # the single-launch form takes the summary too
+        dst1 = dev(np.zeros((nn, n_p), np.uint8))
+        cp1 = dev(np.zeros(nn, np.float32))
+        P.check("propagate_cp", P.lib.pump_propagate_cp(
+            nn, 0, ptr(g["off"]), ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
+            ptr(g["bs"]), m, ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"],
+            ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(src), ptr(dst1), 0, n_p, ptr(cp1), 0,
+            None, None, 0, None))
+        assert np.array_equal(host(dst1), ref_v[npar:])
+        assert np.array_equal(host(cp1), ref_cp[npar:])
 
 
 def test_gather_rows():
@@ -308,13 +398,17 @@ def test_free_edges_and_discretize_entry_points():
     hs = n_wp * max_hs
     as_, bs = dev(np.zeros((hs, 3), np.float32)), dev(np.zeros(hs, np.float32))
     cnt = dev(np.full(n_wp, -7, np.int32))
+    emask = dev(np.full(E, 12345, np.uint64))
     P.check("setup_as_bs", P.lib.pump_setup_as_bs(hs, ptr(bs), ptr(as_), None))
     P.check("cache_hs", P.lib.pump_cache_halfspaces(
         E, ptr(gdisc), ptr(dev(orc["is_free_edges"])), o.shape[0], max_hs, ptr(go), 0, ptr(goff),
-        ptr(as_), ptr(bs), ptr(cnt), orc["maxd2"], 0, None))
+        ptr(as_), ptr(bs), ptr(cnt), orc["maxd2"], 0, ptr(emask), None))
     assert np.array_equal(host(cnt), orc["count_stored"])
     assert np.array_equal(host(bs), orc["bs"][:hs])
     assert np.array_equal(host(as_), orc["as_"][:hs])
+    ref_mask = edge_mask_host(orc["wp_offsets"], orc["bs"][:hs], max_hs)
+    assert ref_mask.any()
+    assert np.array_equal(host(emask), ref_mask)
     rm.close()
 
 
