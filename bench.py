@@ -203,7 +203,7 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
     g_edge, g_prev = P.DeviceArray(free), P.DeviceArray(prev)
     g_times = P.DeviceArray(times)
     g_cp = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
-    ws_bytes = int(P.lib.pump_propagate_workspace(n_new))
+    ws_bytes = int(P.lib.pump_propagate_workspace(n_new, T))
     g_ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
     out = {"n_new_paths": n_new, "particle_steps": steps, "stored_halfspace_tests": n_p * hs_total,
            "paths_with_halfspaces": int((marked_per_path > 0).sum()),

@@ -203,12 +203,12 @@ int pump_calculate_cp_hsmc(int num_new_paths, int64_t path_base, float *d_path_c
  * dst_base+w of d_valid_dst).  Same results as the two calls above; used by
  * the search consumer so that compaction needs no aliasing copies
  * (PUMP.cu:236-237 issues one D2D memcpy per surviving path).  With a
- * workspace of pump_propagate_workspace(num_new_paths) bytes, large
- * wavefronts run in a batched form (classify / copy / half-space paths);
- * pass NULL, 0 for a single launch.  d_edge_hs_mask: the per-edge waypoint
+ * workspace of pump_propagate_workspace(num_new_paths, tmax_steps) bytes,
+ * large wavefronts run in a batched form (classify, copy, and the
+ * half-space tests binned by time index); pass NULL, 0 for a single launch.  d_edge_hs_mask: the per-edge waypoint
  * summary written by pump_cache_halfspaces for these d_bs, or NULL (then
  * every waypoint's slots are scanned, as the reference does). */
-size_t pump_propagate_workspace(int num_new_paths);
+size_t pump_propagate_workspace(int num_new_paths, int tmax_steps);
 int pump_propagate_cp(int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets,
                       const int *d_wf_edge, const int *d_wf_prev, const float *d_as,
                       const float *d_bs, const uint64_t *d_edge_hs_mask, const float *d_times,

@@ -751,7 +751,7 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
 
         // HSMC step on the device (PUMP.cu:80-160)
         TRY(stage.ensure((size_t)n_new * row_bytes));
-        TRY(d_ws.ensure(pump_propagate_workspace(n_new)));
+        TRY(d_ws.ensure(pump_propagate_workspace(n_new, rm->tmax_steps)));
         TRY(d_cp.ensure(sizeof(float) * n_new));
         TRY(d_prev.ensure(sizeof(int) * n_new));
         TRY(d_edge.ensure(sizeof(int) * n_new));

@@ -19,6 +19,7 @@
 // (waypoint, half-space) pairs is order-independent.
 #include <cstdlib>
 
+#include "hsmc_wave.hpp"
 #include "pump_device.hpp"
 #include "pump_internal.hpp"
 
@@ -26,6 +27,8 @@
 
 namespace pump {
 namespace {
+
+using namespace wave;
 
 constexpr int kEdgesPerBlock = 32;
 constexpr int kHsBlock = 256;
@@ -376,187 +379,6 @@ __global__ __launch_bounds__(256) void k_propagate_bits(
 // streamed 64 at a time (lane = particle) and tested only against those
 // waypoints' half-spaces, whose data are wave-uniform loads.  Decisions use
 // the reference's float order.  No LDS, no workgroup barriers.
-struct PathInfo {
-    int64_t g0;
-    int W;
-    float dtl, tbase;
-};
-
-__device__ __forceinline__ PathInfo path_info(int e, int prev, const int64_t *wp_offsets,
-                                              int w_fixed, const float *topts,
-                                              const float *times)
-{
-    PathInfo pi;
-    pi.g0 = wp_offsets ? wp_offsets[e] : (int64_t)e * w_fixed;
-    pi.W = (int)((wp_offsets ? wp_offsets[e + 1] : pi.g0 + w_fixed) - pi.g0);
-    pi.dtl = topts[e] / (float)(pi.W - 1);
-    pi.tbase = times[prev];
-    return pi;
-}
-
-__device__ __forceinline__ bool time_clamped(const PathInfo &pi, int i, float dt, int tmax_steps)
-{
-    float time = pi.tbase + (float)i * pi.dtl;
-    int ti = (int)(time / dt);
-    return ti < 0 || ti >= tmax_steps;
-}
-
-// An edge's waypoint mask (bit k: waypoint k holds a stored half-space, bit
-// 63: waypoint 63 or a later one does) is exact when the edge has at most 64
-// waypoints or bit 63 is clear.
-__device__ __forceinline__ bool mask_exact(uint64_t emask, int W)
-{
-    return W <= 64 || !(emask >> 63);
-}
-
-// the summary's bits that name waypoints of a W-waypoint edge
-__device__ __forceinline__ uint64_t mask_bits(uint64_t emask, int W)
-{
-    return W >= 64 ? emask : emask & ((1ull << W) - 1ull);
-}
-
-// A stored half-space: the reference skips entries with b >= 1000
-// (PUMP.cu:642), so "holds a half-space" means some b that is not >= 1000.
-__device__ __forceinline__ bool waypoint_has_hs(const float *__restrict__ bs, int64_t head,
-                                                int max_hs)
-{
-    bool has = false;
-    for (int h = 0; h < max_hs; ++h)
-        has |= !(bs[head + h] >= 1000.0f);
-    return has;
-}
-
-// masks[c] bit l: waypoint i0+c*64+l holds a stored half-space (i0..i0+255),
-// from the edge's waypoint mask when it is known and exact, else by ballots
-// over bs.  Counts clamped time indices of marked waypoints into *clamped
-// when non-null.
-__device__ __forceinline__ uint64_t hs_masks(uint64_t *masks, const PathInfo &pi, int i0,
-                                             bool known, uint64_t emask, int max_hs,
-                                             const float *__restrict__ bs, float dt,
-                                             int tmax_steps, unsigned long long *clamped)
-{
-    const int lane = threadIdx.x & 63;
-    if (known && mask_exact(emask, pi.W)) {
-        const uint64_t m = i0 == 0 ? mask_bits(emask, pi.W) : 0ull;
-        masks[0] = m;
-        masks[1] = masks[2] = masks[3] = 0;
-        if (clamped && ((m >> lane) & 1ull) && time_clamped(pi, lane, dt, tmax_steps))
-            atomicAdd(clamped, 1ull);
-        return m;
-    }
-    uint64_t any = 0;
-    for (int c = 0; c < 4; ++c) {
-        const int i = i0 + c * 64 + lane;
-        bool has = false;
-        if (i < pi.W) {
-            has = waypoint_has_hs(bs, (pi.g0 + i) * (int64_t)max_hs, max_hs);
-            if (has && clamped && time_clamped(pi, i, dt, tmax_steps))
-                atomicAdd(clamped, 1ull);
-        }
-        const uint64_t m = __ballot(has);
-        masks[c] = m;
-        any |= m;
-    }
-    return any;
-}
-
-__device__ __forceinline__ float lane_f(float x, int s)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), s));
-}
-
-// Tests NQ particle groups (lane = particle of each group) against every
-// stored half-space of the marked waypoints; v[q] becomes the AND of the
-// particle's tests (PUMP.cu:636-656, same float order).  Per waypoint, the
-// slots are loaded lane-parallel (lane h: slot h), the live ones found by a
-// ballot and broadcast by readlane, and the offsets of the NQ particles are
-// loaded together: one round of memory latency per marked waypoint.
-template <int NQ>
-__device__ __forceinline__ void particle_walk(bool (&v)[NQ], const int (&p)[NQ], int n_p,
-                                              const uint64_t *masks, int i0, const PathInfo &pi,
-                                              int max_hs, const float *__restrict__ as,
-                                              const float *__restrict__ bs,
-                                              const float *__restrict__ off, int n_mc,
-                                              int tmax_steps, float dt, float om)
-{
-    const int lane = threadIdx.x & 63;
-    float o[NQ][3];
-    int last_ti = -1; // consecutive waypoints often share a time index
-    for (int c = 0; c < 4; ++c) {
-        uint64_t m = masks[c];
-        while (m) {
-            const int l = __builtin_ctzll(m);
-            m &= m - 1;
-            const int i = i0 + c * 64 + l;
-            float time = pi.tbase + (float)i * pi.dtl;
-            int ti = (int)(time / dt);
-            ti = ti < 0 ? 0 : (ti >= tmax_steps ? tmax_steps - 1 : ti);
-            const int64_t head = (pi.g0 + i) * (int64_t)max_hs;
-            if (ti != last_ti) {
-                last_ti = ti;
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) {
-                    // p < n_p <= n_mc: in bounds whatever the particle's validity
-                    const float *op = off + ((int64_t)ti * n_mc + p[q]) * 3;
-                    const bool in = p[q] < n_p;
-                    o[q][0] = in ? op[0] : 0.0f;
-                    o[q][1] = in ? op[1] : 0.0f;
-                    o[q][2] = in ? op[2] : 0.0f;
-                }
-            }
-            for (int h0 = 0; h0 < max_hs; h0 += 64) {
-                const int h = h0 + lane;
-                float b = 1111.0f, a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-                if (h < max_hs) {
-                    b = bs[head + h];
-                    a0 = as[(head + h) * 3 + 0];
-                    a1 = as[(head + h) * 3 + 1];
-                    a2 = as[(head + h) * 3 + 2];
-                }
-                // the reference skips slots with b >= 1000 (PUMP.cu:642)
-                uint64_t sm = __ballot(!(b >= 1000.0f));
-                while (sm) {
-                    const int sl = __builtin_ctzll(sm);
-                    sm &= sm - 1;
-                    const float B = lane_f(b, sl), A0 = lane_f(a0, sl), A1 = lane_f(a1, sl),
-                                A2 = lane_f(a2, sl);
-#pragma unroll
-                    for (int q = 0; q < NQ; ++q) {
-                        float lhs = 0.0f;
-                        lhs = lhs + o[q][0] * A0;
-                        lhs = lhs + o[q][1] * A1;
-                        lhs = lhs + o[q][2] * A2;
-                        v[q] = v[q] && (lhs * om < B);
-                    }
-                }
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ int wave_sum(int x)
-{
-    for (int off = 32; off > 0; off >>= 1)
-        x += __shfl_down(x, off, 64);
-    return x;
-}
-
-// 0x01 in every byte of x that is nonzero, 0x00 elsewhere
-__device__ __forceinline__ uint32_t nonzero_bytes01(uint32_t x)
-{
-    return ((((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u) >> 7;
-}
-
-// index of this wave within the grid, as a wave-uniform (scalar) value
-__device__ __forceinline__ int64_t wave_index()
-{
-    return (int64_t)blockIdx.x * (blockDim.x >> 6) +
-           __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-}
-
-constexpr int kWalkGroups = 4; // particle groups of 64 per walk, wave-per-path kernel
-constexpr int kHsGroups = 8;   // the same for the batched half-space kernel (swept: 2, 4, 8)
-
 // One wave per new path (single launch; small wavefronts).
 template <bool kBits>
 __global__ __launch_bounds__(256) void k_propagate_wave(
@@ -680,33 +502,35 @@ __global__ __launch_bounds__(256) void k_propagate_wave(
 // Batched form for large wavefronts:
 //  (1) k_classify: lane = path; a path needs half-space tests iff its edge
 //      has a marked waypoint; those paths are appended to a list of path
-//      records (one atomic per wave) and their CP accumulators cleared;
-//  (2) k_copy_rows: the other paths' parent rows copied by a flat, fully
-//      parallel kernel that also counts them;
-//  (3) k_hs_groups: one wave per (listed path, 256 particles); the path's CP
-//      is finished by the last of its waves (a 64-bit atomic ticket packing
-//      the finished-wave count and the particle count).
-struct HsPath {
-    int64_t g0;     // first waypoint of the edge
-    uint64_t emask; // the edge's waypoint summary (when known)
-    float dtl, tbase;
-    int W, prev, w, pad;
-};
+//      records (one atomic per workgroup) and their marked waypoints counted
+//      per time index;
+//  (2) the time-binned pass (hsmc_bins.hip) scans that histogram;
+//  (3) k_copy_rows: parent rows copied to the child rows by a flat, fully
+//      parallel kernel that also counts the unlisted paths' particles;
+//  (4) the time-binned pass tests the listed paths' (waypoint, time index)
+//      pairs and ANDs the results into their rows, then finishes their CP.
+// When the pairs overflow the workspace, (4) is replaced by k_hs_groups: one
+// wave per (listed path, 512 particles); the path's CP is finished by the
+// last of its waves (a 64-bit atomic ticket packing the finished-wave count
+// and the particle count).
+constexpr int kClassifyBlock = 1024;
 
-__global__ __launch_bounds__(256) void k_classify(
+__global__ __launch_bounds__(kClassifyBlock) void k_classify(
     int n_new, const int64_t *__restrict__ wp_offsets, int w_fixed,
     const int *__restrict__ wf_edge, const int *__restrict__ wf_prev,
     const float *__restrict__ bs, const uint64_t *__restrict__ edge_mask, int max_hs,
     const float *__restrict__ times, const float *__restrict__ topts, float dt, int tmax_steps,
-    uint8_t *__restrict__ has_hs, HsPath *__restrict__ recs, int *__restrict__ hs_count,
-    unsigned long long *__restrict__ acc, unsigned long long *__restrict__ clamped)
+    uint8_t *__restrict__ has_hs, HsPath *__restrict__ recs, int *__restrict__ ctl,
+    int *__restrict__ hist, unsigned long long *__restrict__ acc,
+    unsigned long long *__restrict__ clamped)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    // wave-uniform trip count: every lane reaches the ballots below
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n_new;
-         base += stride) {
-        const int64_t w = base + lane;
+    __shared__ int wcount[kClassifyBlock / 64];
+    __shared__ int block_base;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int64_t stride = (int64_t)gridDim.x * kClassifyBlock;
+    // workgroup-uniform trip count: every thread reaches the barriers below
+    for (int64_t base = (int64_t)blockIdx.x * kClassifyBlock; base < n_new; base += stride) {
+        const int64_t w = base + threadIdx.x;
         bool has = false;
         int n_clamped = 0;
         HsPath r;
@@ -717,24 +541,15 @@ __global__ __launch_bounds__(256) void k_classify(
             if (em0 != 0ull) {
                 r.prev = wf_prev[w];
                 const PathInfo pi = path_info(e, r.prev, wp_offsets, w_fixed, topts, times);
-                int scan_from = 0;
-                r.emask = 0ull;
-                if (edge_mask) {
-                    const uint64_t em = mask_bits(em0, pi.W);
-                    const bool exact = mask_exact(em, pi.W);
-                    has = em != 0ull;
-                    r.emask = em;
-                    // waypoints the summary names exactly: bits 0..62, and bit 63 when exact
-                    for (uint64_t m = exact ? em : em & ~(1ull << 63); m; m &= m - 1)
-                        n_clamped += time_clamped(pi, __builtin_ctzll(m), dt, tmax_steps) ? 1 : 0;
-                    scan_from = exact ? pi.W : 63;
-                }
-                for (int i = scan_from; i < pi.W; ++i) {
-                    if (waypoint_has_hs(bs, (pi.g0 + i) * (int64_t)max_hs, max_hs)) {
-                        has = true;
-                        n_clamped += time_clamped(pi, i, dt, tmax_steps) ? 1 : 0;
-                    }
-                }
+                r.emask = edge_mask ? mask_bits(em0, pi.W) : 0ull;
+                for_each_marked(pi, edge_mask != nullptr, r.emask, bs, max_hs, [&](int i) {
+                    has = true;
+                    const int ti = time_index(pi, i, dt);
+                    const int tc = clamp_time(ti, tmax_steps);
+                    n_clamped += ti != tc ? 1 : 0;
+                    if (hist)
+                        atomicAdd(&hist[tc * kSubBins + (int)(w & (kSubBins - 1))], 1);
+                });
                 r.g0 = pi.g0;
                 r.W = pi.W;
                 r.dtl = pi.dtl;
@@ -747,26 +562,38 @@ __global__ __launch_bounds__(256) void k_classify(
         if (n_clamped && clamped)
             atomicAdd(clamped, (unsigned long long)n_clamped);
         const uint64_t m = __ballot(has);
-        if (m) {
-            int slot0 = 0;
-            if (lane == 0)
-                slot0 = atomicAdd(hs_count, __popcll(m));
-            slot0 = __builtin_amdgcn_readfirstlane(slot0);
-            if (has) {
-                const int slot = slot0 + __popcll(m & ((1ull << lane) - 1ull));
-                recs[slot] = r;
-                acc[slot] = 0ull;
+        if (lane == 0)
+            wcount[wid] = __popcll(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int total = 0;
+            for (int k = 0; k < kClassifyBlock / 64; ++k) {
+                const int c = wcount[k];
+                wcount[k] = total;
+                total += c;
             }
+            block_base = total ? atomicAdd(&ctl[kCtlListed], total) : 0;
         }
+        __syncthreads();
+        if (has) {
+            const int slot = block_base + wcount[wid] + __popcll(m & ((1ull << lane) - 1ull));
+            recs[slot] = r;
+            acc[slot] = 0ull;
+        }
+        __syncthreads(); // wcount and block_base are reused by the next pass
     }
 }
 
-// Copy unit per lane: 8-byte words (bit masks) or 16/4/1-byte chunks of a
-// byte row.  Each path is served by a segment of S lanes (S a power of two,
-// at most 64); the segment sums its count with shuffles of width S.
+// Copy unit per lane: 16-byte chunks (bit masks or byte rows), 8-byte words
+// or 4/1-byte chunks of a byte row.  Each path is served by a segment of S
+// lanes (S a power of two, at most 64); the segment sums its count with
+// shuffles of width S.  Listed paths' rows are copied too when the
+// time-binned pass runs (ctl given and no overflow); their CP is written by
+// that pass.
 template <int kUnit, bool kBits>
 __global__ __launch_bounds__(256) void k_copy_rows(int n_new, const int *__restrict__ wf_prev,
                                                    const uint8_t *__restrict__ has_hs,
+                                                   const int *__restrict__ ctl,
                                                    const uint8_t *__restrict__ src,
                                                    uint8_t *__restrict__ dst, int64_t dst_base,
                                                    int64_t row_bytes, int n_p, int seg,
@@ -777,16 +604,22 @@ __global__ __launch_bounds__(256) void k_copy_rows(int n_new, const int *__restr
     const int per_wave = 64 / seg;
     const int64_t n_waves = (int64_t)gridDim.x * 4;
     const int sub = lane / seg, j0 = lane % seg;
+    const bool copy_listed = ctl && ctl[kCtlOverflow] == 0;
     for (int64_t wb = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * per_wave; wb < n_new;
          wb += n_waves * per_wave) {
         const int64_t w = wb + sub;
         int count = 0;
-        const bool mine = (w < n_new) && !has_hs[w];
+        const bool listed = (w < n_new) && has_hs[w];
+        const bool mine = (w < n_new) && (!listed || copy_listed);
         if (mine) {
             const uint8_t *s = src + (int64_t)wf_prev[w] * row_bytes;
             uint8_t *d = dst + (dst_base + w) * row_bytes;
             for (int j = j0; j < units; j += seg) {
-                if (kUnit == 16) {
+                if (kUnit == 16 && kBits) {
+                    const uint4 x = reinterpret_cast<const uint4 *>(s)[j];
+                    reinterpret_cast<uint4 *>(d)[j] = x;
+                    count += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+                } else if (kUnit == 16) {
                     uint4 x = reinterpret_cast<const uint4 *>(s)[j];
                     x.x = nonzero_bytes01(x.x);
                     x.y = nonzero_bytes01(x.y);
@@ -811,7 +644,7 @@ __global__ __launch_bounds__(256) void k_copy_rows(int n_new, const int *__restr
         }
         for (int off = seg >> 1; off > 0; off >>= 1)
             count += __shfl_down(count, off, seg);
-        if (mine && j0 == 0 && path_cp) {
+        if (mine && !listed && j0 == 0 && path_cp) {
             const int invalid = kBits ? n_p - count : count;
             path_cp[cp_base + w] = ((float)invalid) / (float)n_p;
         }
@@ -820,16 +653,18 @@ __global__ __launch_bounds__(256) void k_copy_rows(int n_new, const int *__restr
 
 template <bool kBits, int NQ>
 __global__ __launch_bounds__(256) void k_hs_groups(
-    const HsPath *__restrict__ recs, const int *__restrict__ hs_count, int known,
+    const HsPath *__restrict__ recs, const int *__restrict__ ctl, int known,
     const float *__restrict__ as, const float *__restrict__ bs, float dt, int max_hs,
     const float *__restrict__ offsets_time, int tmax_steps, float om, int n_mc,
     const void *__restrict__ valid_src, void *__restrict__ valid_dst, int64_t dst_base, int n_p,
     float *__restrict__ path_cp, int64_t cp_base, unsigned long long *__restrict__ acc)
 {
+    if (ctl[kCtlOverflow] == 0)
+        return; // the time-binned pass serves this wavefront
     const int lane = threadIdx.x & 63;
     const int n_groups = (n_p + 63) >> 6;
     const int n_sg = (n_groups + NQ - 1) / NQ; // waves per listed path
-    const int64_t n_items = (int64_t)(*hs_count) * n_sg;
+    const int64_t n_items = (int64_t)ctl[kCtlListed] * n_sg;
     const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t item = wave_index(); item < n_items; item += n_waves) {
         const int slot = (int)(item / n_sg);
@@ -1063,48 +898,111 @@ static size_t round256(size_t x)
     return (x + 255) & ~(size_t)255;
 }
 
-// Workspace of the batched form: the list length, per-path flags, the
-// records of the paths that meet a half-space and one CP accumulator per
-// listed path.
-extern "C" size_t pump_propagate_workspace(int num_new_paths)
+// Workspace of the batched form, in order: control words and the per-time
+// histogram (cleared together), bin starts, bin cursors, per-path flags,
+// listed-path records, CP accumulators of the group form, chunks and binned
+// entries.  The entry capacity, 4 per path + 65536, covers typical
+// wavefronts; beyond it the group form runs instead of the binned one.
+struct BatchLayout {
+    size_t hist, bin_start, cursor, has, recs, acc, chunks, entries, total;
+    int64_t entry_cap, chunk_cap;
+    BatchLayout(int64_t n, int64_t T)
+    {
+        n = n > 0 ? n : 1;
+        T = T > 0 ? T : 1;
+        entry_cap = 4 * n + 65536;
+        chunk_cap = entry_cap / kBinChunk + T + 1;
+        const size_t bins = (size_t)T * kSubBins;
+        hist = 256;
+        bin_start = hist + round256(bins * 4);
+        cursor = bin_start + round256((bins + 1) * 4);
+        has = cursor + round256(bins * 4);
+        recs = has + round256((size_t)n);
+        acc = recs + round256((size_t)n * sizeof(HsPath));
+        chunks = acc + round256((size_t)n * 8);
+        entries = chunks + round256((size_t)chunk_cap * sizeof(BinChunk));
+        total = entries + (size_t)entry_cap * sizeof(BinEntry);
+    }
+};
+
+extern "C" size_t pump_propagate_workspace(int num_new_paths, int tmax_steps)
 {
-    const size_t n = num_new_paths > 0 ? (size_t)num_new_paths : 1;
-    return 256 + round256(n) + round256(n * sizeof(HsPath)) + n * sizeof(uint64_t);
+    return BatchLayout(num_new_paths, tmax_steps).total;
 }
 
-// Launches the propagate + CP step for a wavefront: batched three-kernel
-// form when a workspace is given and the wavefront is large, else one
-// wave-per-path (or, on request, workgroup-per-path) launch.
+__global__ __launch_bounds__(256) void k_zero_words(int *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+// Launches the propagate + CP step for a wavefront: the batched form when a
+// workspace is given and the wavefront is large, else one wave-per-path (or,
+// on request, workgroup-per-path) launch.
 template <bool kBits>
 static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, const int *wf_edge,
                             const int *wf_prev, const float *as, const float *bs,
                             const uint64_t *edge_mask, const float *times, const float *topts,
-                            float dt, int max_hs,
-                            const float *offsets_time, int tmax_steps, float om, int n_mc,
-                            const void *src, void *dst, int64_t dst_base, int n_p, float *path_cp,
-                            int64_t cp_base, unsigned long long *clamped, void *ws,
-                            size_t ws_bytes, hipStream_t s)
+                            float dt, int max_hs, const float *offsets_time, int tmax_steps,
+                            float om, int n_mc, const void *src, void *dst, int64_t dst_base,
+                            int n_p, float *path_cp, int64_t cp_base, unsigned long long *clamped,
+                            void *ws, size_t ws_bytes, hipStream_t s)
 {
     const int64_t row_bytes = kBits ? (int64_t)((n_p + 63) / 64) * 8 : (int64_t)n_p;
-    if (ws && ws_bytes >= pump_propagate_workspace(n_new) && n_new >= 4096 && !use_block_kernel()) {
+    const BatchLayout L(n_new, tmax_steps);
+    if (ws && ws_bytes >= L.total && n_new >= 4096 && !use_block_kernel()) {
         uint8_t *base = reinterpret_cast<uint8_t *>(ws);
-        int *count = reinterpret_cast<int *>(base);
-        uint8_t *has = base + 256;
-        HsPath *recs = reinterpret_cast<HsPath *>(has + round256((size_t)n_new));
-        unsigned long long *acc = reinterpret_cast<unsigned long long *>(
-            reinterpret_cast<uint8_t *>(recs) + round256((size_t)n_new * sizeof(HsPath)));
-        PUMP_HIP(hipMemsetAsync(count, 0, sizeof(int), s));
-        hipLaunchKernelGGL(k_classify, dim3(grid_for(n_new, 256, 65536)), dim3(256), 0, s, n_new,
-                           wp_offsets, W_fixed, wf_edge, wf_prev, bs, edge_mask, max_hs, times,
-                           topts, dt, tmax_steps, has, recs, count, acc, clamped);
+        int *ctl = reinterpret_cast<int *>(base);
+        int *hist = reinterpret_cast<int *>(base + L.hist);
+        uint8_t *has = base + L.has;
+        HsPath *recs = reinterpret_cast<HsPath *>(base + L.recs);
+        unsigned long long *acc = reinterpret_cast<unsigned long long *>(base + L.acc);
+        BinArgs B;
+        B.n_new = n_new;
+        B.n_p = n_p;
+        B.n_mc = n_mc;
+        B.tmax_steps = tmax_steps;
+        B.max_hs = max_hs;
+        B.known = edge_mask ? 1 : 0;
+        B.dt = dt;
+        B.om = om;
+        B.as = as;
+        B.bs = bs;
+        B.offsets_time = offsets_time;
+        B.recs = recs;
+        B.ctl = ctl;
+        B.hist = hist;
+        B.bin_start = reinterpret_cast<int *>(base + L.bin_start);
+        B.cursor = reinterpret_cast<int *>(base + L.cursor);
+        B.chunks = reinterpret_cast<BinChunk *>(base + L.chunks);
+        B.entries = reinterpret_cast<BinEntry *>(base + L.entries);
+        B.entry_cap = L.entry_cap;
+        B.chunk_cap = L.chunk_cap;
+        B.valid_dst = dst;
+        B.dst_base = dst_base;
+        B.path_cp = path_cp;
+        B.cp_base = cp_base;
+        // control words + histogram, cleared by a kernel (safe under capture)
+        const int64_t n_zero = (int64_t)(L.bin_start / 4);
+        hipLaunchKernelGGL(k_zero_words, dim3(grid_for(n_zero, 256, 1024)), dim3(256), 0, s, ctl,
+                           n_zero);
         PUMP_LAUNCHED();
+        hipLaunchKernelGGL(k_classify, dim3(grid_for(n_new, kClassifyBlock, 2048)),
+                           dim3(kClassifyBlock), 0, s, n_new, wp_offsets, W_fixed, wf_edge,
+                           wf_prev, bs, edge_mask, max_hs, times, topts, dt, tmax_steps, has,
+                           recs, ctl, hist, acc, clamped);
+        PUMP_LAUNCHED();
+        int r = bins_scan(B, s);
+        if (r)
+            return r;
         // copy unit and segment width
         const uintptr_t al = ((uintptr_t)src | (uintptr_t)dst);
         int unit = 1;
-        if (kBits)
-            unit = 8;
-        else if (row_bytes % 16 == 0 && (al & 15) == 0)
+        if (row_bytes % 16 == 0 && (al & 15) == 0)
             unit = 16;
+        else if (kBits)
+            unit = 8;
         else if (row_bytes % 4 == 0 && (al & 3) == 0)
             unit = 4;
         const int units = (int)(row_bytes / unit);
@@ -1115,23 +1013,36 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
         const unsigned grid = grid_for(waves * 64, 256, 65536);
         const uint8_t *sb = (const uint8_t *)src;
         uint8_t *db = (uint8_t *)dst;
-        if (unit == 8)
-            hipLaunchKernelGGL((k_copy_rows<8, true>), dim3(grid), dim3(256), 0, s, n_new, wf_prev,
-                               has, sb, db, dst_base, row_bytes, n_p, seg, path_cp, cp_base);
+        const int *cc = ctl;
+        if (kBits && unit == 16)
+            hipLaunchKernelGGL((k_copy_rows<16, true>), dim3(grid), dim3(256), 0, s, n_new,
+                               wf_prev, has, cc, sb, db, dst_base, row_bytes, n_p, seg, path_cp,
+                               cp_base);
+        else if (kBits)
+            hipLaunchKernelGGL((k_copy_rows<8, true>), dim3(grid), dim3(256), 0, s, n_new,
+                               wf_prev, has, cc, sb, db, dst_base, row_bytes, n_p, seg, path_cp,
+                               cp_base);
         else if (unit == 16)
             hipLaunchKernelGGL((k_copy_rows<16, false>), dim3(grid), dim3(256), 0, s, n_new,
-                               wf_prev, has, sb, db, dst_base, row_bytes, n_p, seg, path_cp, cp_base);
+                               wf_prev, has, cc, sb, db, dst_base, row_bytes, n_p, seg, path_cp,
+                               cp_base);
         else if (unit == 4)
             hipLaunchKernelGGL((k_copy_rows<4, false>), dim3(grid), dim3(256), 0, s, n_new,
-                               wf_prev, has, sb, db, dst_base, row_bytes, n_p, seg, path_cp, cp_base);
+                               wf_prev, has, cc, sb, db, dst_base, row_bytes, n_p, seg, path_cp,
+                               cp_base);
         else
             hipLaunchKernelGGL((k_copy_rows<1, false>), dim3(grid), dim3(256), 0, s, n_new,
-                               wf_prev, has, sb, db, dst_base, row_bytes, n_p, seg, path_cp, cp_base);
+                               wf_prev, has, cc, sb, db, dst_base, row_bytes, n_p, seg, path_cp,
+                               cp_base);
         PUMP_LAUNCHED();
+        r = bins_finish(B, kBits, s);
+        if (r)
+            return r;
+        // group form, which runs only when the binned pass overflowed:
         // (listed paths) x (64*kHsGroups particles), one wave each, grid-stride
         const int64_t per_path = ((n_p + 63) / 64 + kHsGroups - 1) / kHsGroups;
         hipLaunchKernelGGL((k_hs_groups<kBits, kHsGroups>), dim3(wave_grid(n_new * per_path)),
-                           dim3(256), 0, s, (const HsPath *)recs, (const int *)count,
+                           dim3(256), 0, s, (const HsPath *)recs, (const int *)ctl,
                            edge_mask ? 1 : 0, as, bs, dt, max_hs, offsets_time, tmax_steps, om,
                            n_mc, src, dst, dst_base, n_p, path_cp, cp_base, acc);
         PUMP_LAUNCHED();

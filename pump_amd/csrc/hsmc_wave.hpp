@@ -1,0 +1,286 @@
+// hsmc_wave.hpp -- wave-level building blocks of the half-space Monte Carlo
+// step (PUMP.cu:613-705): per-path edge data, waypoint half-space masks and
+// the particle test loop.  Used by the wave-per-path, batched and time-binned
+// forms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace pump {
+namespace wave {
+
+struct PathInfo {
+    int64_t g0;
+    int W;
+    float dtl, tbase;
+};
+
+__device__ __forceinline__ PathInfo path_info(int e, int prev, const int64_t *wp_offsets,
+                                              int w_fixed, const float *topts,
+                                              const float *times)
+{
+    PathInfo pi;
+    pi.g0 = wp_offsets ? wp_offsets[e] : (int64_t)e * w_fixed;
+    pi.W = (int)((wp_offsets ? wp_offsets[e + 1] : pi.g0 + w_fixed) - pi.g0);
+    pi.dtl = topts[e] / (float)(pi.W - 1);
+    pi.tbase = times[prev];
+    return pi;
+}
+
+// time index of waypoint i: (int)((times[prev] + i*dtLocal)/dt), PUMP.cu:637-638
+__device__ __forceinline__ int time_index(const PathInfo &pi, int i, float dt)
+{
+    float time = pi.tbase + (float)i * pi.dtl;
+    return (int)(time / dt);
+}
+
+__device__ __forceinline__ bool time_clamped(const PathInfo &pi, int i, float dt, int tmax_steps)
+{
+    const int ti = time_index(pi, i, dt);
+    return ti < 0 || ti >= tmax_steps;
+}
+
+// the library reads row ti of the offsets table clamped into [0, tmax_steps)
+__device__ __forceinline__ int clamp_time(int ti, int tmax_steps)
+{
+    return ti < 0 ? 0 : (ti >= tmax_steps ? tmax_steps - 1 : ti);
+}
+
+// An edge's waypoint mask (bit k: waypoint k holds a stored half-space, bit
+// 63: waypoint 63 or a later one does) is exact when the edge has at most 64
+// waypoints or bit 63 is clear.
+__device__ __forceinline__ bool mask_exact(uint64_t emask, int W)
+{
+    return W <= 64 || !(emask >> 63);
+}
+
+// the summary's bits that name waypoints of a W-waypoint edge
+__device__ __forceinline__ uint64_t mask_bits(uint64_t emask, int W)
+{
+    return W >= 64 ? emask : emask & ((1ull << W) - 1ull);
+}
+
+// A stored half-space: the reference skips entries with b >= 1000
+// (PUMP.cu:642), so "holds a half-space" means some b that is not >= 1000.
+__device__ __forceinline__ bool waypoint_has_hs(const float *__restrict__ bs, int64_t head,
+                                                int max_hs)
+{
+    bool has = false;
+    for (int h = 0; h < max_hs; ++h)
+        has |= !(bs[head + h] >= 1000.0f);
+    return has;
+}
+
+// Calls f(i), in increasing i, for every waypoint i of a path's edge that
+// holds a stored half-space (lane-level: one path per lane).  The summary
+// names waypoints 0..62 exactly, and 63 too when exact; later waypoints of
+// an inexact summary, or all of them without one, are found by scanning bs.
+template <class F>
+__device__ __forceinline__ void for_each_marked(const PathInfo &pi, bool known, uint64_t emask,
+                                                const float *__restrict__ bs, int max_hs, F f)
+{
+    int scan_from = 0;
+    if (known) {
+        const uint64_t m0 = mask_bits(emask, pi.W);
+        const bool exact = mask_exact(m0, pi.W);
+        for (uint64_t m = exact ? m0 : m0 & ~(1ull << 63); m; m &= m - 1)
+            f(__builtin_ctzll(m));
+        scan_from = exact ? pi.W : 63;
+    }
+    for (int i = scan_from; i < pi.W; ++i)
+        if (waypoint_has_hs(bs, (pi.g0 + i) * (int64_t)max_hs, max_hs))
+            f(i);
+}
+
+// masks[c] bit l: waypoint i0+c*64+l holds a stored half-space (i0..i0+255),
+// from the edge's waypoint mask when it is known and exact, else by ballots
+// over bs.  Counts clamped time indices of marked waypoints into *clamped
+// when non-null.
+__device__ __forceinline__ uint64_t hs_masks(uint64_t *masks, const PathInfo &pi, int i0,
+                                             bool known, uint64_t emask, int max_hs,
+                                             const float *__restrict__ bs, float dt,
+                                             int tmax_steps, unsigned long long *clamped)
+{
+    const int lane = threadIdx.x & 63;
+    if (known && mask_exact(emask, pi.W)) {
+        const uint64_t m = i0 == 0 ? mask_bits(emask, pi.W) : 0ull;
+        masks[0] = m;
+        masks[1] = masks[2] = masks[3] = 0;
+        if (clamped && ((m >> lane) & 1ull) && time_clamped(pi, lane, dt, tmax_steps))
+            atomicAdd(clamped, 1ull);
+        return m;
+    }
+    uint64_t any = 0;
+    for (int c = 0; c < 4; ++c) {
+        const int i = i0 + c * 64 + lane;
+        bool has = false;
+        if (i < pi.W) {
+            has = waypoint_has_hs(bs, (pi.g0 + i) * (int64_t)max_hs, max_hs);
+            if (has && clamped && time_clamped(pi, i, dt, tmax_steps))
+                atomicAdd(clamped, 1ull);
+        }
+        const uint64_t m = __ballot(has);
+        masks[c] = m;
+        any |= m;
+    }
+    return any;
+}
+
+__device__ __forceinline__ float lane_f(float x, int s)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), s));
+}
+
+// Tests NQ particle groups (lane = particle of each group) against every
+// stored half-space of the marked waypoints; v[q] becomes the AND of the
+// particle's tests (PUMP.cu:636-656, same float order).  Per waypoint, the
+// slots are loaded lane-parallel (lane h: slot h), the live ones found by a
+// ballot and broadcast by readlane, and the offsets of the NQ particles are
+// loaded together: one round of memory latency per marked waypoint.
+template <int NQ>
+__device__ __forceinline__ void particle_walk(bool (&v)[NQ], const int (&p)[NQ], int n_p,
+                                              const uint64_t *masks, int i0, const PathInfo &pi,
+                                              int max_hs, const float *__restrict__ as,
+                                              const float *__restrict__ bs,
+                                              const float *__restrict__ off, int n_mc,
+                                              int tmax_steps, float dt, float om)
+{
+    const int lane = threadIdx.x & 63;
+    float o[NQ][3];
+    int last_ti = -1; // consecutive waypoints often share a time index
+    for (int c = 0; c < 4; ++c) {
+        uint64_t m = masks[c];
+        while (m) {
+            const int l = __builtin_ctzll(m);
+            m &= m - 1;
+            const int i = i0 + c * 64 + l;
+            float time = pi.tbase + (float)i * pi.dtl;
+            int ti = (int)(time / dt);
+            ti = ti < 0 ? 0 : (ti >= tmax_steps ? tmax_steps - 1 : ti);
+            const int64_t head = (pi.g0 + i) * (int64_t)max_hs;
+            if (ti != last_ti) {
+                last_ti = ti;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    // p < n_p <= n_mc: in bounds whatever the particle's validity
+                    const float *op = off + ((int64_t)ti * n_mc + p[q]) * 3;
+                    const bool in = p[q] < n_p;
+                    o[q][0] = in ? op[0] : 0.0f;
+                    o[q][1] = in ? op[1] : 0.0f;
+                    o[q][2] = in ? op[2] : 0.0f;
+                }
+            }
+            for (int h0 = 0; h0 < max_hs; h0 += 64) {
+                const int h = h0 + lane;
+                float b = 1111.0f, a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+                if (h < max_hs) {
+                    b = bs[head + h];
+                    a0 = as[(head + h) * 3 + 0];
+                    a1 = as[(head + h) * 3 + 1];
+                    a2 = as[(head + h) * 3 + 2];
+                }
+                // the reference skips slots with b >= 1000 (PUMP.cu:642)
+                uint64_t sm = __ballot(!(b >= 1000.0f));
+                while (sm) {
+                    const int sl = __builtin_ctzll(sm);
+                    sm &= sm - 1;
+                    const float B = lane_f(b, sl), A0 = lane_f(a0, sl), A1 = lane_f(a1, sl),
+                                A2 = lane_f(a2, sl);
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        float lhs = 0.0f;
+                        lhs = lhs + o[q][0] * A0;
+                        lhs = lhs + o[q][1] * A1;
+                        lhs = lhs + o[q][2] * A2;
+                        v[q] = v[q] && (lhs * om < B);
+                    }
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ int wave_sum(int x)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        x += __shfl_down(x, off, 64);
+    return x;
+}
+
+// 0x01 in every byte of x that is nonzero, 0x00 elsewhere
+__device__ __forceinline__ uint32_t nonzero_bytes01(uint32_t x)
+{
+    return ((((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u) >> 7;
+}
+
+// index of this wave within the grid, as a wave-uniform (scalar) value
+__device__ __forceinline__ int64_t wave_index()
+{
+    return (int64_t)blockIdx.x * (blockDim.x >> 6) +
+           __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+
+constexpr int kWalkGroups = 4; // particle groups of 64 per walk, wave-per-path kernel
+constexpr int kHsGroups = 8;   // the same for the batched half-space kernel (swept: 2, 4, 8)
+
+// A path that meets at least one stored half-space, as listed by the
+// batched classification pass.
+struct HsPath {
+    int64_t g0;     // first waypoint of the edge
+    uint64_t emask; // the edge's waypoint summary (when known)
+    float dtl, tbase;
+    int W, prev, w, pad;
+};
+
+// One (listed path, waypoint with a stored half-space) pair of the
+// time-binned form; wp is the waypoint's global index (as/bs row).
+struct BinEntry {
+    int w;  // path index within the wavefront
+    int ti; // clamped time index
+    int64_t wp;
+};
+
+// A run of at most kBinChunk entries sharing one time index.
+struct BinChunk {
+    int ti, begin, end;
+};
+
+constexpr int kBinChunk = 16;
+// Each time index is split into kSubBins counters (by path index) so that
+// the counting sort's atomics are spread over more addresses.
+constexpr int kSubBins = 16;
+
+// control words at the head of the batched workspace
+enum : int {
+    kCtlListed = 0,   // listed paths
+    kCtlEntries = 1,  // binned entries
+    kCtlOverflow = 2, // entries or chunks exceed the workspace: the group form runs
+    kCtlChunks = 3,   // binned chunks
+};
+
+} // namespace wave
+
+// Time-binned half-space pass (hsmc_bins.hip).  bins_scan turns the per-time
+// histogram into bin starts, cursors and chunks and sets the overflow flag;
+// bins_finish emits the entries, tests them and writes the listed paths' CP.
+struct BinArgs {
+    int n_new, n_p, n_mc, tmax_steps, max_hs, known;
+    float dt, om;
+    const float *as, *bs, *offsets_time;
+    const wave::HsPath *recs;
+    int *ctl, *hist, *bin_start, *cursor;
+    wave::BinChunk *chunks;
+    wave::BinEntry *entries;
+    int64_t entry_cap, chunk_cap;
+    void *valid_dst;
+    int64_t dst_base;
+    float *path_cp;
+    int64_t cp_base;
+};
+int bins_scan(const BinArgs &a, hipStream_t s);
+int bins_finish(const BinArgs &a, bool bits, hipStream_t s);
+
+} // namespace pump

@@ -266,15 +266,16 @@ def unpack_bits(words, n_p):
     return np.stack([np.unpackbits(w.view(np.uint8), bitorder="little")[:n_p] for w in words])
 
 
-@pytest.mark.parametrize("n_p,w_range", [(333, (2, 40)), (2000, (2, 40)), (512, (2, 40)),
-                                          (640, (40, 300))])
-def test_propagate_batched_form(n_p, w_range):
+@pytest.mark.parametrize("n_p,w_range,T", [(333, (2, 40), 120), (2000, (2, 40), 120),
+                                            (512, (2, 40), 3000), (640, (40, 300), 120)])
+def test_propagate_batched_form(n_p, w_range, T):
     """Large wavefronts (>= 4096 paths) with a workspace take the batched
-    classify / copy / half-space group form; results must equal the oracle,
-    with and without the per-edge waypoint summary (edges up to 300
-    waypoints exercise the summary's inexact case and waypoint groups past
-    256)."""
-    d = random_hsmc_inputs(21 + n_p, n_edges=3000, W=9, max_hs=5, n_p=n_p, n_mc=2048, T=120,
+    form: classify, copy, then the half-space tests binned by time index, or,
+    when the binned pairs overflow the workspace (the 40-300 waypoint case),
+    one wave per (path, particle group).  Results must equal the oracle, with
+    and without the per-edge waypoint summary (edges past 64 waypoints
+    exercise the summary's inexact case and waypoint groups past 256)."""
+    d = random_hsmc_inputs(21 + n_p, n_edges=3000, W=9, max_hs=5, n_p=n_p, n_mc=2048, T=T,
                            n_new=6000, n_parent=64, variable=True, w_range=w_range)
     # most edges hold no half-space, as in the real workloads
     keep = np.random.default_rng(1).uniform(size=d["off"].shape[0] - 1) < 0.15
@@ -286,7 +287,7 @@ def test_propagate_batched_form(n_p, w_range):
     mask = edge_mask_host(d["off"], d["bs"], d["max_hs"])
     assert (mask >> np.uint64(63)).any() == (w_range[1] > 64)
     g_mask = dev(mask)
-    ws_bytes = P.lib.pump_propagate_workspace(d["n_new"])
+    ws_bytes = P.lib.pump_propagate_workspace(d["n_new"], d["T"])
     ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
     npar, nn = d["n_parent"], d["n_new"]
     for m in (None, ptr(g_mask)):
@@ -302,6 +303,9 @@ def test_propagate_batched_form(n_p, w_range):
         assert np.array_equal(host(dst), ref_v[npar:])
         assert np.array_equal(host(cp), ref_cp[npar:])
         assert int(host(clamp)[0]) == ref_cl
+        # control words: listed paths, binned pairs, overflow flag
+        ctl = ws.numpy()[:16].view(np.int32)
+        assert ctl[0] > 0 and ctl[2] == (1 if w_range[1] > 64 else 0), ctl[:4]
         bsrc = dev(pack_bits(d["valid"][:npar], n_p))
         bdst = dev(np.zeros((nn, (n_p + 63) // 64), np.uint64))
         cpb = dev(np.zeros(nn, np.float32))
