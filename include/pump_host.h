@@ -175,6 +175,10 @@ int pump_find_optimal_path(float dt, const float *xs, int n_wp, float *split, in
 /* single-pair helpers (host) for tests */
 float pump_host_topt(const float *x0, const float *x1, float tmax);
 float pump_host_cost(float tau, const float *x0, const float *x1);
+/* pathPoint (2pBVP.cu:117-131) with the kernels' arithmetic */
+void pump_host_path_point(float t, float tau, const float *x0, const float *x1, float *x);
+/* the kernels' FP64 division a/b (reciprocal + residual steps), n pairs */
+void pump_host_div_rn(const double *a, const double *b, double *q, int64_t n);
 
 #ifdef __cplusplus
 }

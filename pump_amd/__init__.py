@@ -186,6 +186,8 @@ _SIGS = {
     "pump_timer_destroy": (None, [c_void_p]),
     "pump_host_topt": (c_float, [f32p, f32p, c_float]),
     "pump_host_cost": (c_float, [c_float, f32p, f32p]),
+    "pump_host_path_point": (None, [c_float, c_float, f32p, f32p, f32p]),
+    "pump_host_div_rn": (None, [S, S, S, c_int64]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

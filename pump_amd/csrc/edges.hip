@@ -31,11 +31,11 @@ constexpr int kTile = 256;
 // these bounds in some axis is separated from every segment's bounding box,
 // i.e. the reference's broad phase (collisionCheck.cu:104-112) rejects it for
 // every segment, so skipping it leaves every result unchanged.
-__device__ __forceinline__ void edge_bounds(const float *x0, const float *x1, float tau,
+__device__ __forceinline__ void edge_bounds(const float *x0, const float *x1, float tau, int d,
                                             float *lo, float *hi)
 {
     const double T = (double)tau;
-    for (int d = 0; d < 3; ++d) {
+    {
         const double p0 = x0[d], v0 = x0[d + 3], p1 = x1[d], v1 = x1[d + 3];
         const double c3 = (2.0 * p0 - 2.0 * p1 + (v0 + v1) * T) / (T * T * T);
         const double c2 = (3.0 * p0 - 3.0 * p1 + (2.0 * v0 + v1) * T) / (T * T);
@@ -80,6 +80,8 @@ __global__ __launch_bounds__(kTile) void k_edges(
     __shared__ ObsLds o;
     __shared__ float x0s[kEdgesPerBlock][kDim], x1s[kEdgesPerBlock][kDim];
     __shared__ float taus[kEdgesPerBlock];
+    __shared__ TauPowers tps[kEdgesPerBlock]; // per-edge powers, reciprocals and dt
+    __shared__ float dts[kEdgesPerBlock];
     __shared__ float eb_lo[kEdgesPerBlock][3], eb_hi[kEdgesPerBlock][3];
     __shared__ unsigned long long cand[kEdgesPerBlock][kMaskWords];
     __shared__ int64_t off[kEdgesPerBlock + 1];
@@ -109,9 +111,17 @@ __global__ __launch_bounds__(kTile) void k_edges(
         ok[q] = 1;
     }
     __syncthreads();
+    for (int q = t; q < ne; q += kTile) {
+        tps[q] = tau_powers(taus[q]);
+        dts[q] = taus[q] / (float)(off[q + 1] - off[q] - 1);
+    }
+    __syncthreads();
     if (kCheck) {
-        for (int q = t; q < ne; q += kTile)
-            edge_bounds(x0s[q], x1s[q], taus[q], eb_lo[q], eb_hi[q]);
+        // one lane per (edge, axis)
+        for (int q = t; q < ne * 3; q += kTile) {
+            const int le = q / 3, d = q - 3 * le;
+            edge_bounds(x0s[le], x1s[le], taus[le], d, eb_lo[le], eb_hi[le]);
+        }
         __syncthreads();
         // candidate boxes per edge: not separated from the edge bounds
         for (int q = t; q < ne * n_masked; q += kTile) {
@@ -134,8 +144,8 @@ __global__ __launch_bounds__(kTile) void k_edges(
             k = (int)(g - off[le]);
             W = (int)(off[le + 1] - off[le]);
             float x[kDim];
-            TauPowers P = tau_powers(taus[le]);
-            waypoint(k, W, P, x0s[le], x1s[le], x);
+            const TauPowers P = tps[le];
+            waypoint_dt(k, W, P, dts[le], x0s[le], x1s[le], x);
             for (int d = 0; d < kDim; ++d)
                 wp[t][d] = x[d];
             // the last lane's waypoint is the first of the next tile

@@ -565,6 +565,18 @@ extern "C" float pump_host_cost(float tau, const float *x0, const float *x1)
     return cost_eval(c, tau);
 }
 
+extern "C" void pump_host_path_point(float t, float tau, const float *x0, const float *x1,
+                                     float *x)
+{
+    path_point(t, tau_powers(tau), x0, x1, x);
+}
+
+extern "C" void pump_host_div_rn(const double *a, const double *b, double *q, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i)
+        q[i] = div_rn(a[i], b[i], 1.0 / b[i]);
+}
+
 extern "C" int pump_find_optimal_path(float dt, const float *xs, int n_wp, float *split, int cap,
                                       float *total_time)
 {

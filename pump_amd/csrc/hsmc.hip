@@ -206,11 +206,7 @@ __device__ __forceinline__ bool particle_passes(const HsEntry &en, const float *
                                                 int n_mc, int p, float om)
 {
     const float *o = off + ((int64_t)en.ti * n_mc + p) * 3;
-    float lhs = 0.0f;
-    lhs = lhs + o[0] * en.a0;
-    lhs = lhs + o[1] * en.a1;
-    lhs = lhs + o[2] * en.a2;
-    return lhs * om < en.b;
+    return hs_pass(o[0], o[1], o[2], en.a0, en.a1, en.a2, om, en.b);
 }
 
 // Byte-validity propagate (+ optional fused CP).  Reads parent rows of
@@ -509,10 +505,8 @@ __global__ __launch_bounds__(256) void k_propagate_wave(
 //      parallel kernel that also counts the unlisted paths' particles;
 //  (4) the time-binned pass tests the listed paths' (waypoint, time index)
 //      pairs and ANDs the results into their rows, then finishes their CP.
-// When the pairs overflow the workspace, (4) is replaced by k_hs_groups: one
-// wave per (listed path, 512 particles); the path's CP is finished by the
-// last of its waves (a 64-bit atomic ticket packing the finished-wave count
-// and the particle count).
+// When the pairs overflow the workspace, (4) is replaced by the group form:
+// one wave per (listed path, 512 particles).
 constexpr int kClassifyBlock = 1024;
 
 __global__ __launch_bounds__(kClassifyBlock) void k_classify(
@@ -586,10 +580,62 @@ __global__ __launch_bounds__(kClassifyBlock) void k_classify(
 
 // Copy unit per lane: 16-byte chunks (bit masks or byte rows), 8-byte words
 // or 4/1-byte chunks of a byte row.  Each path is served by a segment of S
-// lanes (S a power of two, at most 64); the segment sums its count with
-// shuffles of width S.  Listed paths' rows are copied too when the
+// lanes (S a power of two, at most 64); a segment takes kCopyRows paths per
+// pass so that their loads are in flight together, and sums each path's count
+// with shuffles of width S.  Listed paths' rows are copied too when the
 // time-binned pass runs (ctl given and no overflow); their CP is written by
 // that pass.
+constexpr int kCopyRows = 4;
+
+template <int kUnit>
+struct CopyWord;
+template <>
+struct CopyWord<16> {
+    using T = uint4;
+};
+template <>
+struct CopyWord<8> {
+    using T = uint64_t;
+};
+template <>
+struct CopyWord<4> {
+    using T = uint32_t;
+};
+template <>
+struct CopyWord<1> {
+    using T = uint8_t;
+};
+
+// byte rows: normalise to 0/1 and count zeros; bit rows: count ones
+template <bool kBits>
+__device__ __forceinline__ int copy_count(uint4 &x)
+{
+    if (kBits)
+        return __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
+    x.x = nonzero_bytes01(x.x);
+    x.y = nonzero_bytes01(x.y);
+    x.z = nonzero_bytes01(x.z);
+    x.w = nonzero_bytes01(x.w);
+    return 16 - __popc(x.x) - __popc(x.y) - __popc(x.z) - __popc(x.w);
+}
+template <bool kBits>
+__device__ __forceinline__ int copy_count(uint64_t &x)
+{
+    return __popcll(x); // 8-byte units are used for bit rows only
+}
+template <bool kBits>
+__device__ __forceinline__ int copy_count(uint32_t &x)
+{
+    x = nonzero_bytes01(x);
+    return 4 - __popc(x);
+}
+template <bool kBits>
+__device__ __forceinline__ int copy_count(uint8_t &x)
+{
+    x = x ? 1 : 0;
+    return 1 - x;
+}
+
 template <int kUnit, bool kBits>
 __global__ __launch_bounds__(256) void k_copy_rows(int n_new, const int *__restrict__ wf_prev,
                                                    const uint8_t *__restrict__ has_hs,
@@ -599,137 +645,50 @@ __global__ __launch_bounds__(256) void k_copy_rows(int n_new, const int *__restr
                                                    int64_t row_bytes, int n_p, int seg,
                                                    float *__restrict__ path_cp, int64_t cp_base)
 {
+    using T = typename CopyWord<kUnit>::T;
+    constexpr int R = kCopyRows;
     const int lane = threadIdx.x & 63;
     const int units = (int)(row_bytes / kUnit);
     const int per_wave = 64 / seg;
-    const int64_t n_waves = (int64_t)gridDim.x * 4;
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     const int sub = lane / seg, j0 = lane % seg;
     const bool copy_listed = ctl && ctl[kCtlOverflow] == 0;
-    for (int64_t wb = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * per_wave; wb < n_new;
-         wb += n_waves * per_wave) {
-        const int64_t w = wb + sub;
-        int count = 0;
-        const bool listed = (w < n_new) && has_hs[w];
-        const bool mine = (w < n_new) && (!listed || copy_listed);
-        if (mine) {
-            const uint8_t *s = src + (int64_t)wf_prev[w] * row_bytes;
-            uint8_t *d = dst + (dst_base + w) * row_bytes;
-            for (int j = j0; j < units; j += seg) {
-                if (kUnit == 16 && kBits) {
-                    const uint4 x = reinterpret_cast<const uint4 *>(s)[j];
-                    reinterpret_cast<uint4 *>(d)[j] = x;
-                    count += __popc(x.x) + __popc(x.y) + __popc(x.z) + __popc(x.w);
-                } else if (kUnit == 16) {
-                    uint4 x = reinterpret_cast<const uint4 *>(s)[j];
-                    x.x = nonzero_bytes01(x.x);
-                    x.y = nonzero_bytes01(x.y);
-                    x.z = nonzero_bytes01(x.z);
-                    x.w = nonzero_bytes01(x.w);
-                    reinterpret_cast<uint4 *>(d)[j] = x;
-                    count += 16 - __popc(x.x) - __popc(x.y) - __popc(x.z) - __popc(x.w);
-                } else if (kUnit == 8) {
-                    const uint64_t x = reinterpret_cast<const uint64_t *>(s)[j];
-                    reinterpret_cast<uint64_t *>(d)[j] = x;
-                    count += __popcll(x);
-                } else if (kUnit == 4) {
-                    const uint32_t x = nonzero_bytes01(reinterpret_cast<const uint32_t *>(s)[j]);
-                    reinterpret_cast<uint32_t *>(d)[j] = x;
-                    count += 4 - __popc(x);
-                } else {
-                    const uint8_t x = s[j] ? 1 : 0;
-                    d[j] = x;
-                    count += 1 - x;
+    for (int64_t wb = wave_index() * per_wave * R; wb < n_new; wb += n_waves * per_wave * R) {
+        const T *sp[R];
+        T *dp[R];
+        bool mine[R], listed[R];
+        int count[R];
+        int64_t w[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            w[r] = wb + r * per_wave + sub;
+            listed[r] = (w[r] < n_new) && has_hs[w[r]];
+            mine[r] = (w[r] < n_new) && (!listed[r] || copy_listed);
+            const int prev = mine[r] ? wf_prev[w[r]] : 0;
+            sp[r] = reinterpret_cast<const T *>(src + (int64_t)prev * row_bytes);
+            dp[r] = reinterpret_cast<T *>(dst + (dst_base + (mine[r] ? w[r] : 0)) * row_bytes);
+            count[r] = 0;
+        }
+        for (int j = j0; j < units; j += seg) {
+            T x[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (mine[r])
+                    x[r] = sp[r][j];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (mine[r]) {
+                    count[r] += copy_count<kBits>(x[r]);
+                    dp[r][j] = x[r];
                 }
-            }
         }
-        for (int off = seg >> 1; off > 0; off >>= 1)
-            count += __shfl_down(count, off, seg);
-        if (mine && !listed && j0 == 0 && path_cp) {
-            const int invalid = kBits ? n_p - count : count;
-            path_cp[cp_base + w] = ((float)invalid) / (float)n_p;
-        }
-    }
-}
-
-template <bool kBits, int NQ>
-__global__ __launch_bounds__(256) void k_hs_groups(
-    const HsPath *__restrict__ recs, const int *__restrict__ ctl, int known,
-    const float *__restrict__ as, const float *__restrict__ bs, float dt, int max_hs,
-    const float *__restrict__ offsets_time, int tmax_steps, float om, int n_mc,
-    const void *__restrict__ valid_src, void *__restrict__ valid_dst, int64_t dst_base, int n_p,
-    float *__restrict__ path_cp, int64_t cp_base, unsigned long long *__restrict__ acc)
-{
-    if (ctl[kCtlOverflow] == 0)
-        return; // the time-binned pass serves this wavefront
-    const int lane = threadIdx.x & 63;
-    const int n_groups = (n_p + 63) >> 6;
-    const int n_sg = (n_groups + NQ - 1) / NQ; // waves per listed path
-    const int64_t n_items = (int64_t)ctl[kCtlListed] * n_sg;
-    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t item = wave_index(); item < n_items; item += n_waves) {
-        const int slot = (int)(item / n_sg);
-        const int sg = (int)(item - (int64_t)slot * n_sg);
-        const HsPath r = recs[slot];
-        PathInfo pi;
-        pi.g0 = r.g0;
-        pi.W = r.W;
-        pi.dtl = r.dtl;
-        pi.tbase = r.tbase;
-        bool v[NQ];
-        int p[NQ];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int grp = sg * NQ + q;
-            p[q] = grp * 64 + lane;
-            if (kBits) {
-                const uint64_t word =
-                    grp < n_groups
-                        ? ((const uint64_t *)valid_src)[(int64_t)r.prev * n_groups + grp]
-                        : 0ull;
-                v[q] = (p[q] < n_p) && ((word >> lane) & 1ull);
-            } else {
-                v[q] = (p[q] < n_p) &&
-                       ((const uint8_t *)valid_src)[(int64_t)r.prev * n_p + p[q]] != 0;
-            }
-        }
-        for (int i0 = 0; i0 < pi.W; i0 += 256) {
-            uint64_t masks[4];
-            const uint64_t any = hs_masks(masks, pi, i0, known != 0, r.emask, max_hs, bs, dt,
-                                          tmax_steps, (unsigned long long *)nullptr);
-            uint64_t alive = 0;
-#pragma unroll
-            for (int q = 0; q < NQ; ++q)
-                alive |= __ballot(v[q]);
-            if (any && alive)
-                particle_walk<NQ>(v, p, n_p, masks, i0, pi, max_hs, as, bs, offsets_time, n_mc,
-                                  tmax_steps, dt, om);
-        }
-        int count = 0; // valid (bits) or invalid (bytes) particles of this wave
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int grp = sg * NQ + q;
-            const uint64_t out = __ballot(v[q]);
-            if (grp < n_groups) {
-                if (kBits) {
-                    if (lane == q)
-                        ((uint64_t *)valid_dst)[(dst_base + r.w) * (int64_t)n_groups + grp] = out;
-                    count += __popcll(out);
-                } else {
-                    if (p[q] < n_p)
-                        ((uint8_t *)valid_dst)[(dst_base + r.w) * (int64_t)n_p + p[q]] =
-                            v[q] ? 1 : 0;
-                    const int in_group = (n_p - grp * 64) < 64 ? (n_p - grp * 64) : 64;
-                    count += in_group - __popcll(out);
-                }
-            }
-        }
-        if (lane == 0) {
-            const unsigned long long old =
-                atomicAdd(&acc[slot], (1ull << 32) | (unsigned long long)count);
-            if ((int)(old >> 32) + 1 == n_sg && path_cp) {
-                const int total = (int)(old & 0xffffffffull) + count;
-                const int invalid = kBits ? n_p - total : total;
-                path_cp[cp_base + r.w] = ((float)invalid) / (float)n_p;
+        for (int r = 0; r < R; ++r) {
+            for (int off = seg >> 1; off > 0; off >>= 1)
+                count[r] += __shfl_down(count[r], off, seg);
+            if (mine[r] && !listed[r] && j0 == 0 && path_cp) {
+                const int invalid = kBits ? n_p - count[r] : count[r];
+                path_cp[cp_base + w[r]] = ((float)invalid) / (float)n_p;
             }
         }
     }
@@ -898,13 +857,14 @@ static size_t round256(size_t x)
     return (x + 255) & ~(size_t)255;
 }
 
-// Workspace of the batched form, in order: control words and the per-time
-// histogram (cleared together), bin starts, bin cursors, per-path flags,
-// listed-path records, CP accumulators of the group form, chunks and binned
-// entries.  The entry capacity, 4 per path + 65536, covers typical
-// wavefronts; beyond it the group form runs instead of the binned one.
+// Workspace of the batched form, in order: control words, the sub-binned
+// per-time histogram and the sub-bin cursors (cleared together), time-index
+// starts, per-path flags, listed-path records, CP accumulators of the group
+// form, chunks and binned entries.  The entry capacity, 4 per path + 65536,
+// covers typical wavefronts; beyond it the group form runs instead of the
+// binned one.
 struct BatchLayout {
-    size_t hist, bin_start, cursor, has, recs, acc, chunks, entries, total;
+    size_t hist, cursor, tstart, has, recs, acc, chunks, entries, total;
     int64_t entry_cap, chunk_cap;
     BatchLayout(int64_t n, int64_t T)
     {
@@ -914,9 +874,9 @@ struct BatchLayout {
         chunk_cap = entry_cap / kBinChunk + T + 1;
         const size_t bins = (size_t)T * kSubBins;
         hist = 256;
-        bin_start = hist + round256(bins * 4);
-        cursor = bin_start + round256((bins + 1) * 4);
-        has = cursor + round256(bins * 4);
+        cursor = hist + round256(bins * 4);
+        tstart = cursor + round256(bins * 4);
+        has = tstart + round256((size_t)(T + 1) * 4);
         recs = has + round256((size_t)n);
         acc = recs + round256((size_t)n * sizeof(HsPath));
         chunks = acc + round256((size_t)n * 8);
@@ -971,20 +931,23 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
         B.bs = bs;
         B.offsets_time = offsets_time;
         B.recs = recs;
+        B.acc = acc;
         B.ctl = ctl;
         B.hist = hist;
-        B.bin_start = reinterpret_cast<int *>(base + L.bin_start);
         B.cursor = reinterpret_cast<int *>(base + L.cursor);
+        B.tstart = reinterpret_cast<int *>(base + L.tstart);
         B.chunks = reinterpret_cast<BinChunk *>(base + L.chunks);
         B.entries = reinterpret_cast<BinEntry *>(base + L.entries);
         B.entry_cap = L.entry_cap;
         B.chunk_cap = L.chunk_cap;
+        B.valid_src = src;
         B.valid_dst = dst;
         B.dst_base = dst_base;
         B.path_cp = path_cp;
         B.cp_base = cp_base;
-        // control words + histogram, cleared by a kernel (safe under capture)
-        const int64_t n_zero = (int64_t)(L.bin_start / 4);
+        // control words, histogram and cursors, cleared by a kernel (safe
+        // under graph capture)
+        const int64_t n_zero = (int64_t)(L.tstart / 4);
         hipLaunchKernelGGL(k_zero_words, dim3(grid_for(n_zero, 256, 1024)), dim3(256), 0, s, ctl,
                            n_zero);
         PUMP_LAUNCHED();
@@ -1009,7 +972,7 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
         int seg = 1;
         while (seg < units && seg < 64)
             seg <<= 1;
-        const int64_t waves = ((int64_t)n_new * seg + 63) / 64;
+        const int64_t waves = ((int64_t)n_new * seg + 64 * kCopyRows - 1) / (64 * kCopyRows);
         const unsigned grid = grid_for(waves * 64, 256, 65536);
         const uint8_t *sb = (const uint8_t *)src;
         uint8_t *db = (uint8_t *)dst;
@@ -1035,18 +998,7 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
                                wf_prev, has, cc, sb, db, dst_base, row_bytes, n_p, seg, path_cp,
                                cp_base);
         PUMP_LAUNCHED();
-        r = bins_finish(B, kBits, s);
-        if (r)
-            return r;
-        // group form, which runs only when the binned pass overflowed:
-        // (listed paths) x (64*kHsGroups particles), one wave each, grid-stride
-        const int64_t per_path = ((n_p + 63) / 64 + kHsGroups - 1) / kHsGroups;
-        hipLaunchKernelGGL((k_hs_groups<kBits, kHsGroups>), dim3(wave_grid(n_new * per_path)),
-                           dim3(256), 0, s, (const HsPath *)recs, (const int *)ctl,
-                           edge_mask ? 1 : 0, as, bs, dt, max_hs, offsets_time, tmax_steps, om,
-                           n_mc, src, dst, dst_base, n_p, path_cp, cp_base, acc);
-        PUMP_LAUNCHED();
-        return PUMP_OK;
+        return bins_finish(B, kBits, s);
     }
     if (use_block_kernel()) {
         if (kBits)

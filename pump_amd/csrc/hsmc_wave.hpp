@@ -12,6 +12,19 @@
 namespace pump {
 namespace wave {
 
+// One particle's test against one stored half-space, PUMP.cu:645-651:
+//   LHS = 0; LHS += o_d*a_d (d = 0, 1, 2); valid iff LHS*offsetMult < b.
+// The leading 0 + o0*a0 is dropped: it differs from o0*a0 only in the sign
+// of a zero LHS, and (-0)*om < b has the same truth value as (+0)*om < b.
+__device__ __forceinline__ bool hs_pass(float o0, float o1, float o2, float a0, float a1,
+                                        float a2, float om, float b)
+{
+    float lhs = o0 * a0;
+    lhs = lhs + o1 * a1;
+    lhs = lhs + o2 * a2;
+    return lhs * om < b;
+}
+
 struct PathInfo {
     int64_t g0;
     int W;
@@ -190,13 +203,8 @@ __device__ __forceinline__ void particle_walk(bool (&v)[NQ], const int (&p)[NQ],
                     const float B = lane_f(b, sl), A0 = lane_f(a0, sl), A1 = lane_f(a1, sl),
                                 A2 = lane_f(a2, sl);
 #pragma unroll
-                    for (int q = 0; q < NQ; ++q) {
-                        float lhs = 0.0f;
-                        lhs = lhs + o[q][0] * A0;
-                        lhs = lhs + o[q][1] * A1;
-                        lhs = lhs + o[q][2] * A2;
-                        v[q] = v[q] && (lhs * om < B);
-                    }
+                    for (int q = 0; q < NQ; ++q)
+                        v[q] = v[q] && hs_pass(o[q][0], o[q][1], o[q][2], A0, A1, A2, om, B);
                 }
             }
         }
@@ -263,18 +271,22 @@ enum : int {
 
 } // namespace wave
 
-// Time-binned half-space pass (hsmc_bins.hip).  bins_scan turns the per-time
-// histogram into bin starts, cursors and chunks and sets the overflow flag;
-// bins_finish emits the entries, tests them and writes the listed paths' CP.
+// Listed-path half-space pass of the batched form (hsmc_bins.hip).
+// bins_scan turns the sub-binned per-time histogram into time-index starts
+// and chunks and sets the overflow flag; bins_finish emits the entries,
+// tests them (or, on overflow, runs the group form) and writes the listed
+// paths' CP.
 struct BinArgs {
     int n_new, n_p, n_mc, tmax_steps, max_hs, known;
     float dt, om;
     const float *as, *bs, *offsets_time;
     const wave::HsPath *recs;
-    int *ctl, *hist, *bin_start, *cursor;
+    unsigned long long *acc;
+    int *ctl, *hist, *cursor, *tstart;
     wave::BinChunk *chunks;
     wave::BinEntry *entries;
     int64_t entry_cap, chunk_cap;
+    const void *valid_src;
     void *valid_dst;
     int64_t dst_base;
     float *path_cp;

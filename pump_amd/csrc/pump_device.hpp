@@ -153,6 +153,7 @@ __host__ __device__ __forceinline__ float topt_bisection(const CostTerms &c, flo
 struct TauPowers {
     float tau;
     double t2, t3; // tau^2 (exact), tau^3
+    double r2, r3; // correctly rounded 1/t2, 1/t3
 };
 
 __host__ __device__ __forceinline__ TauPowers tau_powers(float tau)
@@ -161,7 +162,24 @@ __host__ __device__ __forceinline__ TauPowers tau_powers(float tau)
     p.tau = tau;
     p.t2 = (double)tau * (double)tau;
     p.t3 = p.t2 * (double)tau;
+    p.r2 = 1.0 / p.t2;
+    p.r3 = 1.0 / p.t3;
     return p;
+}
+
+// Correctly rounded a/b from r = RN(1/b) (Markstein, IBM J. Res. Dev. 34(1),
+// 1990): q0 = RN(a r) is within 1.5 ulp of a/b; one residual step
+// q1 = RN(q0 + (a - b q0) r), with a - b q0 exact by FMA, brings it within
+// 1 ulp, and a second step from a quotient within 1 ulp and a reciprocal
+// within 1/2 ulp rounds correctly.  Valid while a/b, a and b stay clear of
+// overflow and subnormals (|a| < 1e300, 1e-300 < |b| < 1e300 here).
+__host__ __device__ __forceinline__ double div_rn(double a, double b, double r)
+{
+    const double q0 = a * r;
+    const double e0 = fma(-b, q0, a);
+    const double q1 = fma(e0, r, q0);
+    const double e1 = fma(-b, q1, a);
+    return fma(e1, r, q1);
 }
 
 __host__ __device__ __forceinline__ float pp_pos_xy(float t, double s2, double s3, const TauPowers &P,
@@ -171,7 +189,8 @@ __host__ __device__ __forceinline__ float pp_pos_xy(float t, double s2, double s
     float lin = t * v0 + p0;
     float c3 = 2.0f * p0 - 2.0f * p1 + (v0 + v1) * tau;
     float c2 = 3.0f * p0 - 3.0f * p1 + (2.0f * v0 + v1) * tau;
-    return (float)((double)lin + (s3 * (double)c3) / P.t3 - (s2 * (double)c2) / P.t2);
+    return (float)((double)lin + div_rn(s3 * (double)c3, P.t3, P.r3) -
+                   div_rn(s2 * (double)c2, P.t2, P.r2));
 }
 
 __host__ __device__ __forceinline__ float pp_vel_xy(float t, double s2, const TauPowers &P, float p0,
@@ -182,7 +201,8 @@ __host__ __device__ __forceinline__ float pp_vel_xy(float t, double s2, const Ta
     float f = -(2.0f * t * e);
     float c = 2.0f * p0 - 2.0f * p1 + v1 * tau;
     double q = 3.0 * s2 - (double)(4.0f * t * tau) + P.t2;
-    return (float)((double)f / P.t2 + (3.0 * s2 * (double)c) / P.t3 + ((double)v0 * q) / P.t2);
+    return (float)(div_rn((double)f, P.t2, P.r2) + div_rn(3.0 * s2 * (double)c, P.t3, P.r3) +
+                   div_rn((double)v0 * q, P.t2, P.r2));
 }
 
 __host__ __device__ __forceinline__ void path_point(float t, const TauPowers &P, const float *x0,
@@ -200,7 +220,7 @@ __host__ __device__ __forceinline__ void path_point(float t, const TauPowers &P,
                    (-3.0 * (double)x0[2] + 3.0 * (double)x1[2] - 2.0 * (double)x0[5] * (double)tau -
                     1.0 * (double)x1[5] * (double)tau);
         double d = s3 * (2.0 * (double)x0[2] - 2.0 * (double)x1[2] + (double)((x0[5] + x1[5]) * tau));
-        x[2] = (float)((a + b + c + d) / P.t3);
+        x[2] = (float)div_rn(a + b + c + d, P.t3, P.r3);
     }
     x[3] = pp_vel_xy(t, s2, P, x0[0], x0[3], x1[0], x1[3]);
     x[4] = pp_vel_xy(t, s2, P, x0[1], x0[4], x1[1], x1[4]);
@@ -211,14 +231,15 @@ __host__ __device__ __forceinline__ void path_point(float t, const TauPowers &P,
                     2.0 * (double)x1[5] * (double)tau);
         double c = s2 * (6.0 * (double)x0[2] - 6.0 * (double)x1[2] + 3.0 * (double)x0[5] * (double)tau +
                          3.0 * (double)x1[5] * (double)tau);
-        x[5] = (float)((a + b + c) / P.t3);
+        x[5] = (float)div_rn(a + b + c, P.t3, P.r3);
     }
 }
 
 // Waypoint k of an edge with W waypoints (findDiscretizedPath,
 // 2pBVP.cu:53-69): endpoints are copies, interior points at (tau/(W-1))*k.
-__host__ __device__ __forceinline__ void waypoint(int k, int W, const TauPowers &P, const float *x0,
-                                         const float *x1, float *x)
+// dt = tau/(W-1) is passed in when the caller has formed it once per edge.
+__host__ __device__ __forceinline__ void waypoint_dt(int k, int W, const TauPowers &P, float dt,
+                                            const float *x0, const float *x1, float *x)
 {
     if (k == 0) {
         for (int d = 0; d < kDim; ++d)
@@ -227,9 +248,14 @@ __host__ __device__ __forceinline__ void waypoint(int k, int W, const TauPowers 
         for (int d = 0; d < kDim; ++d)
             x[d] = x1[d];
     } else {
-        float dt = P.tau / (float)(W - 1);
         path_point(dt * (float)k, P, x0, x1, x);
     }
+}
+
+__host__ __device__ __forceinline__ void waypoint(int k, int W, const TauPowers &P, const float *x0,
+                                         const float *x1, float *x)
+{
+    waypoint_dt(k, W, P, P.tau / (float)(W - 1), x0, x1, x);
 }
 
 // ---------------------------------------------------------------------------

@@ -144,3 +144,92 @@ def test_configs_map_reference_parameters():
     assert P.lib.pump_maxd2(np.float32(0.5)) == np.float32(0.1 * np.float32(0.5) ** 2)
     b = configs.batch_config(16)
     assert all(1000 <= x["spec"].num <= 8000 for x in b)
+
+
+def _path_point_ieee(t, tau, x0, x1):
+    """The kernels' pathPoint expression tree (2pBVP.cu:117-131 with exact
+    powers) evaluated by numpy with IEEE division; vectorised over rows."""
+    f32, f64 = np.float32, np.float64
+    t, tau = t.astype(f32), tau.astype(f32)
+    td, taud = t.astype(f64), tau.astype(f64)
+    s2 = td * td
+    s3 = s2 * td
+    t2 = taud * taud
+    t3 = t2 * taud
+    out = np.zeros((t.shape[0], 6), f32)
+    for d in (0, 1):
+        p0, v0, p1, v1 = x0[:, d], x0[:, d + 3], x1[:, d], x1[:, d + 3]
+        lin = t * v0 + p0
+        c3 = f32(2) * p0 - f32(2) * p1 + (v0 + v1) * tau
+        c2 = f32(3) * p0 - f32(3) * p1 + (f32(2) * v0 + v1) * tau
+        out[:, d] = (lin.astype(f64) + (s3 * c3.astype(f64)) / t3
+                     - (s2 * c2.astype(f64)) / t2).astype(f32)
+        e = f32(3) * p0 - f32(3) * p1 + v1 * tau
+        f = -((f32(2) * t) * e)
+        c = f32(2) * p0 - f32(2) * p1 + v1 * tau
+        q = 3.0 * s2 - ((f32(4) * t) * tau).astype(f64) + t2
+        out[:, d + 3] = (f.astype(f64) / t2 + ((3.0 * s2) * c.astype(f64)) / t3
+                         + (v0.astype(f64) * q) / t2).astype(f32)
+    z0, vz0, z1, vz1 = (x.astype(f64) for x in (x0[:, 2], x0[:, 5], x1[:, 2], x1[:, 5]))
+    a = (t * x0[:, 5]).astype(f64) * t3
+    b = z0 * t3
+    c = (s2 * taud) * ((((-3.0 * z0) + 3.0 * z1) - (2.0 * vz0) * taud) - (1.0 * vz1) * taud)
+    dd = s3 * ((2.0 * z0 - 2.0 * z1) + ((x0[:, 5] + x1[:, 5]) * tau).astype(f64))
+    out[:, 2] = ((((a + b) + c) + dd) / t3).astype(f32)
+    a = vz0 * t3
+    b = (t * tau).astype(f64) * ((((-6.0 * z0) + 6.0 * z1) - (4.0 * vz0) * taud)
+                                 - (2.0 * vz1) * taud)
+    c = s2 * ((((6.0 * z0) - 6.0 * z1) + (3.0 * vz0) * taud) + (3.0 * vz1) * taud)
+    out[:, 5] = (((a + b) + c) / t3).astype(f32)
+    return out
+
+
+def test_path_point_division_is_correctly_rounded():
+    """pathPoint's FP64 divisions run as a reciprocal plus two exact FMA
+    residual steps; every waypoint must equal the one computed with IEEE
+    division, bit for bit, over taus spanning the bisection's range."""
+    rng = np.random.default_rng(11)
+    n = 200000
+    x0 = np.concatenate([rng.uniform(0, 1, (n, 3)), rng.uniform(-0.5, 0.5, (n, 3))], 1)
+    x1 = np.concatenate([rng.uniform(0, 1, (n, 3)), rng.uniform(-0.5, 0.5, (n, 3))], 1)
+    x0, x1 = x0.astype(np.float32), x1.astype(np.float32)
+    tau = np.exp(rng.uniform(np.log(0.01), np.log(5.0), n)).astype(np.float32)
+    k = rng.integers(1, 63, n)
+    t = ((tau / np.float32(63)) * k.astype(np.float32)).astype(np.float32)
+    ref = _path_point_ieee(t, tau, x0, x1)
+    got = np.zeros((n, 6), np.float32)
+    o = np.zeros(6, np.float32)
+    for i in range(n):
+        P.lib.pump_host_path_point(t[i], tau[i], P.fp(x0[i]), P.fp(x1[i]), P.fp(o))
+        got[i] = o
+    bad = np.nonzero(np.any(got.view(np.uint32) != ref.view(np.uint32), axis=1))[0]
+    assert bad.size == 0, (bad[:5], got[bad[:2]], ref[bad[:2]])
+
+
+def test_fp64_division_matches_ieee():
+    """The kernels' FP64 division (correctly rounded reciprocal, then two
+    residual steps with exact FMA) equals IEEE a/b: random operands across
+    the magnitudes pathPoint produces, divisors with all-ones significands,
+    near-exact quotients and quotients close to a rounding midpoint."""
+    rng = np.random.default_rng(5)
+    n = 400000
+    sig = rng.uniform(1, 2, n)
+    b = sig * 2.0 ** rng.integers(-30, 10, n)
+    a = rng.uniform(-1, 1, n) * 2.0 ** rng.integers(-40, 12, n)
+    # divisors whose significand is all ones
+    b[: n // 8] = np.ldexp(2.0 - 2.0 ** -52, rng.integers(-20, 8, n // 8))
+    # exact and near-midpoint quotients: a = q*b (rounded), q = random double
+    q = rng.uniform(1, 2, n) * 2.0 ** rng.integers(-20, 20, n)
+    sl = slice(n // 8, n // 2)
+    a[sl] = q[sl] * b[sl]
+    half = slice(n // 2, 3 * n // 4)
+    a[half] = (q[half] + np.spacing(q[half]) / 2) * b[half]
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    out = np.zeros(n)
+    P.lib.pump_host_div_rn(a.ctypes.data, b.ctypes.data, out.ctypes.data, n)
+    ref = a / b
+    assert np.array_equal(out.view(np.uint64), ref.view(np.uint64))
+    # the test set separates: the plain product a*RN(1/b) is not always exact
+    r = 1.0 / b
+    q0 = a * r
+    assert not np.array_equal(q0, ref)
