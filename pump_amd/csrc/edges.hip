@@ -4,14 +4,15 @@
 // plus the reference-shaped freeEdges and sampleFree (sampler.cu:61-88).
 //
 // Layout of the fused kernel: a workgroup owns a run of consecutive edges
-// (kEdgesPerBlock) and walks their waypoints in tiles of 256 -- one waypoint
-// per lane.  Each tile's waypoints go to LDS and to discMotions (consecutive
-// lanes write consecutive 24-byte records); segment (k, k+1) is then tested
-// by the lane holding waypoint k against the obstacle boxes, which are
-// staged once per workgroup in LDS as structure-of-arrays and read as
-// broadcasts.  Tiles overlap by one waypoint so every segment has both ends
-// in LDS.  An edge already known to be blocked skips its remaining segments,
-// and a lane stops scanning boxes at its first hit.
+// (kEdgesPerBlock); its per-edge data and the obstacle boxes (structure of
+// arrays, read as broadcasts) are staged once in LDS.  Each wave then walks
+// tiles of 64 waypoints, one per lane, with no further workgroup barrier:
+// the waypoints go to discMotions (consecutive lanes write consecutive
+// 24-byte records), segment (k, k+1) is tested by the lane holding waypoint
+// k, with the far end shuffled from the next lane, against the boxes that
+// are not separated from the tile's bounding box.  An edge already known to
+// be blocked skips its remaining segments, and a lane stops scanning boxes
+// at its first hit.
 #include "pump_device.hpp"
 #include "pump_internal.hpp"
 
@@ -20,56 +21,33 @@
 namespace pump {
 namespace {
 
-constexpr int kEdgesPerBlock = 32;
+constexpr int kEdgesPerBlock = 128;
 constexpr int kTile = 256;
-
-// Conservative position bounds of an edge: the waypoints lie on the cubic
-//   p(t) = p0 + v0 t - c2 t^2 + c3 t^3,  t in [0, tau]
-// (pathPoint's polynomial), so its extremes are at the endpoints or at zeros
-// of p'(t).  The bounds are inflated by a margin far above the rounding of a
-// waypoint (computed in double, rounded to float).  A box separated from
-// these bounds in some axis is separated from every segment's bounding box,
-// i.e. the reference's broad phase (collisionCheck.cu:104-112) rejects it for
-// every segment, so skipping it leaves every result unchanged.
-__device__ __forceinline__ void edge_bounds(const float *x0, const float *x1, float tau, int d,
-                                            float *lo, float *hi)
-{
-    const double T = (double)tau;
-    {
-        const double p0 = x0[d], v0 = x0[d + 3], p1 = x1[d], v1 = x1[d + 3];
-        const double c3 = (2.0 * p0 - 2.0 * p1 + (v0 + v1) * T) / (T * T * T);
-        const double c2 = (3.0 * p0 - 3.0 * p1 + (2.0 * v0 + v1) * T) / (T * T);
-        double mn = fmin(p0, p1), mx = fmax(p0, p1);
-        // p'(t) = v0 - 2 c2 t + 3 c3 t^2
-        const double A = 3.0 * c3, B = -2.0 * c2, C = v0;
-        double r[2];
-        int nr = 0;
-        if (fabs(A) > 1e-300) {
-            const double disc = B * B - 4.0 * A * C;
-            if (disc >= 0.0) {
-                const double sq = sqrt(disc);
-                r[nr++] = (-B - sq) / (2.0 * A);
-                r[nr++] = (-B + sq) / (2.0 * A);
-            }
-        } else if (fabs(B) > 1e-300) {
-            r[nr++] = -C / B;
-        }
-        for (int k = 0; k < nr; ++k) {
-            const double t = r[k];
-            if (t > 0.0 && t < T) {
-                const double v = p0 + v0 * t - c2 * t * t + c3 * t * t * t;
-                mn = fmin(mn, v);
-                mx = fmax(mx, v);
-            }
-        }
-        const double m = 1e-5 + 1e-5 * fmax(fabs(mn), fabs(mx));
-        lo[d] = (float)(mn - m);
-        hi[d] = (float)(mx + m);
-    }
-}
 
 constexpr int kMaskWords = kMaxObsLds / 64;
 
+__device__ __forceinline__ float wave_min(float x)
+{
+    for (int m = 32; m > 0; m >>= 1)
+        x = fminf(x, __shfl_xor(x, m, 64));
+    return x;
+}
+
+__device__ __forceinline__ float wave_max(float x)
+{
+    for (int m = 32; m > 0; m >>= 1)
+        x = fmaxf(x, __shfl_xor(x, m, 64));
+    return x;
+}
+
+// Each wave walks its own tiles of 64 waypoints (consecutive tiles overlap
+// by one waypoint, so a tile tests 63 segments); the segment's far end comes
+// from the next lane by a shuffle.  The candidate boxes of a tile are those
+// not separated from the bounding box of the tile's waypoints: every segment
+// the tile tests lies inside it, so a box separated from it (with the
+// reference's <= test, collisionCheck.cu:104-112) is rejected by the broad
+// phase for each of those segments and skipping it changes no result.
+// Waypoints are finite (tau >= the bisection's positive lower end).
 template <bool kCheck, bool kWrite>
 __global__ __launch_bounds__(kTile) void k_edges(
     const float *__restrict__ samples, const int *__restrict__ edge_src,
@@ -77,28 +55,23 @@ __global__ __launch_bounds__(kTile) void k_edges(
     const int64_t *__restrict__ wp_offsets, int w_fixed, const float *__restrict__ obstacles,
     int n_obs, float *__restrict__ disc, uint8_t *__restrict__ is_free)
 {
+    constexpr int kWaves = kTile / 64;
     __shared__ ObsLds o;
     __shared__ float x0s[kEdgesPerBlock][kDim], x1s[kEdgesPerBlock][kDim];
-    __shared__ float taus[kEdgesPerBlock];
-    __shared__ TauPowers tps[kEdgesPerBlock]; // per-edge powers, reciprocals and dt
-    __shared__ float dts[kEdgesPerBlock];
-    __shared__ float eb_lo[kEdgesPerBlock][3], eb_hi[kEdgesPerBlock][3];
-    __shared__ unsigned long long cand[kEdgesPerBlock][kMaskWords];
+    __shared__ TauPowers tps[kEdgesPerBlock]; // per-edge powers, reciprocals
+    __shared__ float dts[kEdgesPerBlock];     // and waypoint spacing
     __shared__ int64_t off[kEdgesPerBlock + 1];
     __shared__ int ok[kEdgesPerBlock];
-    __shared__ float wp[kTile][kDim];
 
     const int64_t e0 = (int64_t)blockIdx.x * kEdgesPerBlock;
     const int ne = (int)((n_edges - e0) < kEdgesPerBlock ? (n_edges - e0) : kEdgesPerBlock);
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(t >> 6);
     // boxes past the LDS capacity are tested for every segment (no mask)
     const int n_masked = n_obs < kMaxObsLds ? n_obs : kMaxObsLds;
     const int n_words = (n_masked + 63) >> 6;
-    if (kCheck) {
+    if (kCheck)
         stage_obstacles(o, obstacles, n_obs);
-        for (int q = t; q < ne * kMaskWords; q += kTile)
-            cand[q / kMaskWords][q % kMaskWords] = 0ull;
-    }
     for (int q = t; q < ne * kDim; q += kTile) {
         int le = q / kDim, d = q - kDim * (q / kDim);
         x0s[le][d] = samples[(int64_t)edge_src[e0 + le] * kDim + d];
@@ -106,64 +79,66 @@ __global__ __launch_bounds__(kTile) void k_edges(
     }
     for (int q = t; q <= ne; q += kTile)
         off[q] = wp_offsets ? wp_offsets[e0 + q] : (e0 + q) * (int64_t)w_fixed;
+    __syncthreads();
     for (int q = t; q < ne; q += kTile) {
-        taus[q] = topts_edge[e0 + q];
+        const float tau = topts_edge[e0 + q];
+        tps[q] = tau_powers(tau);
+        dts[q] = tau / (float)(off[q + 1] - off[q] - 1);
         ok[q] = 1;
     }
     __syncthreads();
-    for (int q = t; q < ne; q += kTile) {
-        tps[q] = tau_powers(taus[q]);
-        dts[q] = taus[q] / (float)(off[q + 1] - off[q] - 1);
-    }
-    __syncthreads();
-    if (kCheck) {
-        // one lane per (edge, axis)
-        for (int q = t; q < ne * 3; q += kTile) {
-            const int le = q / 3, d = q - 3 * le;
-            edge_bounds(x0s[le], x1s[le], taus[le], d, eb_lo[le], eb_hi[le]);
-        }
-        __syncthreads();
-        // candidate boxes per edge: not separated from the edge bounds
-        for (int q = t; q < ne * n_masked; q += kTile) {
-            const int le = q / n_masked, b = q - le * n_masked;
-            bool sep = false;
-            for (int d = 0; d < 3; ++d)
-                sep = sep || eb_hi[le][d] <= o.lo[d][b] || o.hi[d][b] <= eb_lo[le][d];
-            if (!sep)
-                atomicOr(&cand[le][b >> 6], 1ull << (b & 63));
-        }
-        __syncthreads();
-    }
 
     const int64_t g_begin = off[0], g_end = off[ne];
-    for (int64_t base = g_begin; base < g_end; base += kTile - 1) {
-        const int64_t g = base + t;
+    // tiles start every 63 waypoints; tile j belongs to wave j % kWaves
+    for (int64_t base = g_begin + (int64_t)wid * 63; base < g_end; base += (int64_t)kWaves * 63) {
+        const int64_t g = base + lane;
+        const bool have = g < g_end;
         int le = 0, k = 0, W = 0;
-        if (g < g_end) {
+        float x[kDim] = {0, 0, 0, 0, 0, 0};
+        if (have) {
             le = find_edge(off, ne, g);
             k = (int)(g - off[le]);
             W = (int)(off[le + 1] - off[le]);
-            float x[kDim];
             const TauPowers P = tps[le];
             waypoint_dt(k, W, P, dts[le], x0s[le], x1s[le], x);
-            for (int d = 0; d < kDim; ++d)
-                wp[t][d] = x[d];
-            // the last lane's waypoint is the first of the next tile
-            if (kWrite && (t < kTile - 1 || g + 1 >= g_end))
+            // lane 63's waypoint is lane 0's of the next tile
+            if (kWrite && (lane < 63 || g + 1 >= g_end))
                 for (int d = 0; d < kDim; ++d)
                     disc[g * kDim + d] = x[d];
         }
         if (!kCheck)
             continue;
-        __syncthreads();
-        bool test = (g < g_end) && (t < kTile - 1) && (k < W - 1);
-        if (test && ok[le]) {
-            const float *v = wp[t], *w = wp[t + 1];
+        // segment (k, k+1): far end from the next lane
+        float w[3];
+        for (int d = 0; d < 3; ++d)
+            w[d] = __shfl_down(x[d], 1, 64);
+        const bool test = have && lane < 63 && k < W - 1 && ok[le];
+        // candidate boxes of the tile: not separated from its waypoints' box
+        float tlo[3], thi[3];
+        for (int d = 0; d < 3; ++d) {
+            tlo[d] = wave_min(have ? x[d] : INFINITY);
+            thi[d] = wave_max(have ? x[d] : -INFINITY);
+        }
+        if (__ballot(test) == 0ull)
+            continue;
+        uint64_t cand[kMaskWords];
+        for (int wd = 0; wd < n_words; ++wd) {
+            const int b = wd * 64 + lane;
+            bool near = false;
+            if (b < n_masked) {
+                near = true;
+                for (int d = 0; d < 3; ++d)
+                    near = near && !(thi[d] <= o.lo[d][b] || o.hi[d][b] <= tlo[d]);
+            }
+            cand[wd] = __ballot(near);
+        }
+        if (test) {
+            const float v[3] = {x[0], x[1], x[2]};
             float bmin[3], bmax[3];
             seg_bbox(v, w, bmin, bmax);
             bool hit = false;
             for (int wd = 0; wd < n_words && !hit; ++wd) {
-                unsigned long long m = cand[le][wd];
+                uint64_t m = cand[wd];
                 while (m && !hit) {
                     const int b = wd * 64 + __builtin_ctzll(m);
                     m &= m - 1;
@@ -178,9 +153,8 @@ __global__ __launch_bounds__(kTile) void k_edges(
                 hit = segment_hits_box(v, w, bmin, bmax, lo, hi);
             }
             if (hit)
-                ok[le] = 0;
+                ok[le] = 0; // only zeros are written: concurrent writes agree
         }
-        __syncthreads();
     }
     if (kCheck) {
         __syncthreads();

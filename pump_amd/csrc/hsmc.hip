@@ -662,11 +662,13 @@ __global__ __launch_bounds__(256) void k_copy_rows(int n_new, const int *__restr
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             w[r] = wb + r * per_wave + sub;
-            listed[r] = (w[r] < n_new) && has_hs[w[r]];
-            mine[r] = (w[r] < n_new) && (!listed[r] || copy_listed);
-            const int prev = mine[r] ? wf_prev[w[r]] : 0;
+            const bool in = w[r] < n_new;
+            // flag and parent index load together (no dependent round trip)
+            const int prev = in ? wf_prev[w[r]] : 0;
+            listed[r] = in && has_hs[w[r]];
+            mine[r] = in && (!listed[r] || copy_listed);
             sp[r] = reinterpret_cast<const T *>(src + (int64_t)prev * row_bytes);
-            dp[r] = reinterpret_cast<T *>(dst + (dst_base + (mine[r] ? w[r] : 0)) * row_bytes);
+            dp[r] = reinterpret_cast<T *>(dst + (dst_base + (in ? w[r] : 0)) * row_bytes);
             count[r] = 0;
         }
         for (int j = j0; j < units; j += seg) {
