@@ -135,7 +135,7 @@ def main():
         cpu = cpu_baseline(rm, samples, obstacles, prm, args.cpu_seconds)
 
     traffic = None
-    tpath = os.path.join(HERE, "profiles", "edge_pass_c1_traffic.json")
+    tpath = os.path.join(HERE, "profiles", "edge_pass_num16000_traffic.json")
     if os.path.exists(tpath) and not args.num and args.config == 1:
         try:
             traffic = json.load(open(tpath)).get("bytes_per_launch")

@@ -54,28 +54,28 @@ def config(i, **over):
         prm = params(rn_perc=rn_perc(50, num), w_fixed=20, offset_mult=0.02, n_p=1000,
                      cp_target=0.01, cp_factor=2.0)
         ns = noise_spec(n_mc=2048)
-        name = f"c0_plain_num{num}_obs10_W20_p1000"
+        name = f"plain_num{num}_obs10_w20_p1000"
     elif i == 1:
         num = num or 16000
         spec = _spec(num, 0, 30, side=(0.03, 0.15))
         prm = params(rn_perc=rn_perc(80, num), w_fixed=0, wp_dt=0.05, w_cap=64, offset_mult=0.03,
                      n_p=2000, cp_target=0.01, cp_factor=2.0)
         ns = noise_spec(n_mc=2048)
-        name = f"c1_num{num}_obs30_Wceil64_p2000"
+        name = f"num{num}_obs30_wceil64_p2000"
     elif i == 3:
         num = num or 32000
         spec = _spec(num, 7, 200, kind=1, centre=(0.0, 1.0))
         prm = params(rn_perc=rn_perc(50, num), w_fixed=0, wp_dt=0.05, w_cap=64, offset_mult=0.02,
                      n_p=4096, cp_target=0.005, cp_factor=1.5, mccp_batch=64)
         ns = noise_spec(n_mc=4096)
-        name = f"c3_walls_num{num}_obs200_p4096"
+        name = f"walls_num{num}_obs200_p4096"
     elif i == 4:
         num = num or 64000
         spec = _spec(num, 4, 100, side=(0.02, 0.1))
         prm = params(rn_perc=rn_perc(50, num), w_fixed=0, wp_dt=0.05, w_cap=128,
                      offset_mult=0.04, n_p=8192, cp_target=0.01, cp_factor=2.0)
         ns = noise_spec(n_mc=8192)
-        name = f"c4_num{num}_obs100_W128_p8192"
+        name = f"num{num}_obs100_w128_p8192"
     else:
         raise ValueError(f"no configuration {i} (the batched set is batch_config())")
     for k, v in over.items():
@@ -99,7 +99,7 @@ def batch_config(n_instances=256, seed=2):
         prm = params(rn_perc=rn_perc(50, num), w_fixed=0, wp_dt=0.05, w_cap=64,
                      offset_mult=float(rng.uniform(0.01, 0.05)), n_p=n_p, cp_target=0.01,
                      cp_factor=2.0)
-        out.append(dict(name=f"c2_{k}_num{num}", spec=spec, params=prm,
+        out.append(dict(name=f"batch{k}_num{num}", spec=spec, params=prm,
                         noise=noise_spec(n_mc=2048)))
     return out
 

@@ -5,18 +5,17 @@
 //
 // cacheHalfspaces: a workgroup owns a run of edges and walks their waypoints
 // one per lane; obstacles are staged in LDS (any count: boxes past the LDS
-// capacity are read from global memory).
+// capacity are read from global memory).  It also records, per edge, which
+// waypoints store a half-space (a 64-bit summary).
 //
-// propagate: one workgroup per new path.  The (waypoint, half-space) pairs
-// that actually hold a half-space are gathered once into LDS together with
-// the waypoint's time index; the particles then stream over that short list.
-// The half-space data is uniform across the workgroup, the offsets row of a
-// time index is read coalesced (consecutive particles are consecutive
-// 12-byte records), and the child validity is written coalesced.  Particle
-// decisions use the reference's float operation order
-//   LHS = ((0 + o0*a0) + o1*a1) + o2*a2;  valid &= LHS*offsetMult < b
-// with no contraction, so they are bit-identical.  The AND over
-// (waypoint, half-space) pairs is order-independent.
+// propagate + CP comes in three forms, all with the reference's particle
+// decisions (float order of PUMP.cu:645-651, no contraction; the AND over
+// (waypoint, half-space) pairs is order-independent), hence identical rows:
+//   * one wave per new path (the drop-in entry point and small wavefronts);
+//   * one workgroup per new path with the path's (waypoint, half-space)
+//     pairs staged in LDS (selected with PUMP_HSMC_KERNEL=block);
+//   * the batched form for large wavefronts: classify, copy, and the
+//     time-binned half-space pass of hsmc_bins.hip.
 #include <cstdlib>
 
 #include "hsmc_wave.hpp"
