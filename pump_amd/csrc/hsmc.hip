@@ -262,22 +262,17 @@ __global__ __launch_bounds__(256) void k_propagate(
 }
 
 // calculateCP_HSMC: one workgroup per path
+// calculateCP_HSMC: one wave per path (grid-stride), 16-byte loads
 __global__ __launch_bounds__(256) void k_cp(int n_new, int64_t base, float *__restrict__ cp,
                                             const uint8_t *__restrict__ valid, int n_p)
 {
-    __shared__ int red[4];
-    const int w = blockIdx.x;
-    const uint8_t *row = valid + (base + w) * (int64_t)n_p;
-    int c = 0;
-    for (int p = threadIdx.x; p < n_p; p += blockDim.x)
-        c += row[p] ? 0 : 1;
-    for (int off = 32; off > 0; off >>= 1)
-        c += __shfl_down(c, off, 64);
-    if ((threadIdx.x & 63) == 0)
-        red[threadIdx.x >> 6] = c;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        cp[base + w] = ((float)(red[0] + red[1] + red[2] + red[3])) / (float)n_p;
+    const int lane = threadIdx.x & 63;
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t w = wave_index(); w < n_new; w += n_waves) {
+        const int c = wave_sum(invalid_bytes(valid + (base + w) * (int64_t)n_p, n_p, lane));
+        if (lane == 0)
+            cp[base + w] = ((float)c) / (float)n_p;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_gather_rows(const uint8_t *__restrict__ src,
@@ -449,7 +444,22 @@ __global__ __launch_bounds__(256) void k_propagate_wave(
                 uint8_t *dst = (uint8_t *)valid_dst + (dst_base + w) * (int64_t)n_p;
                 const bool words = ((n_p & 3) == 0) &&
                                    ((((uintptr_t)src) | ((uintptr_t)dst)) & 3) == 0;
-                if (!any && words) {
+                const bool wide = ((n_p & 15) == 0) &&
+                                  ((((uintptr_t)src) | ((uintptr_t)dst)) & 15) == 0;
+                if (!any && wide) {
+                    const uint4 *s16 = (const uint4 *)src;
+                    uint4 *d16 = (uint4 *)dst;
+                    for (int q = lane; q < (n_p >> 4); q += 64) {
+                        uint4 x = s16[q];
+                        x.x = nonzero_bytes01(x.x);
+                        x.y = nonzero_bytes01(x.y);
+                        x.z = nonzero_bytes01(x.z);
+                        x.w = nonzero_bytes01(x.w);
+                        d16[q] = x;
+                        if (last)
+                            count += 16 - __popc(x.x) - __popc(x.y) - __popc(x.z) - __popc(x.w);
+                    }
+                } else if (!any && words) {
                     const uint32_t *s4 = (const uint32_t *)src;
                     uint32_t *d4 = (uint32_t *)dst;
                     const int nq = n_p >> 2;
@@ -847,8 +857,8 @@ extern "C" int pump_calculate_cp_hsmc(int num_new_paths, int64_t path_base, floa
     if (num_new_paths == 0)
         return PUMP_OK;
     PUMP_REQUIRE(d_path_cp && d_valid);
-    hipLaunchKernelGGL(k_cp, dim3(num_new_paths), dim3(256), 0, as_stream(stream), num_new_paths,
-                       path_base, d_path_cp, d_valid, n_p);
+    hipLaunchKernelGGL(k_cp, dim3(wave_grid(num_new_paths)), dim3(256), 0, as_stream(stream),
+                       num_new_paths, path_base, d_path_cp, d_valid, n_p);
     PUMP_LAUNCHED();
     return PUMP_OK;
 }

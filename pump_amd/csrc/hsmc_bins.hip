@@ -373,22 +373,7 @@ __global__ __launch_bounds__(256) void k_cp_listed(const HsPath *__restrict__ re
             for (int k = lane; k < nw; k += 64)
                 count += __popcll(r[k]);
         } else {
-            const uint8_t *r = (const uint8_t *)valid_dst + row * n_p;
-            if (((n_p & 15) == 0) && (((uintptr_t)r) & 15) == 0) {
-                const uint4 *r16 = (const uint4 *)r;
-                for (int k = lane; k < (n_p >> 4); k += 64) {
-                    const uint4 x = r16[k];
-                    count += 16 - __popc(nonzero_bytes01(x.x)) - __popc(nonzero_bytes01(x.y)) -
-                             __popc(nonzero_bytes01(x.z)) - __popc(nonzero_bytes01(x.w));
-                }
-            } else if (((n_p & 3) == 0) && (((uintptr_t)r) & 3) == 0) {
-                const uint32_t *r4 = (const uint32_t *)r;
-                for (int k = lane; k < (n_p >> 2); k += 64)
-                    count += 4 - __popc(nonzero_bytes01(r4[k]));
-            } else {
-                for (int k = lane; k < n_p; k += 64)
-                    count += r[k] ? 0 : 1;
-            }
+            count = invalid_bytes((const uint8_t *)valid_dst + row * n_p, n_p, lane);
         }
         count = wave_sum(count);
         if (lane == 0 && path_cp) {

@@ -224,6 +224,29 @@ __device__ __forceinline__ uint32_t nonzero_bytes01(uint32_t x)
     return ((((x & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x) & 0x80808080u) >> 7;
 }
 
+// invalid (zero) bytes of a byte-validity row, summed over the wave's lanes
+// by the caller; 16-byte loads where the row allows them
+__device__ __forceinline__ int invalid_bytes(const uint8_t *__restrict__ r, int n_p, int lane)
+{
+    int count = 0;
+    if (((n_p & 15) == 0) && (((uintptr_t)r) & 15) == 0) {
+        const uint4 *r16 = reinterpret_cast<const uint4 *>(r);
+        for (int k = lane; k < (n_p >> 4); k += 64) {
+            const uint4 x = r16[k];
+            count += 16 - __popc(nonzero_bytes01(x.x)) - __popc(nonzero_bytes01(x.y)) -
+                     __popc(nonzero_bytes01(x.z)) - __popc(nonzero_bytes01(x.w));
+        }
+    } else if (((n_p & 3) == 0) && (((uintptr_t)r) & 3) == 0) {
+        const uint32_t *r4 = reinterpret_cast<const uint32_t *>(r);
+        for (int k = lane; k < (n_p >> 2); k += 64)
+            count += 4 - __popc(nonzero_bytes01(r4[k]));
+    } else {
+        for (int k = lane; k < n_p; k += 64)
+            count += r[k] ? 0 : 1;
+    }
+    return count;
+}
+
 // index of this wave within the grid, as a wave-uniform (scalar) value
 __device__ __forceinline__ int64_t wave_index()
 {
