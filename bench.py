@@ -247,11 +247,24 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
         P.check("pump_propagate_cp_bits", P.lib.pump_propagate_cp_bits(
             n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, d.edge_hs_mask, g_times.p,
             d.topts_edge, prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, bsrc.p,
-            bdst.p, 0, n_p, g_cp.p, 0, 0, None, None, g_ws.p, ws_bytes, stream.h))
+            bdst.p, 0, n_p, g_cp.p, 0, 0, None, None, None, g_ws.p, ws_bytes, stream.h))
+
+    # batched, bit-mask validity, split-BF16 MFMA half-space tests (FP32 band re-test)
+    bdst2 = P.DeviceArray(shape=(n_new, nw), dtype=np.uint64, fill=0)
+    cp_split = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
+    band = P.DeviceArray(shape=(1,), dtype=np.uint64, fill=0)
+
+    def run_split():
+        P.check("pump_propagate_cp_bits", P.lib.pump_propagate_cp_bits(
+            n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, d.edge_hs_mask, g_times.p,
+            d.topts_edge, prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult, n_mc, bsrc.p,
+            bdst2.p, 0, n_p, cp_split.p, 0, 1, d.offsets_split, band.p, None, g_ws.p, ws_bytes,
+            stream.h))
 
     forms = (("dropin_fp32_bytes", run_dropin, lambda: cp_in.numpy()[n_par:]),
              ("batched_fp32_bytes", run_bytes, lambda: cp_bytes.numpy()),
-             ("batched_fp32_bits", run_bits, lambda: g_cp.numpy()))
+             ("batched_fp32_bits", run_bits, lambda: g_cp.numpy()),
+             ("batched_split_bf16_bits", run_split, lambda: cp_split.numpy()))
     for name, fn, result in forms:
         fn()
         stream.sync()
@@ -267,8 +280,17 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
     ref = cps["dropin_fp32_bytes"]
     out["forms_agree"] = bool(all(np.array_equal(ref, v) for v in cps.values()))
     out["mean_cp"] = float(ref.mean())
-    out["chosen_vs_dropin"] = out["dropin_fp32_bytes"]["ms"] / out["batched_fp32_bits"]["ms"]
-    out["value"] = out["batched_fp32_bits"]["particle_steps_per_s"]
+    band.upload(np.zeros(1, np.uint64))
+    run_split()
+    stream.sync()
+    out["band_retests_per_pass"] = int(band.numpy()[0])
+    out["split_bf16_vs_dropin"] = (out["dropin_fp32_bytes"]["ms"]
+                                   / out["batched_split_bf16_bits"]["ms"])
+    out["fp32_bits_vs_dropin"] = (out["dropin_fp32_bytes"]["ms"]
+                                  / out["batched_fp32_bits"]["ms"])
+    best = min(("batched_fp32_bits", "batched_split_bf16_bits"), key=lambda k: out[k]["ms"])
+    out["best_form"] = best
+    out["value"] = out[best]["particle_steps_per_s"]
     out["unit"] = "particle-steps/s"
     return out
 

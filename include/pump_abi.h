@@ -226,18 +226,32 @@ int pump_gather_rows(const uint8_t *d_src, const int *d_src_rows, int n_rows, ui
 /* Chosen form: validity as bit masks (bit p%64 of word path*ceil(nP/64) +
  * p/64), child masks written by ballots; CP from popcounts.  Identical
  * per-particle decisions to pump_propagate_cp.  precision: 0 = FP32 (the
- * reference's operation order); 1 = split-BF16 hi/lo products on MFMA with
- * FP32 accumulation and an FP32 re-test of every particle whose
- * |offset.a*offsetMult - b| lies within the stated margin band. */
+ * reference's operation order); 1 = split-BF16: the half-space tests of the
+ * batched form's time-binned pass run as bf16 hi/lo products on MFMA with
+ * FP32 accumulation, and every test whose |offset.a - b/offsetMult| lies
+ * within the margin band 6e-5*max|o|*sum|a| + 2.5e-5*|b/offsetMult| is redone
+ * in FP32 in the reference's order (counted into *d_band_count when
+ * non-NULL), so the decisions stay identical.  precision 1 needs the
+ * particle-vector table of pump_split_offsets (d_offsets_split); other
+ * stages, offsetMult <= 0 or max_hs > 8 run in FP32. */
 int pump_propagate_cp_bits(int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets,
                            const int *d_wf_edge, const int *d_wf_prev, const float *d_as,
                            const float *d_bs, const uint64_t *d_edge_hs_mask,
-                           const float *d_times, const float *d_topts, float dt, int max_hs, const float *d_offsets_time, int tmax_steps,
-                           float offset_mult, int num_mc_particles,
-                           const uint64_t *d_valid_src, uint64_t *d_valid_dst, int64_t dst_base,
-                           int n_p, float *d_path_cp, int64_t cp_base, int precision,
+                           const float *d_times, const float *d_topts, float dt, int max_hs,
+                           const float *d_offsets_time, int tmax_steps, float offset_mult,
+                           int num_mc_particles, const uint64_t *d_valid_src,
+                           uint64_t *d_valid_dst, int64_t dst_base, int n_p, float *d_path_cp,
+                           int64_t cp_base, int precision, const void *d_offsets_split,
                            unsigned long long *d_band_count, unsigned long long *d_clamped,
                            void *d_workspace, size_t workspace_bytes, pump_stream_t stream);
+
+/* Particle-vector table of the split-BF16 test: for each (time index,
+ * particle) of offsetsTime, 16 bf16 (bf16 hi/lo of the three offsets in MFMA
+ * operand order, and max|o| rounded up).  Build once per noise table;
+ * d_split must be 16-byte aligned and pump_split_offsets_bytes() long. */
+size_t pump_split_offsets_bytes(int tmax_steps, int n_mc);
+int pump_split_offsets(const float *d_offsets_time, int tmax_steps, int n_mc, void *d_split,
+                       pump_stream_t stream);
 
 /* Initial bit masks for pump_propagate_cp_bits: all n_p particles valid in
  * each of n_paths rows (the bit form of setupValidParticles). */

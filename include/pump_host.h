@@ -146,6 +146,7 @@ typedef struct pump_roadmap_dev {
     int *edge_src, *edge_dst, *count_stored;
     uint8_t *is_free_edges, *is_free_samples;
     uint64_t *edge_hs_mask; /* per-edge waypoint summary of the stored half-spaces */
+    void *offsets_split;    /* particle vectors of the split-BF16 test */
 } pump_roadmap_dev;
 int pump_roadmap_device(const pump_roadmap *rm, pump_roadmap_dev *dev);
 

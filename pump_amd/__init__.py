@@ -91,7 +91,7 @@ class RoadmapDev(ctypes.Structure):
     _fields_ = [(n, c_void_p) for n in (
         "samples", "obstacles", "copts", "topts", "copts_edge", "topts_edge", "h", "disc", "as_",
         "bs", "offsets_time", "xcomb", "row_start", "wp_offsets", "edge_src", "edge_dst",
-        "count_stored", "is_free_edges", "is_free_samples", "edge_hs_mask")]
+        "count_stored", "is_free_edges", "is_free_samples", "edge_hs_mask", "offsets_split")]
 
 
 class SearchResult(ctypes.Structure):
@@ -143,7 +143,9 @@ _SIGS = {
     "pump_gather_rows": (c_int, [S, S, c_int, S, c_int64, c_int64, S]),
     "pump_propagate_cp_bits": (c_int, [c_int, c_int, S, S, S, S, S, S, S, S, c_float, c_int, S,
                                        c_int, c_float, c_int, S, S, c_int64, c_int, S, c_int64,
-                                       c_int, S, S, S, c_size_t, S]),
+                                       c_int, S, S, S, S, c_size_t, S]),
+    "pump_split_offsets_bytes": (c_size_t, [c_int, c_int]),
+    "pump_split_offsets": (c_int, [S, c_int, c_int, S, S]),
     "pump_setup_valid_bits": (c_int, [S, c_int, c_int64, S]),
     "pump_collision_probability": (c_int, [S, c_int, S, c_int, c_int, c_float, S, c_int, f32p, S]),
     "pump_collision_probability_many": (c_int, [S, c_int, S, c_int, c_int, c_float, S, c_int64, S,

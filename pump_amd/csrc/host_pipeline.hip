@@ -54,6 +54,7 @@ struct pump_roadmap {
     int max_hs = 0;
     float maxd2 = 0;
     float *offsets_time = nullptr, *xcomb = nullptr;
+    uint16_t *offsets_split = nullptr; // particle vectors of the split-BF16 test
     void *ws = nullptr;
     size_t ws_bytes = 0;
     pump_roadmap_info info{};
@@ -155,6 +156,9 @@ int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
         TRYH(hipMemcpyAsync(rm->offsets_time, offsets_time_h, sizeof(float) * nt,
                             hipMemcpyHostToDevice, s));
         TRYH(hipMemcpyAsync(rm->xcomb, xcomb_h, sizeof(float) * nt, hipMemcpyHostToDevice, s));
+        TRY(dalloc(&rm->offsets_split,
+                   pump_split_offsets_bytes(rm->tmax_steps, rm->n_mc) / sizeof(uint16_t)));
+        TRY(pump_split_offsets(rm->offsets_time, rm->tmax_steps, rm->n_mc, rm->offsets_split, s));
     }
 
     // (a) all ordered pairs
@@ -405,6 +409,7 @@ extern "C" void pump_roadmap_free(pump_roadmap *rm)
     dfree(rm->count);
     dfree(rm->edge_hs_mask);
     dfree(rm->offsets_time);
+    dfree(rm->offsets_split);
     dfree(rm->xcomb);
     dfree(rm->ws);
     if (rm->stream)
@@ -536,6 +541,7 @@ extern "C" int pump_roadmap_device(const pump_roadmap *rm, pump_roadmap_dev *d)
     d->as = rm->as;
     d->bs = rm->bs;
     d->offsets_time = rm->offsets_time;
+    d->offsets_split = rm->offsets_split;
     d->xcomb = rm->xcomb;
     d->row_start = rm->row_start;
     d->wp_offsets = rm->wp_offsets;
@@ -780,7 +786,7 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
                 rm->max_hs,
                 rm->offsets_time, rm->tmax_steps, P.offset_mult, rm->n_mc,
                 (const uint64_t *)valid.p, (uint64_t *)stage.p, 0, n_p, (float *)d_cp.p, 0,
-                P.precision, clamp + 1, clamp, d_ws.p, d_ws.bytes, s));
+                P.precision, rm->offsets_split, clamp + 1, clamp, d_ws.p, d_ws.bytes, s));
         } else {
             TRY(pump_propagate_cp(n_new, 0, rm->wp_offsets, (const int *)d_edge.p,
                                   (const int *)d_prev.p, rm->as, rm->bs, rm->edge_hs_mask,

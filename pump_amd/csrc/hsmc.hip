@@ -918,7 +918,8 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
                             float dt, int max_hs, const float *offsets_time, int tmax_steps,
                             float om, int n_mc, const void *src, void *dst, int64_t dst_base,
                             int n_p, float *path_cp, int64_t cp_base, unsigned long long *clamped,
-                            void *ws, size_t ws_bytes, hipStream_t s)
+                            const uint16_t *split, unsigned long long *band, void *ws,
+                            size_t ws_bytes, hipStream_t s)
 {
     const int64_t row_bytes = kBits ? (int64_t)((n_p + 63) / 64) * 8 : (int64_t)n_p;
     const BatchLayout L(n_new, tmax_steps);
@@ -956,6 +957,8 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
         B.dst_base = dst_base;
         B.path_cp = path_cp;
         B.cp_base = cp_base;
+        B.split = split;
+        B.band = band;
         // control words, histogram and cursors, cleared by a kernel (safe
         // under graph capture)
         const int64_t n_zero = (int64_t)(L.tstart / 4);
@@ -1054,9 +1057,10 @@ extern "C" int pump_propagate_cp(int num_new_paths, int waypoints_count,
     return launch_propagate<false>(num_new_paths, waypoints_count, d_wp_offsets, d_wf_edge,
                                    d_wf_prev, d_as, d_bs, d_edge_hs_mask, d_times, d_topts, dt,
                                    max_hs, d_offsets_time, tmax_steps, offset_mult,
-                                   num_mc_particles,
-                                   d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp, cp_base,
-                                   d_clamped, d_workspace, workspace_bytes, as_stream(stream));
+                                   num_mc_particles, d_valid_src, d_valid_dst, dst_base, n_p,
+                                   d_path_cp, cp_base, d_clamped, (const uint16_t *)nullptr,
+                                   (unsigned long long *)nullptr, d_workspace, workspace_bytes,
+                                   as_stream(stream));
 }
 
 extern "C" int pump_gather_rows(const uint8_t *d_src, const int *d_src_rows, int n_rows,
@@ -1073,17 +1077,6 @@ extern "C" int pump_gather_rows(const uint8_t *d_src, const int *d_src_rows, int
     return PUMP_OK;
 }
 
-// split-BF16 MFMA form, implemented in hsmc_mfma.hip
-namespace pump {
-int propagate_bits_mfma(int n_new, const int64_t *wp_offsets, int w_fixed, const int *wf_edge,
-                        const int *wf_prev, const float *as, const float *bs, const float *times,
-                        const float *topts, float dt, int max_hs, const float *offsets_time,
-                        int tmax_steps, float om, int n_mc, const uint64_t *valid_src,
-                        uint64_t *valid_dst, int64_t dst_base, int n_p, float *path_cp,
-                        int64_t cp_base, unsigned long long *band, unsigned long long *clamped,
-                        void *ws, size_t ws_bytes, hipStream_t s);
-}
-
 extern "C" int pump_propagate_cp_bits(int num_new_paths, int waypoints_count,
                                       const int64_t *d_wp_offsets, const int *d_wf_edge,
                                       const int *d_wf_prev, const float *d_as, const float *d_bs,
@@ -1094,6 +1087,7 @@ extern "C" int pump_propagate_cp_bits(int num_new_paths, int waypoints_count,
                                       const uint64_t *d_valid_src, uint64_t *d_valid_dst,
                                       int64_t dst_base, int n_p, float *d_path_cp,
                                       int64_t cp_base, int precision,
+                                      const void *d_offsets_split,
                                       unsigned long long *d_band_count,
                                       unsigned long long *d_clamped, void *d_workspace,
                                       size_t workspace_bytes, pump_stream_t stream)
@@ -1106,20 +1100,18 @@ extern "C" int pump_propagate_cp_bits(int num_new_paths, int waypoints_count,
         return PUMP_OK;
     PUMP_REQUIRE(d_wf_edge && d_wf_prev && d_as && d_bs && d_times && d_topts && d_offsets_time &&
                  d_valid_src && d_valid_dst && d_valid_src != d_valid_dst);
-    if (precision == 1)
-        return propagate_bits_mfma(num_new_paths, d_wp_offsets, waypoints_count, d_wf_edge,
-                                   d_wf_prev, d_as, d_bs, d_times, d_topts, dt, max_hs,
-                                   d_offsets_time, tmax_steps, offset_mult, num_mc_particles,
-                                   d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp, cp_base,
-                                   d_band_count, d_clamped, d_workspace, workspace_bytes,
-                                   as_stream(stream));
-    PUMP_REQUIRE(precision == 0);
+    PUMP_REQUIRE(precision == 0 || (precision == 1 && d_offsets_split));
+    // the split-BF16 test needs om > 0 for its sign rule; otherwise FP32 runs
+    // (the decisions are the same either way)
+    const uint16_t *split = (precision == 1 && offset_mult > 0.0f && offset_mult < INFINITY)
+                                ? (const uint16_t *)d_offsets_split
+                                : nullptr;
     return launch_propagate<true>(num_new_paths, waypoints_count, d_wp_offsets, d_wf_edge,
                                   d_wf_prev, d_as, d_bs, d_edge_hs_mask, d_times, d_topts, dt,
                                   max_hs, d_offsets_time, tmax_steps, offset_mult,
-                                  num_mc_particles,
-                                  d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp, cp_base,
-                                  d_clamped, d_workspace, workspace_bytes, as_stream(stream));
+                                  num_mc_particles, d_valid_src, d_valid_dst, dst_base, n_p,
+                                  d_path_cp, cp_base, d_clamped, split, d_band_count, d_workspace,
+                                  workspace_bytes, as_stream(stream));
 }
 
 // initial bit masks: all n_p particles valid

@@ -16,6 +16,7 @@
 // reference's float order, so rows and CPs are bit-identical to the
 // per-path forms.  When the pairs overflow the workspace, the same launch
 // runs the group form instead (one wave per listed path and 512 particles).
+#include "hsmc_mfma.hpp"
 #include "hsmc_wave.hpp"
 #include "pump_internal.hpp"
 
@@ -258,6 +259,245 @@ __device__ __forceinline__ void bin_test(const BinArgs &a, int64_t n_chunks)
     }
 }
 
+// Split-BF16 MFMA form of bin_test (bit-identical decisions; see
+// hsmc_mfma.hpp for the operand layout and the margin).  One wave per
+// (chunk, 256 particles): the chunk's live (pair, slot) half-spaces become
+// rows of 16-row tiles staged in the wave's LDS slice; particles are taken 16
+// at a time from the precomputed particle vectors; each MFMA tile decides 16
+// x 16 tests with two compares per element against the row's margin (built
+// from the table-wide bound on |o|), band cases are re-tested in FP32, and a
+// row's failures in a 16-particle group are ANDed into its path's word with
+// one atomic.  A half-space with a = 0 and b not > 0 fails every particle
+// (LHS is 0 or NaN): it clears the item's particle range of its path at once.
+constexpr int kMfmaRows = kBinChunk * kSlotTile; // rows per chunk at most
+
+struct MfmaRow {
+    float a0, a1, a2, b;
+    int w;
+};
+
+template <bool kBits>
+__device__ __forceinline__ void clear_range(const BinArgs &a, int w, int p_begin, int lane)
+{
+    const int n_p = a.n_p, n_groups = (n_p + 63) >> 6;
+    if (kBits) {
+        // whole 64-bit words: bits past n_p are zero already
+        const int g0 = p_begin >> 6;
+        for (int g = g0; g < g0 + kTestGroups && g < n_groups; ++g)
+            atomicAnd((unsigned long long *)a.valid_dst + (a.dst_base + w) * n_groups + g, 0ull);
+    } else {
+        uint8_t *r = (uint8_t *)a.valid_dst + (a.dst_base + w) * n_p;
+        for (int p = p_begin; p < p_begin + 64 * kTestGroups && p < n_p; ++p)
+            r[p] = 0;
+    }
+    (void)lane;
+}
+
+template <bool kBits>
+__device__ __forceinline__ void bin_test_mfma(const BinArgs &a, int64_t n_chunks, MfmaRow *rows)
+{
+    using namespace mfma;
+    constexpr int NQ = kTestGroups; // 64-particle groups per item (16-particle tiles: 4*NQ)
+    constexpr int NT = 4 * NQ;
+    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
+    const int n_p = a.n_p, max_hs = a.max_hs;
+    const float om = a.om;
+    const int n_groups = (n_p + 63) >> 6;
+    const int n_sg = (n_groups + NQ - 1) / NQ;
+    const int64_t n_items = n_chunks * n_sg;
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    // max |o_d| over the whole offsets table, stored after the vectors
+    const float omax = *reinterpret_cast<const float *>(
+        a.split + (int64_t)a.tmax_steps * a.n_mc * kSplitK);
+    unsigned n_band = 0; // per lane
+    for (int64_t item = wave_index(); item < n_items; item += n_waves) {
+        const int ch = (int)(item / n_sg);
+        const int sg = (int)(item - (int64_t)ch * n_sg);
+        const BinChunk c = a.chunks[ch];
+        const int ne = c.end - c.begin;
+        const int pbase = sg * 64 * NQ;
+        // stage the chunk's live half-spaces as rows, pair by pair
+        int my_w = 0, live = 0, dead = 0;
+        float b[kSlotTile], av[kSlotTile][3];
+        if (lane < ne) {
+            const BinEntry en = a.entries[c.begin + lane];
+            my_w = en.w;
+#pragma unroll
+            for (int h = 0; h < kSlotTile; ++h) {
+                b[h] = 1111.0f;
+                av[h][0] = av[h][1] = av[h][2] = 0.0f;
+                if (h < max_hs) {
+                    const int64_t k = en.wp * max_hs + h;
+                    b[h] = a.bs[k];
+                    av[h][0] = a.as[k * 3 + 0];
+                    av[h][1] = a.as[k * 3 + 1];
+                    av[h][2] = a.as[k * 3 + 2];
+                }
+                // the reference skips slots with b >= 1000 (PUMP.cu:642)
+                if (!(b[h] >= 1000.0f)) {
+                    if (av[h][0] == 0.0f && av[h][1] == 0.0f && av[h][2] == 0.0f &&
+                        !(b[h] > 0.0f))
+                        dead = 1;
+                    else
+                        live |= 1 << h;
+                }
+            }
+        }
+        if (dead) {
+            clear_range<kBits>(a, my_w, pbase, lane);
+            live = 0; // every particle of the range has failed already
+        }
+        const int cnt = __popc(live);
+        int pre = cnt; // inclusive scan over lanes
+        for (int off = 1; off < 64; off <<= 1) {
+            const int u = __shfl_up(pre, off, 64);
+            if (lane >= off)
+                pre += u;
+        }
+        const int n_rows = __builtin_amdgcn_readlane(pre, 63);
+        pre -= cnt;
+        if (lane < ne) {
+            int r = pre;
+#pragma unroll
+            for (int h = 0; h < kSlotTile; ++h)
+                if (live & (1 << h)) {
+                    MfmaRow e;
+                    e.a0 = av[h][0];
+                    e.a1 = av[h][1];
+                    e.a2 = av[h][2];
+                    e.b = b[h];
+                    e.w = my_w;
+                    rows[r++] = e;
+                }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // this lane's 4 bf16 of particle pbase + col; tile nt is nt*16 vectors on
+        const bf16x4 *sb = reinterpret_cast<const bf16x4 *>(
+            a.split + ((int64_t)c.ti * a.n_mc + pbase + col) * kSplitK + 4 * q);
+        const int p_left = n_p - pbase - col; // tile nt holds this lane's particle iff nt*16 < p_left
+        auto load_b = [&](int nt) {
+            bf16x4 v = pack4(0, 0, 0, 0);
+            if (nt < NT && nt * 16 < p_left)
+                v = sb[nt * 16 * (kSplitK / 4)];
+            return v;
+        };
+        for (int m0 = 0; m0 < n_rows; m0 += 16) {
+            // A operand: row m0 + col; margins of the rows this lane's D holds
+            bf16x4 aop = pack4(0, 0, 0, 0);
+            int wcol = 0; // path of row m0 + col
+            {
+                const int r = m0 + col;
+                if (r < n_rows) {
+                    const MfmaRow e = rows[r];
+                    aop = halfspace_operand(q, e.a0, e.a1, e.a2, e.b / om);
+                    wcol = e.w;
+                }
+            }
+            float mg[4];
+            uint64_t rv[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int r = m0 + 4 * q + j;
+                mg[j] = 0.0f;
+                if (r < n_rows) {
+                    const MfmaRow e = rows[r];
+                    mg[j] = fmaf(omax, kMarginDot * (fabsf(e.a0) + fabsf(e.a1) + fabsf(e.a2)),
+                                 kMarginB * fabsf(e.b / om) + 1e-30f);
+                }
+                rv[j] = __ballot(r < n_rows);
+            }
+            // one 64-particle word (four 16-particle tiles) per step: the four
+            // MFMAs issue back to back, the next word's operands load meanwhile
+            bf16x4 nb[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                nb[t] = load_b(t);
+            for (int wd = 0; wd < NQ; ++wd) {
+                const int pw = pbase + wd * 64;
+                if (pw >= n_p)
+                    break;
+                bf16x4 cb[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    cb[t] = nb[t];
+                    nb[t] = load_b((wd + 1) * 4 + t);
+                }
+                f32x4 d[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+                    d[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aop, cb[t], zero, 0, 0, 0);
+                }
+                // lane r < 16: failures of row m0 + r in this word
+                uint64_t acc = 0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int p0 = pw + t * 16;
+                    const int p = p0 + col;
+                    const uint64_t pv = __ballot(p < n_p);
+                    uint64_t fail[4], band = 0, any_fail = 0;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint64_t use = pv & rv[j];
+                        const uint64_t clear = __ballot(fabsf(d[t][j]) > mg[j]); // not NaN
+                        const uint64_t pos = __ballot(d[t][j] > 0.0f);
+                        fail[j] = use & clear & pos;
+                        any_fail |= fail[j];
+                        band |= (use & ~clear) ? (1ull << j) : 0ull; // rows with band lanes
+                    }
+                    if (band) {
+                        // FP32 re-test in the reference's order for band cases
+                        const float *op = a.offsets_time + ((int64_t)c.ti * a.n_mc + p) * 3;
+                        for (int j = 0; j < 4; ++j) {
+                            if (!((band >> j) & 1))
+                                continue;
+                            const uint64_t use = pv & rv[j];
+                            const bool mine = (use >> lane) & 1ull && !(fabsf(d[t][j]) > mg[j]);
+                            bool f = false;
+                            if (mine) {
+                                const MfmaRow e = rows[m0 + 4 * q + j];
+                                f = !hs_pass(op[0], op[1], op[2], e.a0, e.a1, e.a2, om, e.b);
+                                ++n_band;
+                            }
+                            fail[j] |= __ballot(f);
+                            any_fail |= fail[j];
+                        }
+                    }
+                    if (!any_fail)
+                        continue;
+                    if (kBits) {
+                        // row m0+4g+j, particles p0..p0+15: bits 16g.. of fail[j]
+                        const int j = col & 3, g = col >> 2;
+                        const uint64_t fj = j == 0 ? fail[0] : j == 1 ? fail[1]
+                                          : j == 2 ? fail[2] : fail[3];
+                        acc |= ((fj >> (16 * g)) & 0xffffull) << (16 * t);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 4; ++j)
+                            if ((fail[j] >> lane) & 1ull)
+                                ((uint8_t *)a.valid_dst)[(a.dst_base + rows[m0 + 4 * q + j].w) *
+                                                             n_p +
+                                                         p] = 0;
+                    }
+                }
+                if (kBits && lane < 16 && acc)
+                    atomicAnd((unsigned long long *)a.valid_dst +
+                                  (a.dst_base + wcol) * n_groups + (pw >> 6),
+                              ~(unsigned long long)acc);
+            }
+        }
+        // the next item overwrites the rows
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    const int total_band = wave_sum((int)n_band);
+    if (a.band && lane == 0 && total_band)
+        atomicAdd(a.band, (unsigned long long)total_band);
+}
+
 // Group form (used when the binned pairs overflow the workspace): one wave
 // per (listed path, 64*NQ particles) walks the path's marked waypoints and
 // writes the child words from the parent row; the path's CP is finished by
@@ -341,14 +581,28 @@ __device__ __forceinline__ void group_pass(const BinArgs &a, int64_t n_listed)
     }
 }
 
-// The listed paths' half-space tests: binned, or the group form on overflow.
-template <bool kBits>
+// The listed paths' half-space tests, binned (FP32, or split-BF16 MFMA when
+// a particle-vector table is given); k_hs_groups runs instead when the
+// binned pairs overflowed.  (Separate kernels keep each one's registers.)
+template <bool kBits, bool kMfma>
 __global__ __launch_bounds__(256) void k_hs_pass(const BinArgs a)
 {
-    if (a.ctl[kCtlOverflow] == 0)
+    if (a.ctl[kCtlOverflow] != 0)
+        return;
+    if (kMfma) {
+        __shared__ MfmaRow rows[256 / 64][kMfmaRows];
+        bin_test_mfma<kBits>(a, a.ctl[kCtlChunks], rows[threadIdx.x >> 6]);
+    } else {
         bin_test<kBits>(a, a.ctl[kCtlChunks]);
-    else
-        group_pass<kBits, kHsGroups>(a, a.ctl[kCtlListed]);
+    }
+}
+
+template <bool kBits>
+__global__ __launch_bounds__(256) void k_hs_groups(const BinArgs a)
+{
+    if (a.ctl[kCtlOverflow] == 0)
+        return;
+    group_pass<kBits, kHsGroups>(a, a.ctl[kCtlListed]);
 }
 
 // One wave per listed path: CP from its finished child row.
@@ -411,18 +665,28 @@ int bins_finish(const BinArgs &a, bool bits, hipStream_t s)
                        (const int *)a.hist, (const int *)a.tstart, a.cursor, a.entries);
     PUMP_LAUNCHED();
     const int64_t groups = (a.n_p + 63) / 64;
-    const int64_t w_bins = a.chunk_cap * ((groups + kTestGroups - 1) / kTestGroups);
-    const int64_t w_groups = (int64_t)a.n_new * ((groups + kHsGroups - 1) / kHsGroups);
-    const unsigned g_hs = wave_blocks(w_bins > w_groups ? w_bins : w_groups);
+    const unsigned g_hs = wave_blocks(a.chunk_cap * ((groups + kTestGroups - 1) / kTestGroups));
+    const unsigned g_gr = wave_blocks((int64_t)a.n_new * ((groups + kHsGroups - 1) / kHsGroups));
     const unsigned g_cp = wave_blocks(a.n_new);
+    const bool mfma = a.split && a.max_hs <= kSlotTile;
     if (bits) {
-        hipLaunchKernelGGL(k_hs_pass<true>, dim3(g_hs), dim3(256), 0, s, a);
+        if (mfma)
+            hipLaunchKernelGGL((k_hs_pass<true, true>), dim3(g_hs), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_hs_pass<true, false>), dim3(g_hs), dim3(256), 0, s, a);
+        PUMP_LAUNCHED();
+        hipLaunchKernelGGL(k_hs_groups<true>, dim3(g_gr), dim3(256), 0, s, a);
         PUMP_LAUNCHED();
         hipLaunchKernelGGL(k_cp_listed<true>, dim3(g_cp), dim3(256), 0, s, a.recs,
                            (const int *)a.ctl, (const void *)a.valid_dst, a.dst_base, a.n_p,
                            a.path_cp, a.cp_base);
     } else {
-        hipLaunchKernelGGL(k_hs_pass<false>, dim3(g_hs), dim3(256), 0, s, a);
+        if (mfma)
+            hipLaunchKernelGGL((k_hs_pass<false, true>), dim3(g_hs), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_hs_pass<false, false>), dim3(g_hs), dim3(256), 0, s, a);
+        PUMP_LAUNCHED();
+        hipLaunchKernelGGL(k_hs_groups<false>, dim3(g_gr), dim3(256), 0, s, a);
         PUMP_LAUNCHED();
         hipLaunchKernelGGL(k_cp_listed<false>, dim3(g_cp), dim3(256), 0, s, a.recs,
                            (const int *)a.ctl, (const void *)a.valid_dst, a.dst_base, a.n_p,

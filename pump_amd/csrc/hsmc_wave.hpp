@@ -255,7 +255,7 @@ __device__ __forceinline__ int64_t wave_index()
 }
 
 constexpr int kWalkGroups = 4; // particle groups of 64 per walk, wave-per-path kernel
-constexpr int kHsGroups = 8;   // the same for the batched half-space kernel (swept: 2, 4, 8)
+constexpr int kHsGroups = 4;   // the same for the group form of the batched pass
 
 // A path that meets at least one stored half-space, as listed by the
 // batched classification pass.
@@ -314,6 +314,10 @@ struct BinArgs {
     int64_t dst_base;
     float *path_cp;
     int64_t cp_base;
+    // split-BF16 MFMA test (precision 1): particle vectors of the offsets
+    // table, and the count of FP32 re-tests in the margin band
+    const uint16_t *split;
+    unsigned long long *band;
 };
 int bins_scan(const BinArgs &a, hipStream_t s);
 int bins_finish(const BinArgs &a, bool bits, hipStream_t s);

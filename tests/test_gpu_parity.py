@@ -244,12 +244,20 @@ def test_propagate_entry_points(variable):
         d["n_new"], W, off_ptr, ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
         ptr(g["bs"]), None, ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"],
         ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(bsrc), ptr(bdst), 0, d["n_p"], ptr(cp3),
-        0, 0, None, None, None, 0, None))
+        0, 0, None, None, None, None, 0, None))
     out = host(bdst)
     for r in range(d["n_new"]):
         unpacked = np.unpackbits(out[r].view(np.uint8), bitorder="little")[: d["n_p"]]
         assert np.array_equal(unpacked, ref_v[d["n_parent"] + r])
     assert np.array_equal(host(cp3), ref_cp[d["n_parent"]:])
+
+
+def split_table(g_offs, T, n_mc):
+    """The split-BF16 particle-vector table of a device offsets table."""
+    nbytes = P.lib.pump_split_offsets_bytes(T, n_mc)
+    t = P.DeviceArray(shape=(nbytes,), dtype=np.uint8, fill=0)
+    P.check("split_offsets", P.lib.pump_split_offsets(ptr(g_offs), T, n_mc, t.p, None))
+    return t
 
 
 def pack_bits(rows, n_p):
@@ -287,6 +295,7 @@ def test_propagate_batched_form(n_p, w_range, T):
     mask = edge_mask_host(d["off"], d["bs"], d["max_hs"])
     assert (mask >> np.uint64(63)).any() == (w_range[1] > 64)
     g_mask = dev(mask)
+    g_split = split_table(g["offs"], d["T"], d["n_mc"])
     ws_bytes = P.lib.pump_propagate_workspace(d["n_new"], d["T"])
     ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
     npar, nn = d["n_parent"], d["n_new"]
@@ -306,18 +315,20 @@ def test_propagate_batched_form(n_p, w_range, T):
         # control words: listed paths, binned pairs, overflow flag
         ctl = ws.numpy()[:16].view(np.int32)
         assert ctl[0] > 0 and ctl[2] == (1 if w_range[1] > 64 else 0), ctl[:4]
-        bsrc = dev(pack_bits(d["valid"][:npar], n_p))
-        bdst = dev(np.zeros((nn, (n_p + 63) // 64), np.uint64))
-        cpb = dev(np.zeros(nn, np.float32))
-        clamp_b = dev(np.zeros(2, np.uint64))
-        P.check("propagate_cp_bits", P.lib.pump_propagate_cp_bits(
-            nn, 0, ptr(g["off"]), ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
-            ptr(g["bs"]), m, ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"],
-            ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(bsrc), ptr(bdst), 0, n_p, ptr(cpb), 0,
-            0, None, ptr(clamp_b), ws.p, ws_bytes, None))
-        assert np.array_equal(unpack_bits(host(bdst), n_p), ref_v[npar:])
-        assert np.array_equal(host(cpb), ref_cp[npar:])
-        assert int(host(clamp_b)[0]) == ref_cl
+        for precision, split in ((0, None), (1, g_split)):
+            bsrc = dev(pack_bits(d["valid"][:npar], n_p))
+            bdst = dev(np.zeros((nn, (n_p + 63) // 64), np.uint64))
+            cpb = dev(np.zeros(nn, np.float32))
+            clamp_b = dev(np.zeros(2, np.uint64))
+            P.check("propagate_cp_bits", P.lib.pump_propagate_cp_bits(
+                nn, 0, ptr(g["off"]), ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
+                ptr(g["bs"]), m, ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"],
+                ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(bsrc), ptr(bdst), 0, n_p,
+                ptr(cpb), 0, precision, split.p if split else None, clamp_b.ptr + 8,
+                ptr(clamp_b), ws.p, ws_bytes, None))
+            assert np.array_equal(unpack_bits(host(bdst), n_p), ref_v[npar:]), precision
+            assert np.array_equal(host(cpb), ref_cp[npar:]), precision
+            assert int(host(clamp_b)[0]) == ref_cl
         # the single-launch form takes the summary too
         dst1 = dev(np.zeros((nn, n_p), np.uint8))
         cp1 = dev(np.zeros(nn, np.float32))
@@ -328,6 +339,60 @@ def test_propagate_batched_form(n_p, w_range, T):
             None, None, 0, None))
         assert np.array_equal(host(dst1), ref_v[npar:])
         assert np.array_equal(host(cp1), ref_cp[npar:])
+
+
+def test_split_bf16_band_decisions():
+    """Split-BF16 MFMA tests with half-spaces placed exactly on a particle's
+    decision boundary (b = RN(LHS*offsetMult) in the reference's FP32 order,
+    and the next float above): those tests fall in the margin band, are redone
+    in FP32 and counted, and every row and CP equals the oracle's."""
+    d = random_hsmc_inputs(77, n_edges=3000, W=9, max_hs=5, n_p=700, n_mc=1024, T=40,
+                           n_new=6000, n_parent=64)
+    keep = np.random.default_rng(2).uniform(size=d["off"].shape[0] - 1) < 0.2
+    hs_rows = d["bs"].reshape(-1, d["max_hs"])
+    for e in np.nonzero(~keep)[0]:
+        hs_rows[d["off"][e]:d["off"][e + 1]] = 1111
+    # every waypoint of every path at time index 0
+    d["times"][:] = 0
+    d["topts"][:] = np.float32(1e-6)
+    f32 = np.float32
+    om = f32(d["om"])
+    rng = np.random.default_rng(3)
+    live = np.nonzero(d["bs"] < 1000)[0]
+    pick = live[rng.uniform(size=live.size) < 0.5]
+    part = rng.integers(0, d["n_p"], pick.size)
+    o = d["offs"][0, part]
+    a = d["as_"][pick]
+    lhs = f32(0) + o[:, 0] * a[:, 0]
+    lhs = lhs + o[:, 1] * a[:, 1]
+    lhs = lhs + o[:, 2] * a[:, 2]
+    edge = (lhs * om).astype(np.float32)
+    up = rng.uniform(size=pick.size) < 0.5
+    d["bs"][pick] = np.where(up, np.nextafter(edge, np.float32(np.inf)), edge)
+    ref_v, ref_cp, _ = oracle_propagate(d)
+    g = {k: dev(d[k]) for k in ("off", "as_", "bs", "offs", "topts", "times", "wf_edge",
+                                 "wf_prev")}
+    W = int(d["w"][0])
+    mask = edge_mask_host(d["off"], d["bs"], d["max_hs"])
+    g_mask = dev(mask)
+    g_split = split_table(g["offs"], d["T"], d["n_mc"])
+    ws_bytes = P.lib.pump_propagate_workspace(d["n_new"], d["T"])
+    ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
+    npar, nn, n_p = d["n_parent"], d["n_new"], d["n_p"]
+    bsrc = dev(pack_bits(d["valid"][:npar], n_p))
+    bdst = dev(np.zeros((nn, (n_p + 63) // 64), np.uint64))
+    cpb = dev(np.zeros(nn, np.float32))
+    cnt = dev(np.zeros(2, np.uint64))
+    P.check("propagate_cp_bits", P.lib.pump_propagate_cp_bits(
+        nn, W, None, ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]), ptr(g["bs"]),
+        ptr(g_mask), ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"], ptr(g["offs"]),
+        d["T"], d["om"], d["n_mc"], ptr(bsrc), ptr(bdst), 0, n_p, ptr(cpb), 0, 1, g_split.p,
+        cnt.ptr + 8, ptr(cnt), ws.p, ws_bytes, None))
+    assert np.array_equal(unpack_bits(host(bdst), n_p), ref_v[npar:])
+    assert np.array_equal(host(cpb), ref_cp[npar:])
+    ctl = ws.numpy()[:16].view(np.int32)
+    assert ctl[2] == 0  # the binned (MFMA) pass ran
+    assert int(host(cnt)[1]) > 0
 
 
 def test_gather_rows():
