@@ -60,6 +60,7 @@ _SIGS = {
     "po_find_optimal_path": (c_float, [c_float, f32p, f32p, c_int, i32p, c_int]),
     "po_fill_copts_topts": (None, [f32p, c_int, c_float, f32p, f32p, c_int]),
     "po_rn_index": (c_int64, [c_int64, c_float]),
+    "po_fill_rows": (None, [f32p, c_int, c_float, i32p, c_int, f32p, f32p, c_int]),
     "po_select_rn": (c_float, [f32p, c_int64, c_int64]),
     "po_count_edges": (c_int64, [f32p, c_int, c_float, i64p]),
     "po_fill_edges": (None, [f32p, f32p, c_int, c_float, i64p, i32p, f32p, f32p]),

@@ -252,6 +252,32 @@ void po_fill_copts_topts(const float *samples, int num, float tmax, float *copts
     (void)nthreads;
 }
 
+/* fillCoptsTopts restricted to the source rows rows[0..n_rows): row k of
+ * the outputs holds the NUM-1 entries of row rows[k], in the layout of
+ * 2pBVP.cu:144-149 (j != i, ascending).  Same per-pair arithmetic as
+ * po_fill_copts_topts; used to check sampled rows of full-size tables. */
+void po_fill_rows(const float *samples, int num, float tmax, const int *rows, int n_rows,
+                  float *copts, float *topts, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1) if (nthreads != 1)
+#endif
+    for (int k = 0; k < n_rows; ++k) {
+        const int i = rows[k];
+        for (int j = 0; j < num; ++j) {
+            if (i == j)
+                continue;
+            int64_t idx = (int64_t)k * (num - 1) + j - (i < j ? 1 : 0);
+            float tau = po_topt_bisection(&samples[i * DIM], &samples[j * DIM], tmax);
+            topts[idx] = tau;
+            copts[idx] = po_cost(tau, &samples[i * DIM], &samples[j * DIM]);
+        }
+    }
+    (void)nthreads;
+}
+
 /* rnIdx -- mainPUMP.cu:262: (int)((NUM-1)*NUM*rnPerc); the int pair count
  * is converted to float and multiplied in float.  (The int product is
  * formed in 64 bits here; the reference overflows for NUM >= 46341.) */

@@ -37,6 +37,8 @@ float po_find_optimal_path(float dt, float *split, const float *xs, int n_wp, in
 /* ---- roadmap precompute (2pBVP.cu:133-150, mainPUMP.cu:257-392) ------- */
 void    po_fill_copts_topts(const float *samples, int num, float tmax,
                             float *copts, float *topts, int nthreads);
+void    po_fill_rows(const float *samples, int num, float tmax, const int *rows, int n_rows,
+                     float *copts, float *topts, int nthreads);
 int64_t po_rn_index(int64_t num, float rn_perc);
 float   po_select_rn(const float *copts, int64_t n_pairs, int64_t rn_idx);
 /* count edges (copts < rn); fills row_start[num+1] (int64) */

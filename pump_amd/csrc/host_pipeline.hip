@@ -626,7 +626,11 @@ struct DevBuf {
         bytes = 0;
         size_t want = b + b / 2;
         if (hipMalloc(&p, want) != hipSuccess) {
+            // the failed attempt leaves its error as the thread's last error:
+            // clear it, so that a later launch check does not report it
+            (void)hipGetLastError();
             if (hipMalloc(&p, b) != hipSuccess) {
+                (void)hipGetLastError();
                 p = nullptr;
                 return PUMP_E_NOMEM;
             }

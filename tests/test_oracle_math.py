@@ -456,3 +456,23 @@ def test_search_invariants(om):
     assert cost == np.float32(out["soln_cost"])
     if out["status"] == 0:
         assert out["soln_cp"] < cfg["params"].cp_target
+
+
+def test_fill_rows_matches_whole_table():
+    """po_fill_rows (sampled source rows, used by the full-size GPU checks)
+    equals the corresponding rows of po_fill_copts_topts."""
+    rng = np.random.default_rng(12)
+    num = 150
+    s = np.concatenate([rng.uniform(0, 1, (num, 3)), rng.uniform(-0.5, 0.5, (num, 3))],
+                       1).astype(np.float32)
+    c = np.zeros(num * (num - 1), np.float32)
+    t = np.zeros(num * (num - 1), np.float32)
+    O.po_fill_copts_topts(oracle.fp(s), num, 5.0, oracle.fp(c), oracle.fp(t), 4)
+    rows = np.array([0, 7, 80, 149], np.int32)
+    rc = np.zeros((rows.size, num - 1), np.float32)
+    rt = np.zeros((rows.size, num - 1), np.float32)
+    O.po_fill_rows(oracle.fp(s), num, 5.0, oracle.ip(rows), rows.size, oracle.fp(rc),
+                   oracle.fp(rt), 4)
+    for k, r in enumerate(rows):
+        assert np.array_equal(rc[k], c[r * (num - 1):(r + 1) * (num - 1)])
+        assert np.array_equal(rt[k], t[r * (num - 1):(r + 1) * (num - 1)])
