@@ -3,6 +3,8 @@
 * NUM=4000 (10 boxes, 20 waypoints per edge, 1000 particles): the whole
   pipeline against the CPU oracle, every table and the search result, in both
   validity forms.
+* Three of the 256 uneven batch instances (NUM 1000-8000, 5-50 boxes,
+  random start/goal corners): the same, against the oracle.
 * NUM=16000 (30 boxes, W <= 64, 2000 particles), the NUM=32000 wall world
   (200 thin boxes) and NUM=64000 (100 boxes, W <= 128, 8192 particles):
   sampled rows of the pair tables against the oracle, the
@@ -225,3 +227,42 @@ def test_full_size_sampled(i):
     if got["soln_len"] > 0:
         rederive_path(cfg, s, o, ot, xc, rm, got, e, f, hs)
     rm.close()
+
+
+def test_batch_instances_parity():
+    """Three of the 256 uneven batch instances (the smallest NUM, the
+    fewest and the most boxes): whole pipeline and search against the oracle."""
+    batch = configs.batch_config()
+    nums = np.array([b["spec"].num for b in batch])
+    nobs = np.array([b["spec"].n_obs for b in batch])
+    picks = sorted({int(np.argmin(nums)), int(np.argmin(nobs + nums / 1e5)),
+                    int(np.argmax(nobs - nums / 1e5))})
+    for k in picks:
+        cfg = batch[k]
+        prm = cfg["params"]
+        prm.keep_pairs = 1
+        s, o = P.gen_problem(cfg["spec"])
+        ot, xc = P.make_noise(cfg["noise"])
+        print("instance", cfg["name"], "boxes", o.shape[0], "n_p", prm.n_p, flush=True)
+        rm = P.Roadmap(s, o, s[-1], ot, xc, prm)
+        orc = OP.roadmap(s, o, s[-1], prm, nthreads=NT)
+        c, t = rm.pairs()
+        assert_close_f32(t, orc["topts"])
+        assert_close_f32(c, orc["copts"])
+        assert np.float32(rm.info.rn) == orc["rn"]
+        e = rm.edges()
+        for key in ("row_start", "edge_dst"):
+            assert np.array_equal(e[key], orc[key]), key
+        f = rm.flags()
+        for key in ("wp_offsets", "is_free_edges", "is_free_samples"):
+            assert np.array_equal(f[key], orc[key]), key
+        hs = rm.halfspaces()
+        assert np.array_equal(hs["count_stored"], orc["count_stored"])
+        got = P.search_result_dict(rm.search())
+        ref = OP.search(orc, s, s[-1], o, ot, xc, prm, nthreads=NT, path_cap=prm.path_cap)
+        for key in ("status", "itrs", "num_paths", "num_goal_paths", "soln_nodes"):
+            assert got[key] == ref[key], key
+        for key in ("soln_cost", "soln_cp", "soln_hsmc_cp"):
+            assert np.float32(got[key]) == np.float32(ref[key]), key
+        rm.close()
+
