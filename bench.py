@@ -19,6 +19,10 @@ Also measured in the same run, each over its own timed repetitions:
       byte-validity form (plain FP32, the reference layout) and the bit-mask
       form;
     * kernel (a): the all-pairs 2pBVP solve (pair-solves/s);
+    * the uneven batch workload (256 problems, NUM log-uniform in
+      [1000, 8000], 5-50 boxes): the batched build's pair pass and edge pass +
+      half-space cache (one launch each for all instances, hipGraph replays)
+      against the same passes launched instance by instance;
     * cpu_baseline: the CPU oracle's edge pass on a bounded sample of the
       same edges with all host cores.
 Prints one JSON line.
@@ -54,6 +58,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--hsmc-reps", type=int, default=3)
     ap.add_argument("--skip-hsmc", action="store_true")
+    ap.add_argument("--batch", type=int, default=256,
+                    help="instances of the uneven batch workload (0: skip)")
     return ap.parse_args()
 
 
@@ -129,6 +135,11 @@ def main():
     pairs = {"pair_solves_per_s": info.n_pairs / (info.ms_pairs / 1e3),
              "ms": info.ms_pairs, "n_pairs": int(info.n_pairs)}
 
+    # ---------------- the uneven batch workload ---------------------------
+    batch = None
+    if args.batch > 0:
+        batch = batch_pass(P, args.batch, args.hsmc_reps)
+
     # ---------------- CPU baseline (oracle, bounded sample) ----------------
     cpu = None
     if not args.no_cpu_baseline:
@@ -165,6 +176,7 @@ def main():
         "cpu_baseline": cpu,
         "hsmc": hsmc,
         "pairs": pairs,
+        "batch": batch,
         "pipeline_ms": {k: getattr(info, k) for k in (
             "ms_pairs", "ms_select", "ms_edges_build", "ms_nn", "ms_sample_free",
             "ms_edge_pass", "ms_hs", "ms_total")},
@@ -292,6 +304,79 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
     out["best_form"] = best
     out["value"] = out[best]["particle_steps_per_s"]
     out["unit"] = "particle-steps/s"
+    return out
+
+
+def batch_pass(P, n_inst, reps):
+    """The first n_inst problems of the uneven batch workload built together
+    (pump_batch_build), then the graph replays of its pair pass and of its
+    edge pass + half-space cache, against the same device passes launched
+    instance by instance on one stream (the single-instance entry points on
+    the batch's own buffers)."""
+    from pump_amd import configs
+    cfgs = configs.batch_config()[:n_inst]
+    probs = []
+    for c in cfgs:
+        c["params"].keep_pairs = 1
+        s, o = P.gen_problem(c["spec"])
+        probs.append((s, o, s[-1], c["params"]))
+    ot, xc = P.make_noise(cfgs[0]["noise"])
+    t0 = time.perf_counter()
+    b = P.Batch(probs, ot, xc, autotune=True)
+    build_s = time.perf_counter() - t0
+    info = b.info
+    log(f"batch of {info.n_inst}: built in {build_s:.1f} s, pairs {info.n_pairs}, "
+        f"E={info.n_edges}, waypoints={info.n_waypoints}, launch shapes "
+        f"{info.pairs_blocks_per_cu} WG/CU (pairs), {info.edges_per_block} edges/WG")
+    b.replay(0)
+    b.replay(1)
+    ms_pairs = b.replay(0, reps)
+    ms_edges = b.replay(1, reps)
+    rms = [b.roadmap(k) for k in range(info.n_inst)]
+    devs = [rm.device() for rm in rms]
+    stream = P.Stream()
+    timer = P.Timer(stream)
+
+    def single_pairs():
+        for rm, d, p in zip(rms, devs, probs):
+            P.check("pump_fill_copts_topts", P.lib.pump_fill_copts_topts(
+                d.samples, rm.info.num, p[3].tmax, d.copts, d.topts, stream.h))
+
+    def single_edges():
+        for rm, d in zip(rms, devs):
+            I = rm.info
+            P.check("pump_edges_discretize_check", P.lib.pump_edges_discretize_check(
+                d.samples, d.edge_src, d.edge_dst, d.topts_edge, I.n_edges, d.wp_offsets, 0,
+                d.obstacles, I.n_obs, d.disc, d.is_free_edges, stream.h))
+            P.check("pump_cache_halfspaces", P.lib.pump_cache_halfspaces(
+                I.n_edges, d.disc, d.is_free_edges, I.n_obs, I.max_hs, d.obstacles, 0,
+                d.wp_offsets, d.as_, d.bs, d.count_stored, I.maxd2, 1, d.edge_hs_mask,
+                stream.h))
+
+    def timed(fn):
+        fn()
+        stream.sync()
+        timer.start()
+        for _ in range(reps):
+            fn()
+        return timer.stop() / reps
+
+    ms_pairs_single = timed(single_pairs)
+    ms_edges_single = timed(single_edges)
+    log(f"batch pair pass: {ms_pairs:.2f} ms graph vs {ms_pairs_single:.2f} ms per instance; "
+        f"edge pass + half-spaces: {ms_edges:.3f} ms graph vs {ms_edges_single:.3f} ms")
+    out = {"n_inst": int(info.n_inst), "n_pairs": int(info.n_pairs),
+           "n_edges": int(info.n_edges), "n_waypoints": int(info.n_waypoints),
+           "pairs_blocks_per_cu": int(info.pairs_blocks_per_cu),
+           "edges_per_block": int(info.edges_per_block), "build_s": build_s,
+           "ms_autotune": float(info.ms_autotune),
+           "pairs": {"ms_batched": ms_pairs, "ms_per_instance": ms_pairs_single,
+                     "speedup": ms_pairs_single / ms_pairs,
+                     "pair_solves_per_s": info.n_pairs / (ms_pairs / 1e3)},
+           "edges_and_halfspaces": {"ms_batched": ms_edges, "ms_per_instance": ms_edges_single,
+                                    "speedup": ms_edges_single / ms_edges,
+                                    "edge_checks_per_s": info.n_edges / (ms_edges / 1e3)}}
+    b.close()
     return out
 
 

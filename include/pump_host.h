@@ -150,6 +150,46 @@ typedef struct pump_roadmap_dev {
 } pump_roadmap_dev;
 int pump_roadmap_device(const pump_roadmap *rm, pump_roadmap_dev *dev);
 
+/* Batched roadmap build: the reference's per-problem precompute
+ * (mainPUMP.cu:248-504) run for many independent problems at once.  The
+ * reference has no batched form; this is the multi-instance workload of
+ * 256 problems with NUM in [1000, 8000].  The pair pass of every instance is one persistent
+ * launch, the fused edge pass plus the half-space cache of every instance are
+ * two more, each captured in a hipGraph and replayable.  Launch shapes are
+ * chosen by timing the candidates on the device when autotune != 0. */
+typedef struct pump_batch pump_batch;
+
+typedef struct pump_batch_instance {
+    const float *samples;   /* host, [num * 6] */
+    int num;
+    const float *obstacles; /* host, [n_obs * 12] */
+    int n_obs;
+    const float *goal;      /* host, [6] */
+    pump_params prm;        /* per-instance parameters (fused_edges is implied) */
+} pump_batch_instance;
+
+typedef struct pump_batch_info {
+    int n_inst;
+    int pairs_blocks_per_cu;  /* resident 256-lane workgroups per CU of the pair pass */
+    int edges_per_block;      /* edges per workgroup of the edge pass (64, 128, 256) */
+    int64_t n_pairs, n_edges, n_waypoints; /* totals over the instances */
+    float ms_pairs;           /* one launch of the pair-pass graph */
+    float ms_edges;           /* one launch of the edge-pass + half-space graph */
+    float ms_autotune;        /* device time spent timing candidates */
+    float ms_total;           /* wall time of the build */
+} pump_batch_info;
+
+int pump_batch_build(int n_inst, const pump_batch_instance *inst, const float *offsets_time_host,
+                     const float *xcomb_host, int n_mc, int tmax_steps, int autotune,
+                     pump_batch **out);
+int pump_batch_info_get(const pump_batch *b, pump_batch_info *info);
+/* the roadmap of instance k, owned by the batch (pump_roadmap_free ignores it) */
+int pump_batch_roadmap(pump_batch *b, int k, pump_roadmap **rm);
+/* replays a captured stage reps times: 0 = pair pass (needs keep_pairs on
+ * every instance), 1 = edge pass + half-space cache; ms = mean per replay */
+int pump_batch_replay(pump_batch *b, int stage, int reps, float *ms);
+void pump_batch_free(pump_batch *b);
+
 typedef struct pump_search_result {
     int status;           /* 0 success, 1 no goal path below target, <0 aborted */
     int itrs;

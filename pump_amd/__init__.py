@@ -105,6 +105,18 @@ class SearchResult(ctypes.Structure):
     ]
 
 
+class BatchInstance(ctypes.Structure):
+    _fields_ = [("samples", f32p), ("num", c_int), ("obstacles", f32p), ("n_obs", c_int),
+                ("goal", f32p), ("prm", Params)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("n_inst", c_int), ("pairs_blocks_per_cu", c_int), ("edges_per_block", c_int),
+                ("n_pairs", c_int64), ("n_edges", c_int64), ("n_waypoints", c_int64),
+                ("ms_pairs", c_float), ("ms_edges", c_float), ("ms_autotune", c_float),
+                ("ms_total", c_float)]
+
+
 S = c_void_p  # stream / device pointer
 _SIGS = {
     # pump_abi.h
@@ -169,6 +181,12 @@ _SIGS = {
     "pump_roadmap_copy_halfspaces": (c_int, [c_void_p, f32p, f32p, i32p]),
     "pump_roadmap_device": (c_int, [c_void_p, ctypes.POINTER(RoadmapDev)]),
     "pump_search_run": (c_int, [c_void_p, ctypes.POINTER(Params), ctypes.POINTER(SearchResult)]),
+    "pump_batch_build": (c_int, [c_int, ctypes.POINTER(BatchInstance), f32p, f32p, c_int, c_int,
+                                 c_int, ctypes.POINTER(c_void_p)]),
+    "pump_batch_info_get": (c_int, [c_void_p, ctypes.POINTER(BatchInfo)]),
+    "pump_batch_roadmap": (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p)]),
+    "pump_batch_replay": (c_int, [c_void_p, c_int, c_int, f32p]),
+    "pump_batch_free": (None, [c_void_p]),
     "pump_find_optimal_path": (c_int, [c_float, f32p, c_int, f32p, c_int, f32p]),
     "pump_dev_alloc": (c_void_p, [c_size_t]),
     "pump_dev_free": (c_int, [c_void_p]),
@@ -303,10 +321,21 @@ class Roadmap:
         self.info = RoadmapInfo()
         check("pump_roadmap_info_get", lib.pump_roadmap_info_get(self._h, ctypes.byref(self.info)))
 
+    @classmethod
+    def _borrowed(cls, handle, owner, samples, obstacles, goal, prm):
+        """A roadmap of a batch (freed with the batch)."""
+        rm = cls.__new__(cls)
+        rm.samples, rm.obstacles, rm.goal, rm.prm = samples, obstacles, goal, prm
+        rm._h = handle
+        rm._owner = owner
+        rm.info = RoadmapInfo()
+        check("pump_roadmap_info_get", lib.pump_roadmap_info_get(rm._h, ctypes.byref(rm.info)))
+        return rm
+
     def close(self):
-        if self._h:
+        if self._h and getattr(self, "_owner", None) is None:
             lib.pump_roadmap_free(self._h)
-            self._h = c_void_p()
+        self._h = c_void_p()
 
     def __del__(self):
         try:
@@ -375,6 +404,59 @@ class Roadmap:
         check("pump_search_run",
               lib.pump_search_run(self._h, ctypes.byref(prm or self.prm), ctypes.byref(r)))
         return r
+
+
+class Batch:
+    """Roadmaps of many independent problems built together: one launch per
+    device pass for all instances, captured in hipGraphs (pump_batch_build).
+    problems: list of (samples, obstacles, goal, Params)."""
+
+    def __init__(self, problems, offsets_time, xcomb, autotune=True):
+        self._keep = []
+        arr = (BatchInstance * len(problems))()
+        for k, (samples, obstacles, goal, prm) in enumerate(problems):
+            smp = np.ascontiguousarray(samples, np.float32)
+            obs = np.ascontiguousarray(obstacles, np.float32).reshape(-1, 12)
+            obs_arg = obs if obs.size else np.zeros((1, 12), np.float32)
+            g = np.ascontiguousarray(goal, np.float32)
+            self._keep.append((smp, obs, obs_arg, g, prm))
+            arr[k].samples, arr[k].num = fp(smp), smp.shape[0]
+            arr[k].obstacles, arr[k].n_obs = fp(obs_arg), obs.shape[0]
+            arr[k].goal, arr[k].prm = fp(g), prm
+        n_mc = offsets_time.shape[1] if offsets_time is not None else 0
+        tmax_steps = offsets_time.shape[0] if offsets_time is not None else 0
+        self._h = c_void_p()
+        check("pump_batch_build",
+              lib.pump_batch_build(len(problems), arr, fp(offsets_time), fp(xcomb), n_mc,
+                                   tmax_steps, 1 if autotune else 0, ctypes.byref(self._h)))
+        self.info = BatchInfo()
+        check("pump_batch_info_get", lib.pump_batch_info_get(self._h, ctypes.byref(self.info)))
+
+    def __len__(self):
+        return self.info.n_inst
+
+    def roadmap(self, k):
+        h = c_void_p()
+        check("pump_batch_roadmap", lib.pump_batch_roadmap(self._h, k, ctypes.byref(h)))
+        smp, obs, _, g, prm = self._keep[k]
+        return Roadmap._borrowed(h, self, smp, obs, g, prm)
+
+    def replay(self, stage, reps=1):
+        """Mean ms per replay of stage 0 (pair pass) or 1 (edge pass + half-spaces)."""
+        ms = c_float()
+        check("pump_batch_replay", lib.pump_batch_replay(self._h, stage, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self._h:
+            lib.pump_batch_free(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------

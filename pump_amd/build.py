@@ -25,6 +25,7 @@ SOURCES = [
     "roadmap.hip",
     "edges.hip",
     "hsmc.hip",
+    "batch.hip",
     "hsmc_bins.hip",
     "hsmc_mfma.hip",
     "mccp.hip",

@@ -1,0 +1,350 @@
+// batch.hip -- batched roadmap build over many independent problems (host
+// C++; the kernels are in bvp.hip, edges.hip and hsmc.hip).
+//
+// Each instance is the reference's per-problem precompute
+// (mainPUMP.cu:248-504) with its own samples, boxes and parameters.  The
+// three device passes whose work grows with the instance -- all ordered
+// pairs (2pBVP.cu:133-150), the fused discretize + freeEdges pass
+// (2pBVP.cu:53-69, collisionCheck.cu:5-37) and cacheHalfspaces with the
+// setupAsBs sentinels (hsmc.cu:10-99,201-211) -- run for every instance in one
+// launch each.  The workgroups of a launch are split among the instances by a
+// prefix table in device memory, so small instances share the chip instead of
+// each leaving most CUs idle.  Each launch sequence is captured in a hipGraph
+// and can be replayed.  The launch shape of the pair pass (resident
+// workgroups per CU) and of the edge pass (edges per workgroup) is chosen by
+// timing the candidates on the device.  The host-synchronous stages between
+// the passes (rn, edge numbering, neighbour tables, waypoint offsets) run per
+// instance on the batch's one stream, as in the single-instance build.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "../../include/pump_host.h"
+#include "pump_internal.hpp"
+#include "roadmap_internal.hpp"
+
+using namespace pump;
+using namespace pump::rmb;
+
+struct pump_batch {
+    hipStream_t stream = nullptr;
+    std::vector<pump_roadmap *> rms;
+    float *offsets_time = nullptr, *xcomb = nullptr;
+    uint16_t *offsets_split = nullptr;
+    PairJob *pair_jobs = nullptr;
+    int64_t *pair_prefix = nullptr;
+    unsigned long long *counter = nullptr;
+    EdgeJob *edge_jobs = nullptr;
+    int64_t *edge_prefix = nullptr; // one table of n_inst + 1 per edges-per-block candidate
+    HsJob *hs_jobs = nullptr;
+    int64_t *hs_prefix = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    pump_batch_info info{};
+};
+
+namespace {
+
+constexpr int kEpb[3] = {64, 128, 256};    // edges per workgroup of the edge pass
+constexpr int kBpc[3] = {4, 8, 16};        // resident workgroups per CU of the pair pass
+constexpr int64_t kTunePairs = 1ll << 26;  // pairs timed per pair-pass candidate
+
+template <typename T> int upload(T **d, const std::vector<T> &h, hipStream_t s)
+{
+    TRY(dalloc(d, h.size()));
+    TRYH(hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    return PUMP_OK;
+}
+
+// Captures what launch() enqueues on the batch stream as graph `stage`.
+template <typename F> int capture(pump_batch *b, int stage, F launch)
+{
+    TRYH(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    const int r = launch();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (r != PUMP_OK || e != hipSuccess) {
+        if (g)
+            hipGraphDestroy(g);
+        if (r != PUMP_OK)
+            return r;
+        TRYH(e);
+    }
+    b->graph[stage] = g;
+    TRYH(hipGraphInstantiate(&b->exec[stage], g, nullptr, nullptr, 0));
+    return PUMP_OK;
+}
+
+int launch_timed(pump_batch *b, int stage, float *ms)
+{
+    EventTimer tm(b->stream);
+    tm.start();
+    TRYH(hipGraphLaunch(b->exec[stage], b->stream));
+    *ms = tm.stop();
+    TRYH(hipStreamSynchronize(b->stream));
+    return PUMP_OK;
+}
+
+int new_instance(pump_batch *b, const pump_batch_instance &I, int n_mc, int tmax_steps)
+{
+    pump_roadmap *rm = new pump_roadmap();
+    b->rms.push_back(rm);
+    rm->in_batch = true;
+    rm->owns_stream = false;
+    rm->owns_noise = false;
+    rm->stream = b->stream;
+    rm->num = I.num;
+    rm->n_obs = I.n_obs;
+    rm->n_mc = n_mc;
+    rm->tmax_steps = tmax_steps;
+    rm->prm = I.prm;
+    rm->prm.fused_edges = 1;
+    std::memcpy(rm->goal, I.goal, sizeof(rm->goal));
+    rm->samples_h.assign(I.samples, I.samples + (size_t)I.num * 6);
+    rm->offsets_time = b->offsets_time;
+    rm->xcomb = b->xcomb;
+    rm->offsets_split = b->offsets_split;
+    return rm_upload(rm, I.samples, I.obstacles, nullptr, nullptr);
+}
+
+// all ordered pairs of every instance: one persistent launch
+int pair_pass(pump_batch *b, int autotune)
+{
+    const int n = (int)b->rms.size();
+    hipStream_t s = b->stream;
+    std::vector<PairJob> jobs(n);
+    std::vector<int64_t> prefix(n + 1, 0);
+    for (int k = 0; k < n; ++k) {
+        const pump_roadmap *rm = b->rms[k];
+        jobs[k] = PairJob{rm->samples, rm->copts, rm->topts, rm->num, rm->prm.tmax};
+        prefix[k + 1] = prefix[k] + rm->n_pairs;
+    }
+    const int64_t total = prefix[n];
+    TRY(upload(&b->pair_jobs, jobs, s));
+    TRY(upload(&b->pair_prefix, prefix, s));
+    TRY(dalloc(&b->counter, 1));
+    int bpc = 8;
+    if (autotune) {
+        // the first kTunePairs pairs (rewritten later by the captured pass)
+        const int64_t sample = std::min(total, kTunePairs);
+        float best = 0;
+        EventTimer tm(s);
+        TRY(fill_pairs_many(b->pair_jobs, b->pair_prefix, n, sample, bpc, b->counter, s));
+        for (int c : kBpc) {
+            tm.start();
+            TRY(fill_pairs_many(b->pair_jobs, b->pair_prefix, n, sample, c, b->counter, s));
+            const float ms = tm.stop();
+            b->info.ms_autotune += ms;
+            if (best == 0 || ms < best) {
+                best = ms;
+                bpc = c;
+            }
+        }
+    }
+    b->info.pairs_blocks_per_cu = bpc;
+    TRY(capture(b, 0, [&] {
+        return fill_pairs_many(b->pair_jobs, b->pair_prefix, n, total, bpc, b->counter, s);
+    }));
+    return launch_timed(b, 0, &b->info.ms_pairs);
+}
+
+// the fused edge pass and the half-space cache of every instance: two launches
+int edge_pass(pump_batch *b, int autotune)
+{
+    const int n = (int)b->rms.size();
+    hipStream_t s = b->stream;
+    std::vector<EdgeJob> ejobs(n);
+    std::vector<HsJob> hjobs(n);
+    std::vector<int64_t> eprefix(3 * (size_t)(n + 1), 0), hprefix(n + 1, 0);
+    for (int k = 0; k < n; ++k) {
+        const pump_roadmap *rm = b->rms[k];
+        ejobs[k] = EdgeJob{rm->samples,  rm->edge_src,      rm->edge_dst, rm->topts_edge,
+                           rm->n_edges,  rm->wp_offsets,    0,            rm->obstacles,
+                           rm->n_obs,    rm->disc,          rm->is_free_edges};
+        hjobs[k] = HsJob{rm->n_edges, rm->disc,   rm->wp_offsets, 0,      rm->is_free_edges,
+                         rm->n_obs,   rm->max_hs, rm->obstacles,  rm->as, rm->bs,
+                         rm->count,   rm->maxd2,  rm->edge_hs_mask};
+        for (int c = 0; c < 3; ++c) {
+            int64_t *p = &eprefix[(size_t)c * (n + 1)];
+            p[k + 1] = p[k] + (rm->n_edges + kEpb[c] - 1) / kEpb[c];
+        }
+        hprefix[k + 1] = hprefix[k] + (rm->n_edges + kHsEdgesPerBlock - 1) / kHsEdgesPerBlock;
+    }
+    TRY(upload(&b->edge_jobs, ejobs, s));
+    TRY(upload(&b->hs_jobs, hjobs, s));
+    TRY(upload(&b->edge_prefix, eprefix, s));
+    TRY(upload(&b->hs_prefix, hprefix, s));
+    auto blocks = [&](int c) { return eprefix[(size_t)c * (n + 1) + n]; };
+    auto prefix_of = [&](int c) { return b->edge_prefix + (size_t)c * (n + 1); };
+    int pick = 1;
+    if (autotune) {
+        // every candidate writes the same waypoints and flags
+        float best = 0;
+        EventTimer tm(s);
+        TRY(edges_many(b->edge_jobs, prefix_of(1), n, blocks(1), kEpb[1], s));
+        for (int c = 0; c < 3; ++c) {
+            tm.start();
+            TRY(edges_many(b->edge_jobs, prefix_of(c), n, blocks(c), kEpb[c], s));
+            const float ms = tm.stop();
+            b->info.ms_autotune += ms;
+            if (best == 0 || ms < best) {
+                best = ms;
+                pick = c;
+            }
+        }
+    }
+    b->info.edges_per_block = kEpb[pick];
+    const int64_t hs_blocks = hprefix[n];
+    TRY(capture(b, 1, [&] {
+        int r = edges_many(b->edge_jobs, prefix_of(pick), n, blocks(pick), kEpb[pick], s);
+        if (r == PUMP_OK)
+            r = cache_hs_many(b->hs_jobs, b->hs_prefix, n, hs_blocks, s);
+        return r;
+    }));
+    return launch_timed(b, 1, &b->info.ms_edges);
+}
+
+int build(pump_batch *b, int n, const pump_batch_instance *inst, const float *offsets_time_h,
+          const float *xcomb_h, int n_mc, int tmax_steps, int autotune)
+{
+    const auto t_begin = std::chrono::steady_clock::now();
+    hipStream_t s = b->stream;
+    if (offsets_time_h && xcomb_h && n_mc > 0) {
+        // one copy of the noise tables, shared by every instance
+        const size_t nt = (size_t)tmax_steps * n_mc * 3;
+        TRY(dalloc(&b->offsets_time, nt));
+        TRY(dalloc(&b->xcomb, nt));
+        TRYH(hipMemcpyAsync(b->offsets_time, offsets_time_h, sizeof(float) * nt,
+                            hipMemcpyHostToDevice, s));
+        TRYH(hipMemcpyAsync(b->xcomb, xcomb_h, sizeof(float) * nt, hipMemcpyHostToDevice, s));
+        TRY(dalloc(&b->offsets_split,
+                   pump_split_offsets_bytes(tmax_steps, n_mc) / sizeof(uint16_t)));
+        TRY(pump_split_offsets(b->offsets_time, tmax_steps, n_mc, b->offsets_split, s));
+    }
+    for (int k = 0; k < n; ++k)
+        TRY(new_instance(b, inst[k], n_mc, tmax_steps));
+    TRY(pair_pass(b, autotune));
+    bool pairs_kept = true;
+    for (pump_roadmap *rm : b->rms) {
+        TRY(rm_tables(rm));
+        pairs_kept = pairs_kept && rm->prm.keep_pairs;
+    }
+    if (!pairs_kept) {
+        // the captured pair pass would write freed buffers
+        hipGraphExecDestroy(b->exec[0]);
+        hipGraphDestroy(b->graph[0]);
+        b->exec[0] = nullptr;
+        b->graph[0] = nullptr;
+    }
+    for (pump_roadmap *rm : b->rms)
+        TRY(rm_alloc_halfspaces(rm));
+    TRY(edge_pass(b, autotune));
+    pump_batch_info &I = b->info;
+    I.n_inst = n;
+    for (pump_roadmap *rm : b->rms) {
+        TRY(rm_finish(rm, t_begin));
+        I.n_pairs += rm->n_pairs;
+        I.n_edges += rm->n_edges;
+        I.n_waypoints += rm->n_wp;
+    }
+    I.ms_total = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() -
+                                                          t_begin)
+                     .count();
+    return PUMP_OK;
+}
+
+bool valid_instance(const pump_batch_instance &I)
+{
+    return I.samples && I.num >= 2 && I.n_obs >= 0 && (I.n_obs == 0 || I.obstacles) && I.goal;
+}
+
+} // namespace
+
+extern "C" void pump_batch_free(pump_batch *b)
+{
+    if (!b)
+        return;
+    if (b->stream)
+        hipStreamSynchronize(b->stream);
+    for (int i = 0; i < 2; ++i) {
+        if (b->exec[i])
+            hipGraphExecDestroy(b->exec[i]);
+        if (b->graph[i])
+            hipGraphDestroy(b->graph[i]);
+    }
+    for (pump_roadmap *rm : b->rms)
+        rm_release(rm);
+    dfree(b->offsets_time);
+    dfree(b->xcomb);
+    dfree(b->offsets_split);
+    dfree(b->pair_jobs);
+    dfree(b->pair_prefix);
+    dfree(b->counter);
+    dfree(b->edge_jobs);
+    dfree(b->edge_prefix);
+    dfree(b->hs_jobs);
+    dfree(b->hs_prefix);
+    if (b->stream)
+        hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int pump_batch_build(int n_inst, const pump_batch_instance *inst,
+                                const float *offsets_time_h, const float *xcomb_h, int n_mc,
+                                int tmax_steps, int autotune, pump_batch **out)
+{
+    if (!out)
+        return PUMP_E_INVALID;
+    *out = nullptr;
+    if (n_inst < 1 || !inst || n_mc < 0 || tmax_steps < 0)
+        return PUMP_E_INVALID;
+    for (int k = 0; k < n_inst; ++k)
+        if (!valid_instance(inst[k]))
+            return PUMP_E_INVALID;
+    pump_batch *b = new pump_batch();
+    const hipError_t ce = hipStreamCreate(&b->stream);
+    if (ce != hipSuccess) {
+        note_hip_error(ce, __FILE__, __LINE__);
+        delete b;
+        return PUMP_E_HIP;
+    }
+    const int r = build(b, n_inst, inst, offsets_time_h, xcomb_h, n_mc, tmax_steps, autotune);
+    if (r != PUMP_OK) {
+        pump_batch_free(b);
+        return r;
+    }
+    *out = b;
+    return PUMP_OK;
+}
+
+extern "C" int pump_batch_info_get(const pump_batch *b, pump_batch_info *info)
+{
+    if (!b || !info)
+        return PUMP_E_INVALID;
+    *info = b->info;
+    return PUMP_OK;
+}
+
+extern "C" int pump_batch_roadmap(pump_batch *b, int k, pump_roadmap **rm)
+{
+    if (!b || !rm || k < 0 || k >= (int)b->rms.size())
+        return PUMP_E_INVALID;
+    *rm = b->rms[k];
+    return PUMP_OK;
+}
+
+extern "C" int pump_batch_replay(pump_batch *b, int stage, int reps, float *ms)
+{
+    if (!b || stage < 0 || stage > 1 || reps < 1 || !b->exec[stage])
+        return PUMP_E_INVALID;
+    EventTimer tm(b->stream);
+    tm.start();
+    for (int i = 0; i < reps; ++i)
+        TRYH(hipGraphLaunch(b->exec[stage], b->stream));
+    const float t = tm.stop();
+    TRYH(hipStreamSynchronize(b->stream));
+    if (ms)
+        *ms = t / reps;
+    return PUMP_OK;
+}

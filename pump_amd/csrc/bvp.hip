@@ -60,9 +60,63 @@ __global__ __launch_bounds__(256) void k_fill_pairs(const float *__restrict__ sa
 // one toptBisection performs, so the results are identical.
 enum : int { ST_TMAX = 0, ST_LOW = 1, ST_BIS = 2 };
 
-__global__ __launch_bounds__(256) void k_fill_pairs_persistent(
-    const float *__restrict__ samples, int num, float tmax, float *__restrict__ copts,
-    float *__restrict__ topts, int64_t n_pairs, unsigned long long *__restrict__ next_chunk)
+// Pair sources: map a global pair index to the pair's cost terms, tmax and
+// output slots.  One instance (idx in the reference's output order) ...
+struct OnePairSource {
+    const float *samples;
+    int num;
+    float tmax;
+    float *copts, *topts;
+    __device__ void take(int64_t g, CostTerms &c, float &tm, float *&oc, float *&ot,
+                         int64_t &oi) const
+    {
+        int i, j;
+        pair_of(g, num, i, j);
+        float a[kDim], b[kDim];
+        load_state(samples, i, a);
+        load_state(samples, j, b);
+        cost_terms(c, a, b);
+        tm = tmax;
+        oc = copts;
+        ot = topts;
+        oi = g;
+    }
+};
+
+// ... or many: instance k owns global indices [prefix[k], prefix[k+1]).
+struct ManyPairSource {
+    const PairJob *jobs;
+    const int64_t *prefix;
+    int n_inst;
+    __device__ void take(int64_t g, CostTerms &c, float &tm, float *&oc, float *&ot,
+                         int64_t &oi) const
+    {
+        int lo = 0, hi = n_inst;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (prefix[mid] <= g)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const PairJob J = jobs[lo];
+        const int64_t idx = g - prefix[lo];
+        int i, j;
+        pair_of(idx, J.num, i, j);
+        float a[kDim], b[kDim];
+        load_state(J.samples, i, a);
+        load_state(J.samples, j, b);
+        cost_terms(c, a, b);
+        tm = J.tmax;
+        oc = J.copts;
+        ot = J.topts;
+        oi = idx;
+    }
+};
+
+template <class Src>
+__device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs,
+                                                 unsigned long long *__restrict__ next_chunk)
 {
     const int lane = threadIdx.x & 63;
     // each wave claims 64 pairs at a time from the global counter
@@ -71,7 +125,9 @@ __global__ __launch_bounds__(256) void k_fill_pairs_persistent(
     int64_t pool = 0;    // next unclaimed pair index in the wave's chunk
     int64_t pool_end = 0;
     CostTerms c;
-    float tu = 0, tl = 0, topt = 0, dcval = 0;
+    float tu = 0, tl = 0, topt = 0, dcval = 0, tmax = 0;
+    float *oc = nullptr, *ot = nullptr;
+    int64_t oi = 0;
     int state = ST_TMAX, itr = 0, halvings = 0;
     bool done_all = false;
 
@@ -101,12 +157,7 @@ __global__ __launch_bounds__(256) void k_fill_pairs_persistent(
             bool take = (my < 0) && (rank < avail);
             if (take) {
                 my = pool + rank;
-                int i, j;
-                pair_of(my, num, i, j);
-                float a[kDim], b[kDim];
-                load_state(samples, i, a);
-                load_state(samples, j, b);
-                cost_terms(c, a, b);
+                src.take(my, c, tmax, oc, ot, oi);
                 state = ST_TMAX;
                 tu = tmax;
                 tl = 0.01f;
@@ -159,12 +210,36 @@ __global__ __launch_bounds__(256) void k_fill_pairs_persistent(
                 }
             }
             if (finish) {
-                topts[my] = result;
-                copts[my] = cost_eval(c, result);
+                ot[oi] = result;
+                oc[oi] = cost_eval(c, result);
                 my = -1;
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void k_fill_pairs_persistent(
+    const float *__restrict__ samples, int num, float tmax, float *__restrict__ copts,
+    float *__restrict__ topts, int64_t n_pairs, unsigned long long *__restrict__ next_chunk)
+{
+    const OnePairSource src{samples, num, tmax, copts, topts};
+    pairs_persistent(src, n_pairs, next_chunk);
+}
+
+// all ordered pairs of many instances in one persistent launch
+__global__ __launch_bounds__(256) void k_fill_pairs_many(const PairJob *__restrict__ jobs,
+                                                         const int64_t *__restrict__ prefix,
+                                                         int n_inst, int64_t n_pairs,
+                                                         unsigned long long *__restrict__ next_chunk)
+{
+    const ManyPairSource src{jobs, prefix, n_inst};
+    pairs_persistent(src, n_pairs, next_chunk);
+}
+
+__global__ void k_zero_counter(unsigned long long *c)
+{
+    if (threadIdx.x == 0)
+        *c = 0ull;
 }
 
 // h[i] = cost(toptBisection(x_i, goal)) for i < num-1, h[num-1] = 0
@@ -327,3 +402,31 @@ extern "C" int pump_heuristic(const float *d_samples, int num, const float *goal
     PUMP_LAUNCHED();
     return PUMP_OK;
 }
+
+namespace pump {
+// Many instances' pairs in one persistent launch (jobs and prefix in device
+// memory; prefix[n_inst] = n_pairs).  The chunk counter is cleared by a
+// kernel, so the sequence can be captured in a graph.
+int fill_pairs_many(const PairJob *d_jobs, const int64_t *d_prefix, int n_inst, int64_t n_pairs,
+                    int blocks_per_cu, unsigned long long *d_counter, hipStream_t s)
+{
+    PUMP_REQUIRE(d_jobs && d_prefix && d_counter && n_inst > 0 && n_pairs >= 0 &&
+                 blocks_per_cu > 0);
+    if (n_pairs == 0)
+        return PUMP_OK;
+    int dev = 0, cus = 256;
+    PUMP_HIP(hipGetDevice(&dev));
+    PUMP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int64_t blocks = (int64_t)cus * blocks_per_cu;
+    const int64_t need = (n_pairs + 255) / 256;
+    if (blocks > need)
+        blocks = need;
+    hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, s, d_counter);
+    PUMP_LAUNCHED();
+    hipLaunchKernelGGL(k_fill_pairs_many, dim3((unsigned)blocks), dim3(256), 0, s, d_jobs,
+                       d_prefix, n_inst, n_pairs, d_counter);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+} // namespace pump
+

@@ -48,23 +48,29 @@ __device__ __forceinline__ float wave_max(float x)
 // reference's <= test, collisionCheck.cu:104-112) is rejected by the broad
 // phase for each of those segments and skipping it changes no result.
 // Waypoints are finite (tau >= the bisection's positive lower end).
-template <bool kCheck, bool kWrite>
-__global__ __launch_bounds__(kTile) void k_edges(
-    const float *__restrict__ samples, const int *__restrict__ edge_src,
-    const int *__restrict__ edge_dst, const float *__restrict__ topts_edge, int64_t n_edges,
-    const int64_t *__restrict__ wp_offsets, int w_fixed, const float *__restrict__ obstacles,
-    int n_obs, float *__restrict__ disc, uint8_t *__restrict__ is_free)
+template <bool kCheck, bool kWrite, int EPB>
+__device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0)
 {
+    const float *__restrict__ samples = J.samples;
+    const int *__restrict__ edge_src = J.edge_src;
+    const int *__restrict__ edge_dst = J.edge_dst;
+    const float *__restrict__ topts_edge = J.topts_edge;
+    const int64_t n_edges = J.n_edges;
+    const int64_t *__restrict__ wp_offsets = J.wp_offsets;
+    const int w_fixed = J.w_fixed;
+    const float *__restrict__ obstacles = J.obstacles;
+    const int n_obs = J.n_obs;
+    float *__restrict__ disc = J.disc;
+    uint8_t *__restrict__ is_free = J.is_free;
     constexpr int kWaves = kTile / 64;
     __shared__ ObsLds o;
-    __shared__ float x0s[kEdgesPerBlock][kDim], x1s[kEdgesPerBlock][kDim];
-    __shared__ TauPowers tps[kEdgesPerBlock]; // per-edge powers, reciprocals
-    __shared__ float dts[kEdgesPerBlock];     // and waypoint spacing
-    __shared__ int64_t off[kEdgesPerBlock + 1];
-    __shared__ int ok[kEdgesPerBlock];
+    __shared__ float x0s[EPB][kDim], x1s[EPB][kDim];
+    __shared__ TauPowers tps[EPB]; // per-edge powers, reciprocals
+    __shared__ float dts[EPB];     // and waypoint spacing
+    __shared__ int64_t off[EPB + 1];
+    __shared__ int ok[EPB];
 
-    const int64_t e0 = (int64_t)blockIdx.x * kEdgesPerBlock;
-    const int ne = (int)((n_edges - e0) < kEdgesPerBlock ? (n_edges - e0) : kEdgesPerBlock);
+    const int ne = (int)((n_edges - e0) < EPB ? (n_edges - e0) : EPB);
     const int t = threadIdx.x, lane = t & 63;
     const int wid = __builtin_amdgcn_readfirstlane(t >> 6);
     // boxes past the LDS capacity are tested for every segment (no mask)
@@ -161,6 +167,29 @@ __global__ __launch_bounds__(kTile) void k_edges(
         for (int q = t; q < ne; q += kTile)
             is_free[e0 + q] = (uint8_t)ok[q];
     }
+}
+
+template <bool kCheck, bool kWrite>
+__global__ __launch_bounds__(kTile) void k_edges(
+    const float *__restrict__ samples, const int *__restrict__ edge_src,
+    const int *__restrict__ edge_dst, const float *__restrict__ topts_edge, int64_t n_edges,
+    const int64_t *__restrict__ wp_offsets, int w_fixed, const float *__restrict__ obstacles,
+    int n_obs, float *__restrict__ disc, uint8_t *__restrict__ is_free)
+{
+    const EdgeJob J{samples, edge_src, edge_dst, topts_edge, n_edges, wp_offsets,
+                    w_fixed, obstacles, n_obs, disc, is_free};
+    edge_block<kCheck, kWrite, kEdgesPerBlock>(J, (int64_t)blockIdx.x * kEdgesPerBlock);
+}
+
+// the fused edge pass of many instances in one launch
+template <int EPB>
+__global__ __launch_bounds__(kTile) void k_edges_many(const EdgeJob *__restrict__ jobs,
+                                                      const int64_t *__restrict__ prefix,
+                                                      int n_inst)
+{
+    const int k = instance_of(prefix, n_inst, blockIdx.x);
+    const EdgeJob J = jobs[k];
+    edge_block<true, true, EPB>(J, ((int64_t)blockIdx.x - prefix[k]) * EPB);
 }
 
 // freeEdges over precomputed discMotions: one lane per segment; the edge's
@@ -328,3 +357,26 @@ extern "C" int pump_sample_free(const float *d_obstacles, int n_obs, const float
     PUMP_LAUNCHED();
     return PUMP_OK;
 }
+
+int pump::edges_many(const EdgeJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
+                     int64_t n_blocks, int edges_per_block, hipStream_t s)
+{
+    PUMP_REQUIRE(d_jobs && d_block_prefix && n_inst > 0 && n_blocks >= 0 &&
+                 n_blocks <= 0x7fffffff);
+    if (n_blocks == 0)
+        return PUMP_OK;
+    if (edges_per_block == 64)
+        hipLaunchKernelGGL(k_edges_many<64>, dim3((unsigned)n_blocks), dim3(kTile), 0, s, d_jobs,
+                           d_block_prefix, n_inst);
+    else if (edges_per_block == 128)
+        hipLaunchKernelGGL(k_edges_many<128>, dim3((unsigned)n_blocks), dim3(kTile), 0, s, d_jobs,
+                           d_block_prefix, n_inst);
+    else if (edges_per_block == 256)
+        hipLaunchKernelGGL(k_edges_many<256>, dim3((unsigned)n_blocks), dim3(kTile), 0, s, d_jobs,
+                           d_block_prefix, n_inst);
+    else
+        return PUMP_E_INVALID;
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+

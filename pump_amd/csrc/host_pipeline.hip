@@ -22,134 +22,27 @@
 #include "../../include/pump_host.h"
 #include "pump_device.hpp"
 #include "pump_internal.hpp"
+#include "roadmap_internal.hpp"
 
 #pragma clang fp contract(off)
 
 using namespace pump;
 
-struct pump_roadmap {
-    int num = 0, n_obs = 0, n_mc = 0, tmax_steps = 0;
-    pump_params prm{};
-    float goal[6] = {0, 0, 0, 0, 0, 0};
-    std::vector<float> samples_h;
-    hipStream_t stream = nullptr;
-    float *samples = nullptr, *obstacles = nullptr, *copts = nullptr, *topts = nullptr;
-    double *topts64 = nullptr;
-    int64_t n_pairs = 0, rn_idx = 0;
-    float rn = 0;
-    int64_t *row_start = nullptr;
-    int *edge_src = nullptr, *edge_dst = nullptr;
-    float *copts_edge = nullptr, *topts_edge = nullptr;
-    int64_t n_edges = 0;
-    int *in_degree = nullptr;
-    int max_nn = 0;
-    float *h = nullptr;
-    int64_t *wp_offsets = nullptr;
-    int64_t n_wp = 0;
-    float *disc = nullptr;
-    uint8_t *is_free_edges = nullptr, *is_free_samples = nullptr;
-    float *as = nullptr, *bs = nullptr;
-    int *count = nullptr;
-    uint64_t *edge_hs_mask = nullptr;
-    int max_hs = 0;
-    float maxd2 = 0;
-    float *offsets_time = nullptr, *xcomb = nullptr;
-    uint16_t *offsets_split = nullptr; // particle vectors of the split-BF16 test
-    void *ws = nullptr;
-    size_t ws_bytes = 0;
-    pump_roadmap_info info{};
-};
+namespace pump {
+namespace rmb {
 
-namespace {
-
-template <typename T> int dalloc(T **p, size_t n)
+int rm_upload(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
+              const float *offsets_time_h, const float *xcomb_h)
 {
-    *p = nullptr;
-    if (n == 0)
-        n = 1;
-    hipError_t e = hipMalloc((void **)p, sizeof(T) * n);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        return PUMP_E_NOMEM;
-    }
-    return PUMP_OK;
-}
-
-template <typename T> void dfree(T *&p)
-{
-    if (p)
-        hipFree((void *)p);
-    p = nullptr;
-}
-
-int ensure_ws(pump_roadmap *rm, size_t bytes)
-{
-    if (rm->ws_bytes >= bytes)
-        return PUMP_OK;
-    dfree(rm->ws);
-    char *w = nullptr;
-    int r = dalloc(&w, bytes);
-    rm->ws = w;
-    rm->ws_bytes = r == PUMP_OK ? bytes : 0;
-    return r;
-}
-
-struct EventTimer {
-    hipEvent_t a{}, b{};
-    hipStream_t s;
-    explicit EventTimer(hipStream_t st) : s(st)
-    {
-        hipEventCreate(&a);
-        hipEventCreate(&b);
-    }
-    ~EventTimer()
-    {
-        hipEventDestroy(a);
-        hipEventDestroy(b);
-    }
-    void start() { hipEventRecord(a, s); }
-    float stop()
-    {
-        hipEventRecord(b, s);
-        hipEventSynchronize(b);
-        float ms = 0;
-        hipEventElapsedTime(&ms, a, b);
-        return ms;
-    }
-};
-
-#define TRY(expr)                                                                                  \
-    do {                                                                                           \
-        int r_ = (expr);                                                                           \
-        if (r_ != PUMP_OK)                                                                         \
-            return r_;                                                                             \
-    } while (0)
-
-#define TRYH(expr)                                                                                 \
-    do {                                                                                           \
-        hipError_t e_ = (expr);                                                                    \
-        if (e_ != hipSuccess) {                                                                    \
-            ::pump::note_hip_error(e_, __FILE__, __LINE__);                                        \
-            return PUMP_E_HIP;                                                                     \
-        }                                                                                          \
-    } while (0)
-
-int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
-          const float *offsets_time_h, const float *xcomb_h)
-{
-    const pump_params &P = rm->prm;
     const int num = rm->num;
     hipStream_t s = rm->stream;
-    EventTimer tm(s);
-    auto t_begin = std::chrono::steady_clock::now();
-
     TRY(dalloc(&rm->samples, (size_t)num * 6));
     TRY(dalloc(&rm->obstacles, (size_t)rm->n_obs * 12));
     TRYH(hipMemcpyAsync(rm->samples, samples_h, sizeof(float) * num * 6, hipMemcpyHostToDevice, s));
     if (rm->n_obs)
         TRYH(hipMemcpyAsync(rm->obstacles, obstacles_h, sizeof(float) * rm->n_obs * 12,
                             hipMemcpyHostToDevice, s));
-    if (offsets_time_h && xcomb_h && rm->n_mc > 0) {
+    if (rm->owns_noise && offsets_time_h && xcomb_h && rm->n_mc > 0) {
         size_t nt = (size_t)rm->tmax_steps * rm->n_mc * 3;
         TRY(dalloc(&rm->offsets_time, nt));
         TRY(dalloc(&rm->xcomb, nt));
@@ -160,14 +53,30 @@ int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
                    pump_split_offsets_bytes(rm->tmax_steps, rm->n_mc) / sizeof(uint16_t)));
         TRY(pump_split_offsets(rm->offsets_time, rm->tmax_steps, rm->n_mc, rm->offsets_split, s));
     }
-
-    // (a) all ordered pairs
     rm->n_pairs = (int64_t)num * (num - 1);
     TRY(dalloc(&rm->copts, (size_t)rm->n_pairs));
     TRY(dalloc(&rm->topts, (size_t)rm->n_pairs));
+    return PUMP_OK;
+}
+
+// (a) all ordered pairs, and the optional FP64 polish
+int rm_pairs(pump_roadmap *rm)
+{
+    const pump_params &P = rm->prm;
+    EventTimer tm(rm->stream);
     tm.start();
-    TRY(pump_fill_copts_topts(rm->samples, num, P.tmax, rm->copts, rm->topts, s));
+    TRY(pump_fill_copts_topts(rm->samples, rm->num, P.tmax, rm->copts, rm->topts, rm->stream));
     rm->info.ms_pairs = tm.stop();
+    return PUMP_OK;
+}
+
+// rn, edges, neighbour tables, heuristic, sampleFree, waypoint offsets
+int rm_tables(pump_roadmap *rm)
+{
+    const pump_params &P = rm->prm;
+    const int num = rm->num;
+    hipStream_t s = rm->stream;
+    EventTimer tm(s);
     if (P.polish) {
         TRY(dalloc(&rm->topts64, (size_t)rm->n_pairs));
         TRY(pump_polish_topts(rm->samples, num, P.tmax, rm->topts, rm->topts64, s));
@@ -216,13 +125,22 @@ int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
     TRY(pump_sample_free(rm->obstacles, rm->n_obs, rm->samples, num, rm->is_free_samples, s));
     rm->info.ms_sample_free = tm.stop();
 
-    // waypoints and the edge pass (mainPUMP.cu:441-463)
+    // waypoint offsets (mainPUMP.cu:441-463)
     TRY(dalloc(&rm->wp_offsets, E + 1));
     TRY(ensure_ws(rm, pump_scan_workspace((int64_t)E + 1)));
     TRY(pump_waypoint_offsets(rm->topts_edge, rm->n_edges, P.w_fixed, P.wp_dt, P.w_cap,
                               rm->wp_offsets, &rm->n_wp, rm->ws, rm->ws_bytes, s));
     TRY(dalloc(&rm->disc, (size_t)rm->n_wp * 6));
     TRY(dalloc(&rm->is_free_edges, E));
+    return PUMP_OK;
+}
+
+// the fused (or, on request, split) edge pass
+int rm_edge_pass(pump_roadmap *rm)
+{
+    const pump_params &P = rm->prm;
+    hipStream_t s = rm->stream;
+    EventTimer tm(s);
     tm.start();
     if (P.fused_edges) {
         TRY(pump_edges_discretize_check(rm->samples, rm->edge_src, rm->edge_dst, rm->topts_edge,
@@ -235,8 +153,13 @@ int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
                             rm->wp_offsets, rm->disc, rm->is_free_edges, rm->n_edges, s));
     }
     rm->info.ms_edge_pass = tm.stop();
+    return PUMP_OK;
+}
 
-    // half-spaces (mainPUMP.cu:465-504)
+// half-space buffers (mainPUMP.cu:465-476)
+int rm_alloc_halfspaces(pump_roadmap *rm)
+{
+    const pump_params &P = rm->prm;
     rm->max_hs = rm->n_obs > P.max_hs ? P.max_hs : rm->n_obs;
     rm->maxd2 = pump_maxd2(P.offset_mult);
     const size_t hs = (size_t)rm->n_wp * (size_t)(rm->max_hs > 0 ? rm->max_hs : 1);
@@ -244,19 +167,30 @@ int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
     TRY(dalloc(&rm->bs, hs));
     TRY(dalloc(&rm->count, (size_t)rm->n_wp));
     TRY(dalloc(&rm->edge_hs_mask, (size_t)rm->n_edges));
+    if (rm->max_hs == 0)
+        TRY(pump_setup_as_bs((int64_t)hs, rm->bs, rm->as, rm->stream));
+    return PUMP_OK;
+}
+
+// half-spaces (mainPUMP.cu:477-504)
+int rm_halfspaces(pump_roadmap *rm)
+{
+    hipStream_t s = rm->stream;
+    EventTimer tm(s);
     tm.start();
-    if (rm->max_hs == 0) {
-        TRY(pump_setup_as_bs((int64_t)hs, rm->bs, rm->as, s));
-    }
     TRY(pump_cache_halfspaces(rm->n_edges, rm->disc, rm->is_free_edges, rm->n_obs, rm->max_hs,
                               rm->obstacles, 0, rm->wp_offsets, rm->as, rm->bs, rm->count,
                               rm->maxd2, 1, rm->edge_hs_mask, s));
     rm->info.ms_hs = tm.stop();
-    TRYH(hipStreamSynchronize(s));
+    return PUMP_OK;
+}
 
+int rm_finish(pump_roadmap *rm, std::chrono::steady_clock::time_point t_begin)
+{
+    TRYH(hipStreamSynchronize(rm->stream));
     auto t_end = std::chrono::steady_clock::now();
     pump_roadmap_info &I = rm->info;
-    I.num = num;
+    I.num = rm->num;
     I.n_obs = rm->n_obs;
     I.max_nn = rm->max_nn;
     I.max_hs = rm->max_hs;
@@ -269,16 +203,35 @@ int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
     I.rn = rm->rn;
     I.maxd2 = rm->maxd2;
     I.ms_total = std::chrono::duration<float, std::milli>(t_end - t_begin).count();
-    {
-        std::vector<uint8_t> f(E);
-        if (E)
-            TRYH(hipMemcpy(f.data(), rm->is_free_edges, E, hipMemcpyDeviceToHost));
-        int64_t c = 0;
-        for (uint8_t v : f)
-            c += v;
-        I.n_free_edges = c;
-    }
+    const size_t E = (size_t)rm->n_edges;
+    std::vector<uint8_t> f(E);
+    if (E)
+        TRYH(hipMemcpy(f.data(), rm->is_free_edges, E, hipMemcpyDeviceToHost));
+    int64_t c = 0;
+    for (uint8_t v : f)
+        c += v;
+    I.n_free_edges = c;
     return PUMP_OK;
+}
+
+} // namespace rmb
+} // namespace pump
+
+using namespace pump::rmb;
+
+namespace {
+
+int build(pump_roadmap *rm, const float *samples_h, const float *obstacles_h,
+          const float *offsets_time_h, const float *xcomb_h)
+{
+    const auto t_begin = std::chrono::steady_clock::now();
+    TRY(rm_upload(rm, samples_h, obstacles_h, offsets_time_h, xcomb_h));
+    TRY(rm_pairs(rm));
+    TRY(rm_tables(rm));
+    TRY(rm_edge_pass(rm));
+    TRY(rm_alloc_halfspaces(rm));
+    TRY(rm_halfspaces(rm));
+    return rm_finish(rm, t_begin);
 }
 
 } // namespace
@@ -382,7 +335,7 @@ extern "C" int pump_roadmap_info_get(const pump_roadmap *rm, pump_roadmap_info *
     return PUMP_OK;
 }
 
-extern "C" void pump_roadmap_free(pump_roadmap *rm)
+void pump::rmb::rm_release(pump_roadmap *rm)
 {
     if (!rm)
         return;
@@ -408,13 +361,22 @@ extern "C" void pump_roadmap_free(pump_roadmap *rm)
     dfree(rm->bs);
     dfree(rm->count);
     dfree(rm->edge_hs_mask);
-    dfree(rm->offsets_time);
-    dfree(rm->offsets_split);
-    dfree(rm->xcomb);
+    if (rm->owns_noise) {
+        dfree(rm->offsets_time);
+        dfree(rm->offsets_split);
+        dfree(rm->xcomb);
+    }
     dfree(rm->ws);
-    if (rm->stream)
+    if (rm->stream && rm->owns_stream)
         hipStreamDestroy(rm->stream);
     delete rm;
+}
+
+extern "C" void pump_roadmap_free(pump_roadmap *rm)
+{
+    if (!rm || rm->in_batch)
+        return;
+    rm_release(rm);
 }
 
 static int d2h(void *dst, const void *src, size_t bytes)

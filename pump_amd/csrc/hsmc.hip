@@ -47,17 +47,24 @@ __global__ __launch_bounds__(256) void k_setup_as_bs(int64_t n, float *__restric
 }
 
 template <bool kSentinels>
-__global__ __launch_bounds__(kHsBlock) void k_cache_hs(
-    int64_t n_edges, const float *__restrict__ disc, const int64_t *__restrict__ wp_offsets,
-    int w_fixed, const uint8_t *__restrict__ is_free, int n_obs, int max_hs,
-    const float *__restrict__ obstacles, float *__restrict__ as, float *__restrict__ bs,
-    int *__restrict__ count_stored, float maxd2, uint64_t *__restrict__ edge_mask)
+__device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0)
 {
+    const int64_t n_edges = J.n_edges;
+    const float *__restrict__ disc = J.disc;
+    const int64_t *__restrict__ wp_offsets = J.wp_offsets;
+    const int w_fixed = J.w_fixed;
+    const uint8_t *__restrict__ is_free = J.is_free;
+    const int n_obs = J.n_obs, max_hs = J.max_hs;
+    const float *__restrict__ obstacles = J.obstacles;
+    float *__restrict__ as = J.as;
+    float *__restrict__ bs = J.bs;
+    int *__restrict__ count_stored = J.count;
+    const float maxd2 = J.maxd2;
+    uint64_t *__restrict__ edge_mask = J.edge_mask;
     __shared__ ObsLds o;
     __shared__ int64_t off[kEdgesPerBlock + 1];
     __shared__ uint8_t fr[kEdgesPerBlock];
     __shared__ unsigned long long em[kEdgesPerBlock];
-    const int64_t e0 = (int64_t)blockIdx.x * kEdgesPerBlock;
     const int ne = (int)((n_edges - e0) < kEdgesPerBlock ? (n_edges - e0) : kEdgesPerBlock);
     const int t = threadIdx.x;
     stage_obstacles(o, obstacles, n_obs);
@@ -133,6 +140,29 @@ __global__ __launch_bounds__(kHsBlock) void k_cache_hs(
             edge_mask[e0 + q] = em[q];
     }
 }
+
+template <bool kSentinels>
+__global__ __launch_bounds__(kHsBlock) void k_cache_hs(
+    int64_t n_edges, const float *__restrict__ disc, const int64_t *__restrict__ wp_offsets,
+    int w_fixed, const uint8_t *__restrict__ is_free, int n_obs, int max_hs,
+    const float *__restrict__ obstacles, float *__restrict__ as, float *__restrict__ bs,
+    int *__restrict__ count_stored, float maxd2, uint64_t *__restrict__ edge_mask)
+{
+    const HsJob J{n_edges, disc, wp_offsets, w_fixed, is_free, n_obs, max_hs,
+                  obstacles, as, bs, count_stored, maxd2, edge_mask};
+    hs_block<kSentinels>(J, (int64_t)blockIdx.x * kEdgesPerBlock);
+}
+
+// the half-space cache of many instances in one launch (with sentinel rows)
+__global__ __launch_bounds__(kHsBlock) void k_cache_hs_many(const HsJob *__restrict__ jobs,
+                                                            const int64_t *__restrict__ prefix,
+                                                            int n_inst)
+{
+    const int k = instance_of(prefix, n_inst, blockIdx.x);
+    const HsJob J = jobs[k];
+    hs_block<true>(J, ((int64_t)blockIdx.x - prefix[k]) * kEdgesPerBlock);
+}
+static_assert(kEdgesPerBlock == kHsEdgesPerBlock, "edges per half-space cache block");
 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_setup_valid(uint8_t *__restrict__ valid, int64_t n_bytes,
@@ -766,6 +796,19 @@ extern "C" int pump_cache_halfspaces(int64_t n_edges, const float *d_disc_motion
                            as_stream(stream), n_edges, d_disc_motions, d_wp_offsets,
                            waypoints_count, d_is_free_edges, n_obs, max_hs, d_obstacles, d_as,
                            d_bs, d_count_stored, maxd2, d_edge_hs_mask);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+
+int pump::cache_hs_many(const HsJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
+                       int64_t n_blocks, hipStream_t s)
+{
+    PUMP_REQUIRE(d_jobs && d_block_prefix && n_inst > 0 && n_blocks >= 0 &&
+                 n_blocks <= 0x7fffffff);
+    if (n_blocks == 0)
+        return PUMP_OK;
+    hipLaunchKernelGGL(k_cache_hs_many, dim3((unsigned)n_blocks), dim3(kHsBlock), 0, s, d_jobs,
+                       d_block_prefix, n_inst);
     PUMP_LAUNCHED();
     return PUMP_OK;
 }

@@ -371,4 +371,21 @@ __device__ __forceinline__ int find_edge(const int64_t *off, int ne, int64_t g)
     return lo;
 }
 
+// index k of the instance whose blocks [prefix[k], prefix[k+1]) hold block b
+// (prefix[0] = 0 <= b < prefix[n_inst]) in a batched launch over instances
+__device__ __forceinline__ int instance_of(const int64_t *__restrict__ prefix, int n_inst,
+                                           int64_t b)
+{
+    int lo = 0, hi = n_inst;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 } // namespace pump
+

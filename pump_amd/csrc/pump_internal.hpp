@@ -39,6 +39,52 @@ void note_hip_error(hipError_t e, const char *where, int line);
             return PUMP_E_INVALID;                                                                 \
     } while (0)
 
+// One instance of a batched (multi-instance) launch: pairs, the fused edge
+// pass, and the half-space cache.  Instance k of a launch owns the blocks (or
+// pair indices) [prefix[k], prefix[k+1]) of a prefix table in device memory.
+struct PairJob {
+    const float *samples;
+    float *copts, *topts;
+    int num;
+    float tmax;
+};
+
+struct EdgeJob {
+    const float *samples;
+    const int *edge_src, *edge_dst;
+    const float *topts_edge;
+    int64_t n_edges;
+    const int64_t *wp_offsets;
+    int w_fixed;
+    const float *obstacles;
+    int n_obs;
+    float *disc;
+    uint8_t *is_free;
+};
+
+struct HsJob {
+    int64_t n_edges;
+    const float *disc;
+    const int64_t *wp_offsets;
+    int w_fixed;
+    const uint8_t *is_free;
+    int n_obs, max_hs;
+    const float *obstacles;
+    float *as, *bs;
+    int *count;
+    float maxd2;
+    uint64_t *edge_mask;
+};
+
+int fill_pairs_many(const PairJob *d_jobs, const int64_t *d_prefix, int n_inst, int64_t n_pairs,
+                    int blocks_per_cu, unsigned long long *d_counter, hipStream_t s);
+// edges per workgroup of the batched edge pass: 64, 128 or 256
+int edges_many(const EdgeJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
+               int64_t n_blocks, int edges_per_block, hipStream_t s);
+constexpr int kHsEdgesPerBlock = 32;
+int cache_hs_many(const HsJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
+                  int64_t n_blocks, hipStream_t s);
+
 // Grid size for a flat launch over n items, grid-stride beyond the cap.
 inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20)
 {

@@ -1,0 +1,151 @@
+"""Batched roadmap build (pump_batch_build) against single-instance builds.
+
+The batch runs the pair pass, the fused edge pass and the half-space cache of
+every instance in one launch each (captured in hipGraphs); every table it
+produces must be bit-identical to the single-instance build of the same
+problem, which the other parity tests pin to the CPU oracle.  Instances are
+taken from the uneven 256-problem batch workload (NUM 1000-8000, 5-50 boxes,
+per-instance noise multiplier and particle count), plus an instance with no
+boxes and a fixed waypoint count and a 40-sample instance.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import pump_amd as P
+from pump_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def download(ptr, n, dtype):
+    out = np.empty(n, dtype)
+    if n:
+        P.check("pump_dev_download",
+                P.lib.pump_dev_download(out.ctypes.data_as(ctypes.c_void_p), ptr, out.nbytes))
+    return out
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint8)
+
+
+def tables(rm):
+    d = {}
+    c, t = rm.pairs()
+    d["copts"], d["topts"] = c, t
+    d.update(rm.edges())
+    d.update(rm.flags())
+    d["disc"] = rm.waypoints()
+    d.update(rm.halfspaces())
+    dev = rm.device()
+    d["edge_hs_mask"] = download(dev.edge_hs_mask, rm.info.n_edges, np.uint64)
+    for k in ("rn", "n_edges", "n_waypoints", "max_nn", "max_hs", "n_free_edges", "rn_idx"):
+        d["info_" + k] = getattr(rm.info, k)
+    return d
+
+
+def assert_same(got, ref, label):
+    assert got.keys() == ref.keys()
+    for k in ref:
+        if isinstance(ref[k], np.ndarray):
+            assert got[k].shape == ref[k].shape, (label, k)
+            assert np.array_equal(bits(got[k]), bits(ref[k])), (label, k)
+        else:
+            assert got[k] == ref[k], (label, k)
+
+
+def problems():
+    batch = configs.batch_config()
+    order = np.argsort([b["spec"].num for b in batch])
+    out = []
+    for k in list(order[:6]) + [int(np.argmax([b["spec"].n_obs for b in batch]))]:
+        cfg = batch[int(k)]
+        prm = cfg["params"]
+        prm.keep_pairs = 1
+        s, o = P.gen_problem(cfg["spec"])
+        out.append((cfg["name"], s, o, s[-1], prm))
+    # no boxes and a fixed waypoint count; a 40-sample instance
+    cfg = configs.small(num=300, n_obs=0, w_fixed=20)
+    s, o = P.gen_problem(cfg["spec"])
+    cfg["params"].keep_pairs = 1
+    out.append(("small_num300_noboxes", s, o, s[-1], cfg["params"]))
+    cfg = configs.small(num=40, n_obs=3, w_fixed=5, k=8)
+    s, o = P.gen_problem(cfg["spec"])
+    cfg["params"].keep_pairs = 1
+    out.append(("small_num40", s, o, s[-1], cfg["params"]))
+    return out
+
+
+@pytest.fixture(scope="module")
+def noise():
+    return P.make_noise(configs.batch_config(1)[0]["noise"])
+
+
+def test_batch_matches_single_builds(noise):
+    ot, xc = noise
+    probs = problems()
+    print("instances", [p[0] for p in probs], flush=True)
+    b = P.Batch([(s, o, g, prm) for _, s, o, g, prm in probs], ot, xc, autotune=True)
+    info = b.info
+    assert info.n_inst == len(probs)
+    assert info.pairs_blocks_per_cu in (4, 8, 16)
+    assert info.edges_per_block in (64, 128, 256)
+    print("batch", info.pairs_blocks_per_cu, info.edges_per_block, info.ms_pairs,
+          info.ms_edges, flush=True)
+    singles = []
+    n_pairs = n_edges = n_wp = 0
+    for k, (name, s, o, g, prm) in enumerate(probs):
+        rm = P.Roadmap(s, o, g, ot, xc, prm)
+        ref = tables(rm)
+        singles.append((rm, ref))
+        got = tables(b.roadmap(k))
+        assert_same(got, ref, name)
+        n_pairs += rm.info.n_pairs
+        n_edges += rm.info.n_edges
+        n_wp += rm.info.n_waypoints
+    assert (info.n_pairs, info.n_edges, info.n_waypoints) == (n_pairs, n_edges, n_wp)
+
+    # graph replays rewrite the same tables
+    for stage in (1, 0, 1):
+        assert b.replay(stage, reps=2) > 0
+    for k in (0, len(probs) - 2, len(probs) - 1):
+        assert_same(tables(b.roadmap(k)), singles[k][1], probs[k][0] + " after replay")
+
+    # the search over a batch roadmap returns the single-instance result
+    for k in (0, len(probs) - 2):
+        got = P.search_result_dict(b.roadmap(k).search())
+        ref = P.search_result_dict(singles[k][0].search())
+        for key in ("status", "itrs", "num_paths", "num_goal_paths", "soln_nodes", "soln_cost",
+                    "soln_cp", "soln_hsmc_cp", "particle_steps"):
+            assert got[key] == ref[key], (probs[k][0], key)
+    for rm, _ in singles:
+        rm.close()
+    b.close()
+
+
+def test_batch_without_kept_pairs(noise):
+    """Pairs freed after the build: the pair graph is dropped (replay refused),
+    the edge graph still replays, and pump_roadmap_free leaves batch roadmaps
+    alone."""
+    ot, xc = noise
+    probs = problems()[-2:]
+    for p in probs:
+        p[4].keep_pairs = 0
+    b = P.Batch([(s, o, g, prm) for _, s, o, g, prm in probs], ot, xc, autotune=False)
+    assert b.info.pairs_blocks_per_cu == 8 and b.info.edges_per_block == 128
+    with pytest.raises(P.PumpError) as ex:
+        b.replay(0)
+    assert ex.value.code == -1
+    assert b.replay(1) > 0
+    rm = b.roadmap(1)
+    P.lib.pump_roadmap_free(rm._h)  # ignored for a batch roadmap
+    single = P.Roadmap(probs[1][1], probs[1][2], probs[1][3], ot, xc, probs[1][4])
+    assert np.array_equal(bits(b.roadmap(1).waypoints()), bits(single.waypoints()))
+    hs_b, hs_s = b.roadmap(1).halfspaces(), single.halfspaces()
+    for k in hs_s:
+        assert np.array_equal(bits(hs_b[k]), bits(hs_s[k])), k
+    single.close()
+    b.close()
