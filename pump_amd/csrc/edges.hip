@@ -3,16 +3,14 @@
 // the host loop mainPUMP.cu:327-346, then freeEdges collisionCheck.cu:5-37),
 // plus the reference-shaped freeEdges and sampleFree (sampler.cu:61-88).
 //
-// Layout of the fused kernel: a workgroup owns a run of consecutive edges
-// (kEdgesPerBlock); its per-edge data and the obstacle boxes (structure of
-// arrays, read as broadcasts) are staged once in LDS.  Each wave then walks
+// Layout of the fused kernel: a workgroup owns a run of consecutive edges;
+// its per-edge data (pathPoint constants, candidate boxes) and the obstacle
+// boxes (structure of arrays) are staged once in LDS.  Each wave then walks
 // tiles of 64 waypoints, one per lane, with no further workgroup barrier:
 // the waypoints go to discMotions (consecutive lanes write consecutive
-// 24-byte records), segment (k, k+1) is tested by the lane holding waypoint
-// k, with the far end shuffled from the next lane, against the boxes that
-// are not separated from the tile's bounding box.  An edge already known to
-// be blocked skips its remaining segments, and a lane stops scanning boxes
-// at its first hit.
+// 24-byte records), and segment (k, k+1) is tested by the lane holding
+// waypoint k, with the far end taken from the next lane, against the boxes
+// that are not separated from its edge's bounding box.
 #include "pump_device.hpp"
 #include "pump_internal.hpp"
 
@@ -24,58 +22,96 @@ namespace {
 constexpr int kEdgesPerBlock = 128;
 constexpr int kTile = 256;
 
-constexpr int kMaskWords = kMaxObsLds / 64;
-
-__device__ __forceinline__ float wave_min(float x)
+// DPP lane move: lane i of the result reads src from the lane that kCtrl
+// names (wave_shl:1 = 0x130: from lane i + 1); lanes without a source keep 0.
+template <int kCtrl> __device__ __forceinline__ float dpp(float x)
 {
-    for (int m = 32; m > 0; m >>= 1)
-        x = fminf(x, __shfl_xor(x, m, 64));
-    return x;
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xf, 0xf, false));
 }
 
-__device__ __forceinline__ float wave_max(float x)
+constexpr int kEdgeObs = 256;                  // boxes staged in LDS by the edge pass
+constexpr int kEdgeMaskWords = kEdgeObs / 64;  // candidate-mask words per edge
+constexpr int kMapCap = 3072;                  // block waypoints mapped to their edge in LDS
+
+// Bounding box of the positions of every waypoint of an edge.  The position
+// part of pathPoint (2pBVP.cu:117-131) is the cubic Hermite segment through
+// (x0, v0) and (x1, v1) over [0, tau], which lies in the convex hull of its
+// Bezier control points x0, x0 + v0 tau/3, x1 - v1 tau/3, x1.  The waypoints
+// are that cubic evaluated with FP32 coefficients and FP32 results: their
+// distance from it is below 30 * 2^-24 * S, S = |p0| + |p1| + |v0 tau| +
+// |v1 tau|, and the margin 1e-5 (1 + S) covers it with room to spare.
+__device__ __forceinline__ void edge_bbox(const float *x0, const float *x1, float tau, float *lo,
+                                          float *hi)
 {
-    for (int m = 32; m > 0; m >>= 1)
-        x = fmaxf(x, __shfl_xor(x, m, 64));
-    return x;
+    const double t3 = (double)tau / 3.0;
+    for (int d = 0; d < 3; ++d) {
+        const double p0 = x0[d], p3 = x1[d];
+        const double w0 = (double)x0[d + 3] * t3, w1 = (double)x1[d + 3] * t3;
+        const double p1 = p0 + w0, p2 = p3 - w1;
+        const double mn = fmin(fmin(p0, p1), fmin(p2, p3));
+        const double mx = fmax(fmax(p0, p1), fmax(p2, p3));
+        const double m = 1e-5 * (1.0 + fabs(p0) + fabs(p3) + 3.0 * (fabs(w0) + fabs(w1)));
+        const float l = (float)(mn - m), h = (float)(mx + m);
+        const bool finite = l == l && h == h && fabsf(l) < INFINITY && fabsf(h) < INFINITY;
+        lo[d] = finite ? l : -INFINITY;
+        hi[d] = finite ? h : INFINITY;
+    }
 }
 
-// Each wave walks its own tiles of 64 waypoints (consecutive tiles overlap
-// by one waypoint, so a tile tests 63 segments); the segment's far end comes
-// from the next lane by a shuffle.  The candidate boxes of a tile are those
-// not separated from the bounding box of the tile's waypoints: every segment
-// the tile tests lies inside it, so a box separated from it (with the
-// reference's <= test, collisionCheck.cu:104-112) is rejected by the broad
-// phase for each of those segments and skipping it changes no result.
-// Waypoints are finite (tau >= the bisection's positive lower end).
+// index q of the edge with loff[q] <= g < loff[q + 1] (local waypoint index)
+__device__ __forceinline__ int find_edge_local(const int *loff, int ne, int g)
+{
+    int lo = 0, hi = ne;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (loff[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// A workgroup owns a run of EPB consecutive edges.  Set-up, per edge: the
+// pathPoint constants, and a candidate mask over the boxes -- a box
+// separated from the edge's waypoint bounding box (with the reference's <=
+// test, collisionCheck.cu:104-112) is rejected by the broad phase for every
+// segment of the edge, so leaving it out changes no result.  Each wave then
+// walks its own tiles of 64 waypoints (consecutive tiles overlap by one
+// waypoint, so a tile tests 63 segments): a lane finds its edge in a byte
+// map of the block's waypoints, evaluates and writes its waypoint, takes the
+// far end of its segment from the next lane (DPP), and tests the segment
+// against its edge's candidate boxes only.  An edge already known to be
+// blocked skips its remaining segments, and a lane stops at its first hit.
 template <bool kCheck, bool kWrite, int EPB>
 __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0)
 {
+    static_assert(EPB <= 256, "edge index of the waypoint map is a byte");
     const float *__restrict__ samples = J.samples;
     const int *__restrict__ edge_src = J.edge_src;
     const int *__restrict__ edge_dst = J.edge_dst;
     const float *__restrict__ topts_edge = J.topts_edge;
-    const int64_t n_edges = J.n_edges;
     const int64_t *__restrict__ wp_offsets = J.wp_offsets;
-    const int w_fixed = J.w_fixed;
     const float *__restrict__ obstacles = J.obstacles;
     const int n_obs = J.n_obs;
-    float *__restrict__ disc = J.disc;
     uint8_t *__restrict__ is_free = J.is_free;
     constexpr int kWaves = kTile / 64;
-    __shared__ ObsLds o;
+    __shared__ ObsLdsN<kEdgeObs> o;
     __shared__ float x0s[EPB][kDim], x1s[EPB][kDim];
-    __shared__ TauPowers tps[EPB]; // per-edge powers, reciprocals
-    __shared__ float dts[EPB];     // and waypoint spacing
-    __shared__ int64_t off[EPB + 1];
-    __shared__ int ok[EPB];
+    __shared__ TauPowers tps[EPB];  // per-edge powers, reciprocals,
+    __shared__ EdgePoly polys[EPB]; // pathPoint constants,
+    __shared__ float dts[EPB];      // waypoint spacing,
+    __shared__ int loff[EPB + 1];   // first waypoint (block-local),
+    __shared__ int ok[EPB];         // flag,
+    __shared__ unsigned long long emask[kCheck ? EPB : 1][kEdgeMaskWords]; // candidate boxes
+    __shared__ uint8_t wmap[kMapCap]; // edge of each block-local waypoint
 
-    const int ne = (int)((n_edges - e0) < EPB ? (n_edges - e0) : EPB);
+    const int ne = (int)((J.n_edges - e0) < EPB ? (J.n_edges - e0) : EPB);
     const int t = threadIdx.x, lane = t & 63;
     const int wid = __builtin_amdgcn_readfirstlane(t >> 6);
-    // boxes past the LDS capacity are tested for every segment (no mask)
-    const int n_masked = n_obs < kMaxObsLds ? n_obs : kMaxObsLds;
+    const int n_masked = n_obs < kEdgeObs ? n_obs : kEdgeObs; // boxes with candidate masks
     const int n_words = (n_masked + 63) >> 6;
+    const int64_t g_begin = wp_offsets ? wp_offsets[e0] : e0 * (int64_t)J.w_fixed;
     if (kCheck)
         stage_obstacles(o, obstacles, n_obs);
     for (int q = t; q < ne * kDim; q += kTile) {
@@ -84,72 +120,87 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0)
         x1s[le][d] = samples[(int64_t)edge_dst[e0 + le] * kDim + d];
     }
     for (int q = t; q <= ne; q += kTile)
-        off[q] = wp_offsets ? wp_offsets[e0 + q] : (e0 + q) * (int64_t)w_fixed;
+        loff[q] = (int)((wp_offsets ? wp_offsets[e0 + q] : (e0 + q) * (int64_t)J.w_fixed) -
+                        g_begin);
+    if (kCheck)
+        for (int q = t; q < ne * kEdgeMaskWords; q += kTile)
+            emask[q / kEdgeMaskWords][q % kEdgeMaskWords] = 0ull;
     __syncthreads();
-    for (int q = t; q < ne; q += kTile) {
+    const int n_loc = loff[ne];
+    const bool mapped = n_loc <= kMapCap;
+    // kTile / EPB lanes per edge; part 0 forms the edge's constants, and
+    // every part sets the candidate bits of the boxes b = part (mod parts)
+    constexpr int kParts = kTile / EPB;
+    const int q = t % EPB;
+    const int part = __builtin_amdgcn_readfirstlane(t / EPB); // uniform within a wave
+    if (q < ne) {
         const float tau = topts_edge[e0 + q];
-        tps[q] = tau_powers(tau);
-        dts[q] = tau / (float)(off[q + 1] - off[q] - 1);
-        ok[q] = 1;
+        if (part == 0) {
+            const TauPowers P = tau_powers(tau);
+            tps[q] = P;
+            polys[q] = edge_poly(x0s[q], x1s[q], P);
+            const int a = loff[q], W = loff[q + 1] - a;
+            dts[q] = tau / (float)(W - 1);
+            ok[q] = 1;
+            if (mapped)
+                for (int j = 0; j < W; ++j)
+                    wmap[a + j] = (uint8_t)q;
+        }
+        if (kCheck) {
+            float lo[3], hi[3];
+            edge_bbox(x0s[q], x1s[q], tau, lo, hi);
+            for (int wd = 0; wd < n_words; ++wd) {
+                unsigned long long m = 0ull;
+                const int b_end = n_masked < wd * 64 + 64 ? n_masked : wd * 64 + 64;
+                for (int b = wd * 64 + part; b < b_end; b += kParts) {
+                    const float *ob = obstacles + b * 12;
+                    const bool sep = (hi[0] <= ob[0]) | (ob[6] <= lo[0]) | (hi[1] <= ob[1]) |
+                                     (ob[7] <= lo[1]) | (hi[2] <= ob[2]) | (ob[8] <= lo[2]);
+                    m |= (unsigned long long)(!sep) << (b - wd * 64);
+                }
+                if (m)
+                    atomicOr(&emask[q][wd], m);
+            }
+        }
     }
     __syncthreads();
 
-    const int64_t g_begin = off[0], g_end = off[ne];
+    float *__restrict__ disc = J.disc ? J.disc + g_begin * kDim : nullptr;
     // tiles start every 63 waypoints; tile j belongs to wave j % kWaves
-    for (int64_t base = g_begin + (int64_t)wid * 63; base < g_end; base += (int64_t)kWaves * 63) {
-        const int64_t g = base + lane;
-        const bool have = g < g_end;
+    for (int base = wid * 63; base < n_loc; base += kWaves * 63) {
+        const int g = base + lane;
+        const bool have = g < n_loc;
         int le = 0, k = 0, W = 0;
         float x[kDim] = {0, 0, 0, 0, 0, 0};
         if (have) {
-            le = find_edge(off, ne, g);
-            k = (int)(g - off[le]);
-            W = (int)(off[le + 1] - off[le]);
-            const TauPowers P = tps[le];
-            waypoint_dt(k, W, P, dts[le], x0s[le], x1s[le], x);
+            le = mapped ? (int)wmap[g] : find_edge_local(loff, ne, g);
+            k = g - loff[le];
+            W = loff[le + 1] - loff[le];
+            waypoint_dt(k, W, tps[le], polys[le], dts[le], x0s[le], x1s[le], x);
             // lane 63's waypoint is lane 0's of the next tile
-            if (kWrite && (lane < 63 || g + 1 >= g_end))
+            if (kWrite && (lane < 63 || g + 1 >= n_loc))
                 for (int d = 0; d < kDim; ++d)
-                    disc[g * kDim + d] = x[d];
+                    disc[(int64_t)g * kDim + d] = x[d];
         }
         if (!kCheck)
             continue;
         // segment (k, k+1): far end from the next lane
         float w[3];
         for (int d = 0; d < 3; ++d)
-            w[d] = __shfl_down(x[d], 1, 64);
+            w[d] = dpp<0x130>(x[d]);
         const bool test = have && lane < 63 && k < W - 1 && ok[le];
-        // candidate boxes of the tile: not separated from its waypoints' box
-        float tlo[3], thi[3];
-        for (int d = 0; d < 3; ++d) {
-            tlo[d] = wave_min(have ? x[d] : INFINITY);
-            thi[d] = wave_max(have ? x[d] : -INFINITY);
-        }
-        if (__ballot(test) == 0ull)
-            continue;
-        uint64_t cand[kMaskWords];
-        for (int wd = 0; wd < n_words; ++wd) {
-            const int b = wd * 64 + lane;
-            bool near = false;
-            if (b < n_masked) {
-                near = true;
-                for (int d = 0; d < 3; ++d)
-                    near = near && !(thi[d] <= o.lo[d][b] || o.hi[d][b] <= tlo[d]);
-            }
-            cand[wd] = __ballot(near);
-        }
         if (test) {
             const float v[3] = {x[0], x[1], x[2]};
             float bmin[3], bmax[3];
             seg_bbox(v, w, bmin, bmax);
             bool hit = false;
             for (int wd = 0; wd < n_words && !hit; ++wd) {
-                uint64_t m = cand[wd];
+                unsigned long long m = emask[le][wd];
                 while (m && !hit) {
                     const int b = wd * 64 + __builtin_ctzll(m);
                     m &= m - 1;
-                    float lo[3] = {o.lo[0][b], o.lo[1][b], o.lo[2][b]};
-                    float hi[3] = {o.hi[0][b], o.hi[1][b], o.hi[2][b]};
+                    const float lo[3] = {o.lo[0][b], o.lo[1][b], o.lo[2][b]};
+                    const float hi[3] = {o.hi[0][b], o.hi[1][b], o.hi[2][b]};
                     hit = segment_hits_box(v, w, bmin, bmax, lo, hi);
                 }
             }
@@ -281,6 +332,7 @@ extern "C" int pump_discretize_edges(const float *d_samples, const int *d_edge_s
         return PUMP_OK;
     PUMP_REQUIRE(d_samples && d_edge_src && d_edge_dst && d_topts_edge && d_disc_motions);
     PUMP_REQUIRE(d_wp_offsets || waypoints_count >= 2);
+    PUMP_REQUIRE(waypoints_count < kMaxWaypointsPerEdge);
     unsigned blocks;
     int r = edge_blocks(n_edges, blocks);
     if (r)
@@ -305,6 +357,7 @@ extern "C" int pump_edges_discretize_check(const float *d_samples, const int *d_
     PUMP_REQUIRE(d_samples && d_edge_src && d_edge_dst && d_topts_edge && d_is_free_edges);
     PUMP_REQUIRE(n_obs == 0 || d_obstacles);
     PUMP_REQUIRE(d_wp_offsets || waypoints_count >= 2);
+    PUMP_REQUIRE(waypoints_count < kMaxWaypointsPerEdge);
     unsigned blocks;
     int r = edge_blocks(n_edges, blocks);
     if (r)

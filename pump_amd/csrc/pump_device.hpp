@@ -182,57 +182,79 @@ __host__ __device__ __forceinline__ double div_rn(double a, double b, double r)
     return fma(e1, r, q1);
 }
 
-__host__ __device__ __forceinline__ float pp_pos_xy(float t, double s2, double s3, const TauPowers &P,
-                                           float p0, float v0, float p1, float v1)
+// Per-edge constants of pathPoint (2pBVP.cu:117-131): every subexpression
+// that depends only on (x0, x1, tau), formed with the same operations in the
+// same order as the full expression, so a waypoint evaluated from them is
+// bit-identical to the expression evaluated from scratch.  Per waypoint only
+// the terms in t remain (about 100 FP64 operations instead of 165).
+struct EdgePoly {
+    double c3[2], c2[2], cv[2]; // x, y: cubic and quadratic position terms, velocity term
+    double bz, kz, dz;          // z position: z0 tau^3, bracket of s^2 tau, bracket of s^3
+    double av, bv, cz;          // z velocity: vz0 tau^3, bracket of t tau, bracket of s^2
+    float e[2];                 // x, y: velocity term of -(2 t e)
+};
+
+__host__ __device__ __forceinline__ EdgePoly edge_poly(const float *x0, const float *x1,
+                                                       const TauPowers &P)
 {
+    EdgePoly E;
     const float tau = P.tau;
-    float lin = t * v0 + p0;
-    float c3 = 2.0f * p0 - 2.0f * p1 + (v0 + v1) * tau;
-    float c2 = 3.0f * p0 - 3.0f * p1 + (2.0f * v0 + v1) * tau;
-    return (float)((double)lin + div_rn(s3 * (double)c3, P.t3, P.r3) -
-                   div_rn(s2 * (double)c2, P.t2, P.r2));
+    for (int d = 0; d < 2; ++d) {
+        const float p0 = x0[d], v0 = x0[d + 3], p1 = x1[d], v1 = x1[d + 3];
+        const float c3 = 2.0f * p0 - 2.0f * p1 + (v0 + v1) * tau;
+        const float c2 = 3.0f * p0 - 3.0f * p1 + (2.0f * v0 + v1) * tau;
+        const float c = 2.0f * p0 - 2.0f * p1 + v1 * tau;
+        E.c3[d] = (double)c3;
+        E.c2[d] = (double)c2;
+        E.cv[d] = (double)c;
+        E.e[d] = 3.0f * p0 - 3.0f * p1 + v1 * tau;
+    }
+    const double z0 = (double)x0[2], z1 = (double)x1[2];
+    const double vz0 = (double)x0[5], vz1 = (double)x1[5], td = (double)tau;
+    E.bz = z0 * P.t3;
+    E.kz = -3.0 * z0 + 3.0 * z1 - 2.0 * vz0 * td - 1.0 * vz1 * td;
+    E.dz = 2.0 * z0 - 2.0 * z1 + (double)((x0[5] + x1[5]) * tau);
+    E.av = vz0 * P.t3;
+    E.bv = -6.0 * z0 + 6.0 * z1 - 4.0 * vz0 * td - 2.0 * vz1 * td;
+    E.cz = 6.0 * z0 - 6.0 * z1 + 3.0 * vz0 * td + 3.0 * vz1 * td;
+    return E;
 }
 
-__host__ __device__ __forceinline__ float pp_vel_xy(float t, double s2, const TauPowers &P, float p0,
-                                           float v0, float p1, float v1)
-{
-    const float tau = P.tau;
-    float e = 3.0f * p0 - 3.0f * p1 + v1 * tau;
-    float f = -(2.0f * t * e);
-    float c = 2.0f * p0 - 2.0f * p1 + v1 * tau;
-    double q = 3.0 * s2 - (double)(4.0f * t * tau) + P.t2;
-    return (float)(div_rn((double)f, P.t2, P.r2) + div_rn(3.0 * s2 * (double)c, P.t3, P.r3) +
-                   div_rn((double)v0 * q, P.t2, P.r2));
-}
-
-__host__ __device__ __forceinline__ void path_point(float t, const TauPowers &P, const float *x0,
-                                           const float *x1, float *x)
+// pathPoint at time t from the edge's constants; x0 supplies p0, v0 of x, y
+// and the z velocity
+__host__ __device__ __forceinline__ void path_point(float t, const TauPowers &P, const EdgePoly &E,
+                                                    const float *x0, float *x)
 {
     const float tau = P.tau;
     const double s2 = (double)t * (double)t;
     const double s3 = s2 * (double)t;
-    x[0] = pp_pos_xy(t, s2, s3, P, x0[0], x0[3], x1[0], x1[3]);
-    x[1] = pp_pos_xy(t, s2, s3, P, x0[1], x0[4], x1[1], x1[4]);
-    {
-        double a = (double)(t * x0[5]) * P.t3;
-        double b = (double)x0[2] * P.t3;
-        double c = s2 * (double)tau *
-                   (-3.0 * (double)x0[2] + 3.0 * (double)x1[2] - 2.0 * (double)x0[5] * (double)tau -
-                    1.0 * (double)x1[5] * (double)tau);
-        double d = s3 * (2.0 * (double)x0[2] - 2.0 * (double)x1[2] + (double)((x0[5] + x1[5]) * tau));
-        x[2] = (float)div_rn(a + b + c + d, P.t3, P.r3);
+    const double q = 3.0 * s2 - (double)(4.0f * t * tau) + P.t2;
+    for (int d = 0; d < 2; ++d) {
+        const float p0 = x0[d], v0 = x0[d + 3];
+        const float lin = t * v0 + p0;
+        x[d] = (float)((double)lin + div_rn(s3 * E.c3[d], P.t3, P.r3) -
+                       div_rn(s2 * E.c2[d], P.t2, P.r2));
+        const float f = -(2.0f * t * E.e[d]);
+        x[d + 3] = (float)(div_rn((double)f, P.t2, P.r2) + div_rn(3.0 * s2 * E.cv[d], P.t3, P.r3) +
+                           div_rn((double)v0 * q, P.t2, P.r2));
     }
-    x[3] = pp_vel_xy(t, s2, P, x0[0], x0[3], x1[0], x1[3]);
-    x[4] = pp_vel_xy(t, s2, P, x0[1], x0[4], x1[1], x1[4]);
     {
-        double a = (double)x0[5] * P.t3;
-        double b = (double)(t * tau) *
-                   (-6.0 * (double)x0[2] + 6.0 * (double)x1[2] - 4.0 * (double)x0[5] * (double)tau -
-                    2.0 * (double)x1[5] * (double)tau);
-        double c = s2 * (6.0 * (double)x0[2] - 6.0 * (double)x1[2] + 3.0 * (double)x0[5] * (double)tau +
-                         3.0 * (double)x1[5] * (double)tau);
-        x[5] = (float)div_rn(a + b + c, P.t3, P.r3);
+        const double a = (double)(t * x0[5]) * P.t3;
+        const double c = s2 * (double)tau * E.kz;
+        const double d = s3 * E.dz;
+        x[2] = (float)div_rn(a + E.bz + c + d, P.t3, P.r3);
     }
+    {
+        const double b = (double)(t * tau) * E.bv;
+        const double c = s2 * E.cz;
+        x[5] = (float)div_rn(E.av + b + c, P.t3, P.r3);
+    }
+}
+
+__host__ __device__ __forceinline__ void path_point(float t, const TauPowers &P, const float *x0,
+                                                    const float *x1, float *x)
+{
+    path_point(t, P, edge_poly(x0, x1, P), x0, x);
 }
 
 // Waypoint k of an edge with W waypoints (findDiscretizedPath,
@@ -249,6 +271,21 @@ __host__ __device__ __forceinline__ void waypoint_dt(int k, int W, const TauPowe
             x[d] = x1[d];
     } else {
         path_point(dt * (float)k, P, x0, x1, x);
+    }
+}
+
+__host__ __device__ __forceinline__ void waypoint_dt(int k, int W, const TauPowers &P,
+                                                     const EdgePoly &E, float dt, const float *x0,
+                                                     const float *x1, float *x)
+{
+    if (k == 0) {
+        for (int d = 0; d < kDim; ++d)
+            x[d] = x0[d];
+    } else if (k == W - 1) {
+        for (int d = 0; d < kDim; ++d)
+            x[d] = x1[d];
+    } else {
+        path_point(dt * (float)k, P, E, x0, x);
     }
 }
 
@@ -310,14 +347,16 @@ __host__ __device__ __forceinline__ void seg_bbox(const float *v, const float *w
 // ---------------------------------------------------------------------------
 constexpr int kMaxObsLds = 512;
 
-struct ObsLds {
-    float lo[3][kMaxObsLds];
-    float hi[3][kMaxObsLds];
+template <int N> struct ObsLdsN {
+    float lo[3][N];
+    float hi[3][N];
 };
+using ObsLds = ObsLdsN<kMaxObsLds>;
 
-__device__ __forceinline__ void stage_obstacles(ObsLds &o, const float *__restrict__ obs, int n)
+template <int N>
+__device__ __forceinline__ void stage_obstacles(ObsLdsN<N> &o, const float *__restrict__ obs, int n)
 {
-    const int m = n < kMaxObsLds ? n : kMaxObsLds;
+    const int m = n < N ? n : N;
     for (int q = threadIdx.x; q < m * 3; q += blockDim.x) {
         int b = q / 3, d = q - 3 * (q / 3);
         o.lo[d][b] = obs[b * 12 + d];
@@ -325,10 +364,11 @@ __device__ __forceinline__ void stage_obstacles(ObsLds &o, const float *__restri
     }
 }
 
-__device__ __forceinline__ void box_bounds(const ObsLds &o, const float *__restrict__ obs, int b,
+template <int N>
+__device__ __forceinline__ void box_bounds(const ObsLdsN<N> &o, const float *__restrict__ obs, int b,
                                            float *lo, float *hi)
 {
-    if (b < kMaxObsLds) {
+    if (b < N) {
         for (int d = 0; d < 3; ++d) {
             lo[d] = o.lo[d][b];
             hi[d] = o.hi[d][b];

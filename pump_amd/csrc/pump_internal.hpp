@@ -39,6 +39,10 @@ void note_hip_error(hipError_t e, const char *where, int line);
             return PUMP_E_INVALID;                                                                 \
     } while (0)
 
+// Waypoints per edge stay below this bound (the edge pass indexes the
+// waypoints of a run of up to 256 edges with 32-bit integers).
+constexpr int kMaxWaypointsPerEdge = 1 << 23;
+
 // One instance of a batched (multi-instance) launch: pairs, the fused edge
 // pass, and the half-space cache.  Instance k of a launch owns the blocks (or
 // pair indices) [prefix[k], prefix[k+1]) of a prefix table in device memory.

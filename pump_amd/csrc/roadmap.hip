@@ -600,6 +600,7 @@ extern "C" int pump_waypoint_offsets(const float *d_topts_edge, int64_t n_edges,
 {
     PUMP_REQUIRE(d_wp_offsets && total_host && n_edges >= 0);
     PUMP_REQUIRE(w_fixed >= 2 || (w_fixed <= 0 && w_cap >= 2 && wp_dt > 0.0f && d_topts_edge));
+    PUMP_REQUIRE(w_fixed < kMaxWaypointsPerEdge && w_cap < kMaxWaypointsPerEdge);
     hipStream_t s = as_stream(stream);
     if (n_edges > 0) {
         hipLaunchKernelGGL(k_wcount, dim3(grid_for(n_edges, 256)), dim3(256), 0, s, d_topts_edge,
