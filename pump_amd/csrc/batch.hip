@@ -175,6 +175,9 @@ int edge_pass(pump_batch *b, int autotune)
     TRY(upload(&b->hs_jobs, hjobs, s));
     TRY(upload(&b->edge_prefix, eprefix, s));
     TRY(upload(&b->hs_prefix, hprefix, s));
+    int max_obs = 0;
+    for (const pump_roadmap *rm : b->rms)
+        max_obs = std::max(max_obs, rm->n_obs);
     auto blocks = [&](int c) { return eprefix[(size_t)c * (n + 1) + n]; };
     auto prefix_of = [&](int c) { return b->edge_prefix + (size_t)c * (n + 1); };
     int pick = 1;
@@ -182,10 +185,10 @@ int edge_pass(pump_batch *b, int autotune)
         // every candidate writes the same waypoints and flags
         float best = 0;
         EventTimer tm(s);
-        TRY(edges_many(b->edge_jobs, prefix_of(1), n, blocks(1), kEpb[1], s));
+        TRY(edges_many(b->edge_jobs, prefix_of(1), n, blocks(1), kEpb[1], max_obs, s));
         for (int c = 0; c < 3; ++c) {
             tm.start();
-            TRY(edges_many(b->edge_jobs, prefix_of(c), n, blocks(c), kEpb[c], s));
+            TRY(edges_many(b->edge_jobs, prefix_of(c), n, blocks(c), kEpb[c], max_obs, s));
             const float ms = tm.stop();
             b->info.ms_autotune += ms;
             if (best == 0 || ms < best) {
@@ -197,7 +200,8 @@ int edge_pass(pump_batch *b, int autotune)
     b->info.edges_per_block = kEpb[pick];
     const int64_t hs_blocks = hprefix[n];
     TRY(capture(b, 1, [&] {
-        int r = edges_many(b->edge_jobs, prefix_of(pick), n, blocks(pick), kEpb[pick], s);
+        int r = edges_many(b->edge_jobs, prefix_of(pick), n, blocks(pick), kEpb[pick], max_obs,
+                           s);
         if (r == PUMP_OK)
             r = cache_hs_many(b->hs_jobs, b->hs_prefix, n, hs_blocks, s);
         return r;

@@ -11,6 +11,8 @@
 // 24-byte records), and segment (k, k+1) is tested by the lane holding
 // waypoint k, with the far end taken from the next lane, against the boxes
 // that are not separated from its edge's bounding box.
+#include <cstdlib>
+
 #include "pump_device.hpp"
 #include "pump_internal.hpp"
 
@@ -32,6 +34,40 @@ template <int kCtrl> __device__ __forceinline__ float dpp(float x)
 constexpr int kEdgeObs = 256;                  // boxes staged in LDS by the edge pass
 constexpr int kEdgeMaskWords = kEdgeObs / 64;  // candidate-mask words per edge
 constexpr int kMapCap = 3072;                  // block waypoints mapped to their edge in LDS
+
+// LDS of an edge-pass workgroup, sized at launch: the per-edge arrays for
+// EPB edges, candidate masks of words_cap words per edge and obs_cap staged
+// boxes (so a workload with few boxes keeps more workgroups resident).
+struct EdgeLds {
+    int tps, polys, emask, x0, x1, dts, loff, ok, obs, wmap, bytes;
+};
+
+__host__ __device__ inline EdgeLds edge_lds(int epb, int obs_cap, int words_cap)
+{
+    EdgeLds L{};
+    int o = 0;
+    auto take = [&o](int bytes, int align) {
+        o = (o + align - 1) / align * align;
+        const int at = o;
+        o += bytes;
+        return at;
+    };
+    L.tps = take(epb * (int)sizeof(TauPowers), 8);
+    L.polys = take(epb * (int)sizeof(EdgePoly), 8);
+    L.emask = take(epb * words_cap * 8, 8);
+    L.x0 = take(epb * kDim * 4, 4);
+    L.x1 = take(epb * kDim * 4, 4);
+    L.dts = take(epb * 4, 4);
+    L.loff = take((epb + 1) * 4, 4);
+    L.ok = take(epb * 4, 4);
+    L.obs = take(obs_cap * 6 * 4, 4);
+    L.wmap = take(kMapCap, 4);
+    L.bytes = (o + 15) / 16 * 16;
+    return L;
+}
+
+__host__ __device__ inline int edge_obs_cap(int n_obs) { return n_obs < kEdgeObs ? n_obs : kEdgeObs; }
+__host__ __device__ inline int edge_words(int obs_cap) { return (obs_cap + 63) >> 6; }
 
 // Bounding box of the positions of every waypoint of an edge.  The position
 // part of pathPoint (2pBVP.cu:117-131) is the cubic Hermite segment through
@@ -84,7 +120,8 @@ __device__ __forceinline__ int find_edge_local(const int *loff, int ne, int g)
 // against its edge's candidate boxes only.  An edge already known to be
 // blocked skips its remaining segments, and a lane stops at its first hit.
 template <bool kCheck, bool kWrite, int EPB>
-__device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0)
+__device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *smem, int obs_cap,
+                                           int words_cap)
 {
     static_assert(EPB <= 256, "edge index of the waypoint map is a byte");
     const float *__restrict__ samples = J.samples;
@@ -96,24 +133,31 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0)
     const int n_obs = J.n_obs;
     uint8_t *__restrict__ is_free = J.is_free;
     constexpr int kWaves = kTile / 64;
-    __shared__ ObsLdsN<kEdgeObs> o;
-    __shared__ float x0s[EPB][kDim], x1s[EPB][kDim];
-    __shared__ TauPowers tps[EPB];  // per-edge powers, reciprocals,
-    __shared__ EdgePoly polys[EPB]; // pathPoint constants,
-    __shared__ float dts[EPB];      // waypoint spacing,
-    __shared__ int loff[EPB + 1];   // first waypoint (block-local),
-    __shared__ int ok[EPB];         // flag,
-    __shared__ unsigned long long emask[kCheck ? EPB : 1][kEdgeMaskWords]; // candidate boxes
-    __shared__ uint8_t wmap[kMapCap]; // edge of each block-local waypoint
+    const EdgeLds L = edge_lds(EPB, obs_cap, words_cap);
+    TauPowers *tps = reinterpret_cast<TauPowers *>(smem + L.tps);   // powers, reciprocals
+    EdgePoly *polys = reinterpret_cast<EdgePoly *>(smem + L.polys); // pathPoint constants
+    unsigned long long *emask = reinterpret_cast<unsigned long long *>(smem + L.emask);
+    float(*x0s)[kDim] = reinterpret_cast<float(*)[kDim]>(smem + L.x0);
+    float(*x1s)[kDim] = reinterpret_cast<float(*)[kDim]>(smem + L.x1);
+    float *dts = reinterpret_cast<float *>(smem + L.dts);   // waypoint spacing
+    int *loff = reinterpret_cast<int *>(smem + L.loff);      // first waypoint (block-local)
+    int *ok = reinterpret_cast<int *>(smem + L.ok);          // flag
+    float *olo = reinterpret_cast<float *>(smem + L.obs);    // staged boxes: lo[d][b], hi[d][b]
+    float *ohi = olo + 3 * obs_cap;
+    uint8_t *wmap = reinterpret_cast<uint8_t *>(smem + L.wmap); // edge of each local waypoint
 
     const int ne = (int)((J.n_edges - e0) < EPB ? (J.n_edges - e0) : EPB);
     const int t = threadIdx.x, lane = t & 63;
     const int wid = __builtin_amdgcn_readfirstlane(t >> 6);
-    const int n_masked = n_obs < kEdgeObs ? n_obs : kEdgeObs; // boxes with candidate masks
-    const int n_words = (n_masked + 63) >> 6;
+    const int n_masked = edge_obs_cap(n_obs); // staged boxes with candidate masks
+    const int n_words = edge_words(n_masked);
     const int64_t g_begin = wp_offsets ? wp_offsets[e0] : e0 * (int64_t)J.w_fixed;
     if (kCheck)
-        stage_obstacles(o, obstacles, n_obs);
+        for (int q = t; q < n_masked * 3; q += kTile) {
+            const int b = q / 3, d = q - 3 * (q / 3);
+            olo[d * obs_cap + b] = obstacles[b * 12 + d];
+            ohi[d * obs_cap + b] = obstacles[b * 12 + 6 + d];
+        }
     for (int q = t; q < ne * kDim; q += kTile) {
         int le = q / kDim, d = q - kDim * (q / kDim);
         x0s[le][d] = samples[(int64_t)edge_src[e0 + le] * kDim + d];
@@ -123,8 +167,8 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0)
         loff[q] = (int)((wp_offsets ? wp_offsets[e0 + q] : (e0 + q) * (int64_t)J.w_fixed) -
                         g_begin);
     if (kCheck)
-        for (int q = t; q < ne * kEdgeMaskWords; q += kTile)
-            emask[q / kEdgeMaskWords][q % kEdgeMaskWords] = 0ull;
+        for (int q = t; q < ne * n_words; q += kTile)
+            emask[q] = 0ull;
     __syncthreads();
     const int n_loc = loff[ne];
     const bool mapped = n_loc <= kMapCap;
@@ -159,7 +203,7 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0)
                     m |= (unsigned long long)(!sep) << (b - wd * 64);
                 }
                 if (m)
-                    atomicOr(&emask[q][wd], m);
+                    atomicOr(&emask[q * n_words + wd], m);
             }
         }
     }
@@ -195,19 +239,23 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0)
             seg_bbox(v, w, bmin, bmax);
             bool hit = false;
             for (int wd = 0; wd < n_words && !hit; ++wd) {
-                unsigned long long m = emask[le][wd];
+                unsigned long long m = emask[le * n_words + wd];
                 while (m && !hit) {
                     const int b = wd * 64 + __builtin_ctzll(m);
                     m &= m - 1;
-                    const float lo[3] = {o.lo[0][b], o.lo[1][b], o.lo[2][b]};
-                    const float hi[3] = {o.hi[0][b], o.hi[1][b], o.hi[2][b]};
+                    const float lo[3] = {olo[b], olo[obs_cap + b], olo[2 * obs_cap + b]};
+                    const float hi[3] = {ohi[b], ohi[obs_cap + b], ohi[2 * obs_cap + b]};
                     hit = segment_hits_box(v, w, bmin, bmax, lo, hi);
                 }
             }
-            for (int b = n_masked; b < n_obs && !hit; ++b) {
-                float lo[3], hi[3];
-                box_bounds(o, obstacles, b, lo, hi);
-                hit = segment_hits_box(v, w, bmin, bmax, lo, hi);
+            // boxes past the staged ones: a uniform loop, so that their
+            // bounds come through scalar loads and no vector load is left
+            // pending across the waypoint stores
+            for (int b = n_masked; b < n_obs; ++b) {
+                const float *ob = obstacles + __builtin_amdgcn_readfirstlane(b) * 12;
+                const float lo[3] = {ob[0], ob[1], ob[2]};
+                const float hi[3] = {ob[6], ob[7], ob[8]};
+                hit = hit || segment_hits_box(v, w, bmin, bmax, lo, hi);
             }
             if (hit)
                 ok[le] = 0; // only zeros are written: concurrent writes agree
@@ -220,27 +268,32 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0)
     }
 }
 
-template <bool kCheck, bool kWrite>
+template <bool kCheck, bool kWrite, int EPB>
 __global__ __launch_bounds__(kTile) void k_edges(
     const float *__restrict__ samples, const int *__restrict__ edge_src,
     const int *__restrict__ edge_dst, const float *__restrict__ topts_edge, int64_t n_edges,
     const int64_t *__restrict__ wp_offsets, int w_fixed, const float *__restrict__ obstacles,
     int n_obs, float *__restrict__ disc, uint8_t *__restrict__ is_free)
 {
+    extern __shared__ __align__(16) char smem[];
     const EdgeJob J{samples, edge_src, edge_dst, topts_edge, n_edges, wp_offsets,
                     w_fixed, obstacles, n_obs, disc, is_free};
-    edge_block<kCheck, kWrite, kEdgesPerBlock>(J, (int64_t)blockIdx.x * kEdgesPerBlock);
+    const int cap = kCheck ? edge_obs_cap(n_obs) : 0;
+    edge_block<kCheck, kWrite, EPB>(J, (int64_t)blockIdx.x * EPB, smem, cap, edge_words(cap));
 }
 
-// the fused edge pass of many instances in one launch
+// the fused edge pass of many instances in one launch; LDS holds obs_cap
+// boxes (the largest count of the instances, at most kEdgeObs)
 template <int EPB>
 __global__ __launch_bounds__(kTile) void k_edges_many(const EdgeJob *__restrict__ jobs,
                                                       const int64_t *__restrict__ prefix,
-                                                      int n_inst)
+                                                      int n_inst, int obs_cap)
 {
+    extern __shared__ __align__(16) char smem[];
     const int k = instance_of(prefix, n_inst, blockIdx.x);
     const EdgeJob J = jobs[k];
-    edge_block<true, true, EPB>(J, ((int64_t)blockIdx.x - prefix[k]) * EPB);
+    edge_block<true, true, EPB>(J, ((int64_t)blockIdx.x - prefix[k]) * EPB, smem, obs_cap,
+                                edge_words(obs_cap));
 }
 
 // freeEdges over precomputed discMotions: one lane per segment; the edge's
@@ -321,6 +374,72 @@ static int edge_blocks(int64_t n_edges, unsigned &blocks)
     return PUMP_OK;
 }
 
+// dynamic LDS above 64 KiB has to be allowed per kernel
+static int allow_lds(const void *fn, size_t bytes)
+{
+    if (bytes > 65536)
+        PUMP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return PUMP_OK;
+}
+
+// Edges per workgroup of the single-instance edge pass: 64, 128 (default) or
+// 256, from PUMP_EDGES_PER_BLOCK.
+static int edges_per_block()
+{
+    static int v = 0;
+    if (v == 0) {
+        const char *e = std::getenv("PUMP_EDGES_PER_BLOCK");
+        const int x = e ? std::atoi(e) : 128;
+        v = (x == 64 || x == 256) ? x : 128;
+    }
+    return v;
+}
+
+template <bool kCheck, bool kWrite, int EPB>
+static int launch_edges(const float *d_samples, const int *d_edge_src, const int *d_edge_dst,
+                        const float *d_topts_edge, int64_t n_edges, const int64_t *d_wp_offsets,
+                        int waypoints_count, const float *d_obstacles, int n_obs, float *d_disc,
+                        uint8_t *d_is_free, hipStream_t s)
+{
+    const int64_t nb = (n_edges + EPB - 1) / EPB;
+    if (nb > 0x7fffffff)
+        return PUMP_E_RANGE;
+    const int cap = kCheck ? edge_obs_cap(n_obs) : 0;
+    const size_t lds = (size_t)edge_lds(EPB, cap, edge_words(cap)).bytes;
+    const void *fn = reinterpret_cast<const void *>(&k_edges<kCheck, kWrite, EPB>);
+    const int r = allow_lds(fn, lds);
+    if (r != PUMP_OK)
+        return r;
+    hipLaunchKernelGGL((k_edges<kCheck, kWrite, EPB>), dim3((unsigned)nb), dim3(kTile), lds, s,
+                       d_samples, d_edge_src, d_edge_dst, d_topts_edge, n_edges, d_wp_offsets,
+                       waypoints_count, d_obstacles, n_obs, d_disc, d_is_free);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+
+template <bool kCheck, bool kWrite>
+static int launch_edges_any(const float *d_samples, const int *d_edge_src, const int *d_edge_dst,
+                            const float *d_topts_edge, int64_t n_edges,
+                            const int64_t *d_wp_offsets, int waypoints_count,
+                            const float *d_obstacles, int n_obs, float *d_disc,
+                            uint8_t *d_is_free, hipStream_t s)
+{
+    switch (edges_per_block()) {
+    case 64:
+        return launch_edges<kCheck, kWrite, 64>(d_samples, d_edge_src, d_edge_dst, d_topts_edge,
+                                                n_edges, d_wp_offsets, waypoints_count,
+                                                d_obstacles, n_obs, d_disc, d_is_free, s);
+    case 256:
+        return launch_edges<kCheck, kWrite, 256>(d_samples, d_edge_src, d_edge_dst, d_topts_edge,
+                                                 n_edges, d_wp_offsets, waypoints_count,
+                                                 d_obstacles, n_obs, d_disc, d_is_free, s);
+    default:
+        return launch_edges<kCheck, kWrite, 128>(d_samples, d_edge_src, d_edge_dst, d_topts_edge,
+                                                 n_edges, d_wp_offsets, waypoints_count,
+                                                 d_obstacles, n_obs, d_disc, d_is_free, s);
+    }
+}
+
 extern "C" int pump_discretize_edges(const float *d_samples, const int *d_edge_src,
                                      const int *d_edge_dst, const float *d_topts_edge,
                                      int64_t n_edges, const int64_t *d_wp_offsets,
@@ -333,15 +452,9 @@ extern "C" int pump_discretize_edges(const float *d_samples, const int *d_edge_s
     PUMP_REQUIRE(d_samples && d_edge_src && d_edge_dst && d_topts_edge && d_disc_motions);
     PUMP_REQUIRE(d_wp_offsets || waypoints_count >= 2);
     PUMP_REQUIRE(waypoints_count < kMaxWaypointsPerEdge);
-    unsigned blocks;
-    int r = edge_blocks(n_edges, blocks);
-    if (r)
-        return r;
-    hipLaunchKernelGGL((k_edges<false, true>), dim3(blocks), dim3(kTile), 0, as_stream(stream),
-                       d_samples, d_edge_src, d_edge_dst, d_topts_edge, n_edges, d_wp_offsets,
-                       waypoints_count, nullptr, 0, d_disc_motions, nullptr);
-    PUMP_LAUNCHED();
-    return PUMP_OK;
+    return launch_edges_any<false, true>(d_samples, d_edge_src, d_edge_dst, d_topts_edge, n_edges,
+                                         d_wp_offsets, waypoints_count, nullptr, 0,
+                                         d_disc_motions, nullptr, as_stream(stream));
 }
 
 extern "C" int pump_edges_discretize_check(const float *d_samples, const int *d_edge_src,
@@ -358,21 +471,14 @@ extern "C" int pump_edges_discretize_check(const float *d_samples, const int *d_
     PUMP_REQUIRE(n_obs == 0 || d_obstacles);
     PUMP_REQUIRE(d_wp_offsets || waypoints_count >= 2);
     PUMP_REQUIRE(waypoints_count < kMaxWaypointsPerEdge);
-    unsigned blocks;
-    int r = edge_blocks(n_edges, blocks);
-    if (r)
-        return r;
     if (d_disc_motions)
-        hipLaunchKernelGGL((k_edges<true, true>), dim3(blocks), dim3(kTile), 0, as_stream(stream),
-                           d_samples, d_edge_src, d_edge_dst, d_topts_edge, n_edges, d_wp_offsets,
-                           waypoints_count, d_obstacles, n_obs, d_disc_motions, d_is_free_edges);
-    else
-        hipLaunchKernelGGL((k_edges<true, false>), dim3(blocks), dim3(kTile), 0,
-                           as_stream(stream), d_samples, d_edge_src, d_edge_dst, d_topts_edge,
-                           n_edges, d_wp_offsets, waypoints_count, d_obstacles, n_obs, nullptr,
-                           d_is_free_edges);
-    PUMP_LAUNCHED();
-    return PUMP_OK;
+        return launch_edges_any<true, true>(d_samples, d_edge_src, d_edge_dst, d_topts_edge,
+                                            n_edges, d_wp_offsets, waypoints_count, d_obstacles,
+                                            n_obs, d_disc_motions, d_is_free_edges,
+                                            as_stream(stream));
+    return launch_edges_any<true, false>(d_samples, d_edge_src, d_edge_dst, d_topts_edge, n_edges,
+                                         d_wp_offsets, waypoints_count, d_obstacles, n_obs,
+                                         nullptr, d_is_free_edges, as_stream(stream));
 }
 
 extern "C" int pump_free_edges(const float *d_obstacles, int n_obs, const float *d_samples,
@@ -412,24 +518,35 @@ extern "C" int pump_sample_free(const float *d_obstacles, int n_obs, const float
 }
 
 int pump::edges_many(const EdgeJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
-                     int64_t n_blocks, int edges_per_block, hipStream_t s)
+                     int64_t n_blocks, int edges_per_block, int max_obs, hipStream_t s)
 {
     PUMP_REQUIRE(d_jobs && d_block_prefix && n_inst > 0 && n_blocks >= 0 &&
-                 n_blocks <= 0x7fffffff);
+                 n_blocks <= 0x7fffffff && max_obs >= 0);
     if (n_blocks == 0)
         return PUMP_OK;
+    const int cap = edge_obs_cap(max_obs);
+    const size_t lds = (size_t)edge_lds(edges_per_block, cap, edge_words(cap)).bytes;
+    const void *fn;
     if (edges_per_block == 64)
-        hipLaunchKernelGGL(k_edges_many<64>, dim3((unsigned)n_blocks), dim3(kTile), 0, s, d_jobs,
-                           d_block_prefix, n_inst);
+        fn = reinterpret_cast<const void *>(&k_edges_many<64>);
     else if (edges_per_block == 128)
-        hipLaunchKernelGGL(k_edges_many<128>, dim3((unsigned)n_blocks), dim3(kTile), 0, s, d_jobs,
-                           d_block_prefix, n_inst);
+        fn = reinterpret_cast<const void *>(&k_edges_many<128>);
     else if (edges_per_block == 256)
-        hipLaunchKernelGGL(k_edges_many<256>, dim3((unsigned)n_blocks), dim3(kTile), 0, s, d_jobs,
-                           d_block_prefix, n_inst);
+        fn = reinterpret_cast<const void *>(&k_edges_many<256>);
     else
         return PUMP_E_INVALID;
+    const int r = allow_lds(fn, lds);
+    if (r != PUMP_OK)
+        return r;
+    if (edges_per_block == 64)
+        hipLaunchKernelGGL(k_edges_many<64>, dim3((unsigned)n_blocks), dim3(kTile), lds, s, d_jobs,
+                           d_block_prefix, n_inst, cap);
+    else if (edges_per_block == 128)
+        hipLaunchKernelGGL(k_edges_many<128>, dim3((unsigned)n_blocks), dim3(kTile), lds, s,
+                           d_jobs, d_block_prefix, n_inst, cap);
+    else
+        hipLaunchKernelGGL(k_edges_many<256>, dim3((unsigned)n_blocks), dim3(kTile), lds, s,
+                           d_jobs, d_block_prefix, n_inst, cap);
     PUMP_LAUNCHED();
     return PUMP_OK;
 }
-

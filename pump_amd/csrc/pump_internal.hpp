@@ -83,8 +83,9 @@ struct HsJob {
 int fill_pairs_many(const PairJob *d_jobs, const int64_t *d_prefix, int n_inst, int64_t n_pairs,
                     int blocks_per_cu, unsigned long long *d_counter, hipStream_t s);
 // edges per workgroup of the batched edge pass: 64, 128 or 256
+// (max_obs: the largest box count of the instances)
 int edges_many(const EdgeJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
-               int64_t n_blocks, int edges_per_block, hipStream_t s);
+               int64_t n_blocks, int edges_per_block, int max_obs, hipStream_t s);
 constexpr int kHsEdgesPerBlock = 32;
 int cache_hs_many(const HsJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
                   int64_t n_blocks, hipStream_t s);
