@@ -175,9 +175,11 @@ int edge_pass(pump_batch *b, int autotune)
     TRY(upload(&b->hs_jobs, hjobs, s));
     TRY(upload(&b->edge_prefix, eprefix, s));
     TRY(upload(&b->hs_prefix, hprefix, s));
-    int max_obs = 0;
-    for (const pump_roadmap *rm : b->rms)
+    int max_obs = 0, max_hs = 0;
+    for (const pump_roadmap *rm : b->rms) {
         max_obs = std::max(max_obs, rm->n_obs);
+        max_hs = std::max(max_hs, rm->max_hs);
+    }
     auto blocks = [&](int c) { return eprefix[(size_t)c * (n + 1) + n]; };
     auto prefix_of = [&](int c) { return b->edge_prefix + (size_t)c * (n + 1); };
     int pick = 1;
@@ -203,7 +205,7 @@ int edge_pass(pump_batch *b, int autotune)
         int r = edges_many(b->edge_jobs, prefix_of(pick), n, blocks(pick), kEpb[pick], max_obs,
                            s);
         if (r == PUMP_OK)
-            r = cache_hs_many(b->hs_jobs, b->hs_prefix, n, hs_blocks, s);
+            r = cache_hs_many(b->hs_jobs, b->hs_prefix, n, hs_blocks, max_obs, max_hs, s);
         return r;
     }));
     return launch_timed(b, 1, &b->info.ms_edges);

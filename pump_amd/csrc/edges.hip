@@ -24,13 +24,6 @@ namespace {
 constexpr int kEdgesPerBlock = 128;
 constexpr int kTile = 256;
 
-// DPP lane move: lane i of the result reads src from the lane that kCtrl
-// names (wave_shl:1 = 0x130: from lane i + 1); lanes without a source keep 0.
-template <int kCtrl> __device__ __forceinline__ float dpp(float x)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xf, 0xf, false));
-}
-
 constexpr int kEdgeObs = 256;                  // boxes staged in LDS by the edge pass
 constexpr int kEdgeMaskWords = kEdgeObs / 64;  // candidate-mask words per edge
 constexpr int kMapCap = 3072;                  // block waypoints mapped to their edge in LDS
@@ -231,7 +224,7 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
         // segment (k, k+1): far end from the next lane
         float w[3];
         for (int d = 0; d < 3; ++d)
-            w[d] = dpp<0x130>(x[d]);
+            w[d] = dpp_move<0x130>(x[d]); // from lane + 1
         const bool test = have && lane < 63 && k < W - 1 && ok[le];
         if (test) {
             const float v[3] = {x[0], x[1], x[2]};

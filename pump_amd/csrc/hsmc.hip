@@ -29,7 +29,7 @@ namespace {
 
 using namespace wave;
 
-constexpr int kEdgesPerBlock = 32;
+constexpr int kEdgesPerBlock = 128;
 constexpr int kHsBlock = 256;
 constexpr int kListCap = 1024; // (waypoint, half-space) entries staged per pass
 
@@ -46,8 +46,46 @@ __global__ __launch_bounds__(256) void k_setup_as_bs(int64_t n, float *__restric
     }
 }
 
+constexpr int kHsObsLds = 512; // boxes staged in LDS by the half-space cache
+constexpr int kStageHs = 8;    // largest max_hs whose slots are staged in LDS
+constexpr int kHsMapCap = 3072; // block waypoints mapped to their edge in LDS
+static_assert(kEdgesPerBlock <= 256, "edge index of the waypoint map is a byte");
+
+// LDS of a half-space-cache workgroup, sized at launch: obs_cap staged boxes
+// and, when stage_hs > 0, each wave's 64 x stage_hs slots of as/bs, written
+// out as one contiguous run per wave (slots and sentinels together).
+struct HsLds {
+    int off, em, fr, obs, stage, wmap, bytes;
+};
+
+__host__ __device__ inline HsLds hs_lds(int obs_cap, int stage_hs)
+{
+    HsLds L{};
+    int o = 0;
+    auto take = [&o](int bytes, int align) {
+        o = (o + align - 1) / align * align;
+        const int at = o;
+        o += bytes;
+        return at;
+    };
+    L.off = take((kEdgesPerBlock + 1) * 8, 8);
+    L.em = take(kEdgesPerBlock * 8, 8);
+    L.fr = take(kEdgesPerBlock, 4);
+    L.obs = take(obs_cap * 6 * 4, 4);
+    L.stage = take((kHsBlock / 64) * 64 * stage_hs * 4 * 4, 16);
+    L.wmap = take(kHsMapCap, 4);
+    L.bytes = (o + 15) / 16 * 16;
+    return L;
+}
+
+__host__ __device__ inline int hs_obs_cap(int n_obs) { return n_obs < kHsObsLds ? n_obs : kHsObsLds; }
+
+// cacheHalfspaces (hsmc.cu:10-99) for the waypoints of kEdgesPerBlock
+// consecutive edges, one lane per waypoint; with kSentinels the unfilled
+// slots get setupAsBs's values (hsmc.cu:201-211) in the same pass.
 template <bool kSentinels>
-__device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0)
+__device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0, char *smem, int obs_cap,
+                                         int stage_hs)
 {
     const int64_t n_edges = J.n_edges;
     const float *__restrict__ disc = J.disc;
@@ -61,13 +99,38 @@ __device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0)
     int *__restrict__ count_stored = J.count;
     const float maxd2 = J.maxd2;
     uint64_t *__restrict__ edge_mask = J.edge_mask;
-    __shared__ ObsLds o;
-    __shared__ int64_t off[kEdgesPerBlock + 1];
-    __shared__ uint8_t fr[kEdgesPerBlock];
-    __shared__ unsigned long long em[kEdgesPerBlock];
+    const HsLds L = hs_lds(obs_cap, stage_hs);
+    int64_t *off = reinterpret_cast<int64_t *>(smem + L.off);
+    unsigned long long *em = reinterpret_cast<unsigned long long *>(smem + L.em);
+    uint8_t *fr = reinterpret_cast<uint8_t *>(smem + L.fr);
+    float *olo = reinterpret_cast<float *>(smem + L.obs); // lo[d][b], then hi[d][b]
+    float *ohi = olo + 3 * obs_cap;
+    uint8_t *wmap = reinterpret_cast<uint8_t *>(smem + L.wmap); // edge of each local waypoint
     const int ne = (int)((n_edges - e0) < kEdgesPerBlock ? (n_edges - e0) : kEdgesPerBlock);
-    const int t = threadIdx.x;
-    stage_obstacles(o, obstacles, n_obs);
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int n_staged = n_obs < obs_cap ? n_obs : obs_cap;
+    // slots staged in LDS and written out per wave (sentinel pass only)
+    const bool stage = kSentinels && max_hs > 0 && max_hs <= stage_hs;
+    float *sa = reinterpret_cast<float *>(smem + L.stage) + wid * 64 * stage_hs * 4;
+    float *sb = sa + 64 * max_hs * 3;
+    for (int q = t; q < n_staged * 3; q += kHsBlock) {
+        const int b = q / 3, d = q - 3 * (q / 3);
+        olo[d * obs_cap + b] = obstacles[b * 12 + d];
+        ohi[d * obs_cap + b] = obstacles[b * 12 + 6 + d];
+    }
+    auto bounds = [&](int b, float *lo, float *hi) {
+        if (b < n_staged) {
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = olo[d * obs_cap + b];
+                hi[d] = ohi[d * obs_cap + b];
+            }
+        } else {
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = obstacles[b * 12 + d];
+                hi[d] = obstacles[b * 12 + 6 + d];
+            }
+        }
+    };
     for (int q = t; q <= ne; q += kHsBlock)
         off[q] = wp_offsets ? wp_offsets[e0 + q] : (e0 + q) * (int64_t)w_fixed;
     for (int q = t; q < ne; q += kHsBlock) {
@@ -76,62 +139,123 @@ __device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0)
     }
     __syncthreads();
     const int64_t g_begin = off[0], g_end = off[ne];
-    for (int64_t g = g_begin + t; g < g_end; g += kHsBlock) {
-        const int le = find_edge(off, ne, g);
-        int64_t hs = g * max_hs;
-        int stored = 0;
-        bool live = false; // some stored b passes the sentinel test of PUMP.cu:642
-        if (fr[le]) {
-            float pos[3], vel[3];
+    const bool mapped = g_end - g_begin <= kHsMapCap;
+    if (mapped) {
+        for (int q = t; q < ne; q += kHsBlock)
+            for (int64_t j = off[q]; j < off[q + 1]; ++j)
+                wmap[j - g_begin] = (uint8_t)q;
+        __syncthreads();
+    }
+    const int n_words = (n_obs + 63) >> 6;
+    // A box whose distance from the tile's waypoint bounds exceeds sqrt(maxd2)
+    // along one axis has d2 > maxd2 (hsmc.cu:69) at every waypoint of the
+    // tile, so only the boxes near those bounds are scanned, in index order.
+    // The reach exceeds sqrt(maxd2) by far more than the rounding of the
+    // clamp distances; a NaN or infinite maxd2 scans every box.
+    const float reach =
+        (maxd2 >= 0.0f && maxd2 < INFINITY) ? sqrtf(maxd2) * 1.001f + 1e-6f : INFINITY;
+    for (int64_t base = g_begin; base < g_end; base += kHsBlock) {
+        const int64_t g = base + t;
+        const bool have = g < g_end;
+        const int le = !have ? 0 : mapped ? (int)wmap[g - g_begin] : find_edge(off, ne, g);
+        const bool work = have && fr[le];
+        float pos[3] = {0, 0, 0}, vel[3] = {0, 0, 0};
+        if (work)
             for (int d = 0; d < 3; ++d) {
                 pos[d] = disc[g * kDim + d];
                 vel[d] = disc[g * kDim + 3 + d];
             }
-            for (int b = 0; b < n_obs && stored < max_hs; ++b) {
-                float lo[3], hi[3], x[3];
-                box_bounds(o, obstacles, b, lo, hi);
-                for (int d = 0; d < 3; ++d)
-                    x[d] = (pos[d] <= lo[d]) ? lo[d] : (pos[d] >= hi[d]) ? hi[d] : pos[d];
-                for (int d = 0; d < 3; ++d)
-                    x[d] = x[d] - pos[d];
-                float d2 = 0.0f;
-                for (int d = 0; d < 3; ++d)
-                    d2 = d2 + x[d] * x[d];
-                if (d2 > maxd2)
-                    continue;
-                float xdv = 0.0f, vdv = 0.0f;
-                for (int d = 0; d < 3; ++d) {
-                    xdv = xdv + x[d] * vel[d];
-                    vdv = vdv + vel[d] * vel[d];
+        // slot h of this waypoint: as/bs[slot0 + h], in LDS when staged
+        const int64_t slot0 = stage ? (int64_t)lane * max_hs : g * max_hs;
+        float *const wa = stage ? sa : as;
+        float *const wb = stage ? sb : bs;
+        int stored = 0;
+        bool live = false; // some stored b passes the sentinel test of PUMP.cu:642
+        if (__ballot(work) != 0ull) {
+            float tlo[3], thi[3];
+            for (int d = 0; d < 3; ++d) {
+                tlo[d] = wave_min(work ? pos[d] : INFINITY) - reach;
+                thi[d] = wave_max(work ? pos[d] : -INFINITY) + reach;
+            }
+            for (int wd = 0; wd < n_words; ++wd) {
+                const int bb = wd * 64 + lane;
+                bool near = bb < n_obs;
+                if (near) {
+                    float lo[3], hi[3];
+                    bounds(bb, lo, hi);
+                    for (int d = 0; d < 3; ++d)
+                        near = near && !(hi[d] < tlo[d] || thi[d] < lo[d]);
                 }
-                if (vdv == 0.0f)
-                    vdv = 1e-5f;
-                float ratio = xdv / vdv;
-                for (int d = 0; d < 3; ++d)
-                    x[d] = x[d] - ratio * vel[d];
-                float xx = 0.0f;
-                for (int d = 0; d < 3; ++d) {
-                    as[hs * 3 + d] = x[d];
-                    xx = xx + x[d] * x[d];
+                unsigned long long m = __ballot(near);
+                while (work && m && stored < max_hs) {
+                    const int b = wd * 64 + __builtin_ctzll(m);
+                    m &= m - 1;
+                    float lo[3], hi[3], x[3];
+                    bounds(b, lo, hi);
+                    for (int d = 0; d < 3; ++d)
+                        x[d] = (pos[d] <= lo[d]) ? lo[d] : (pos[d] >= hi[d]) ? hi[d] : pos[d];
+                    for (int d = 0; d < 3; ++d)
+                        x[d] = x[d] - pos[d];
+                    float d2 = 0.0f;
+                    for (int d = 0; d < 3; ++d)
+                        d2 = d2 + x[d] * x[d];
+                    if (d2 > maxd2)
+                        continue;
+                    float xdv = 0.0f, vdv = 0.0f;
+                    for (int d = 0; d < 3; ++d) {
+                        xdv = xdv + x[d] * vel[d];
+                        vdv = vdv + vel[d] * vel[d];
+                    }
+                    if (vdv == 0.0f)
+                        vdv = 1e-5f;
+                    float ratio = xdv / vdv;
+                    for (int d = 0; d < 3; ++d)
+                        x[d] = x[d] - ratio * vel[d];
+                    float xx = 0.0f;
+                    const int64_t hs = slot0 + stored;
+                    for (int d = 0; d < 3; ++d) {
+                        wa[hs * 3 + d] = x[d];
+                        xx = xx + x[d] * x[d];
+                    }
+                    wb[hs] = xx;
+                    live = live || !(xx >= 1000.0f);
+                    ++stored;
                 }
-                bs[hs] = xx;
-                live = live || !(xx >= 1000.0f);
-                ++stored;
-                ++hs;
             }
         }
-        count_stored[g] = stored;
-        if (live && edge_mask) {
-            const int k = (int)(g - off[le]);
-            atomicOr(&em[le], 1ull << (k < 63 ? k : 63));
-        }
-        if (kSentinels) {
-            for (int h = stored; h < max_hs; ++h, ++hs) {
-                as[hs * 3 + 0] = -1.0f;
-                as[hs * 3 + 1] = -1.0f;
-                as[hs * 3 + 2] = -1.0f;
-                bs[hs] = 1111.0f;
+        if (have) {
+            count_stored[g] = stored;
+            if (live && edge_mask) {
+                const int k = (int)(g - off[le]);
+                atomicOr(&em[le], 1ull << (k < 63 ? k : 63));
             }
+            if (kSentinels)
+                for (int h = stored; h < max_hs; ++h) {
+                    const int64_t hs = slot0 + h;
+                    wa[hs * 3 + 0] = -1.0f;
+                    wa[hs * 3 + 1] = -1.0f;
+                    wa[hs * 3 + 2] = -1.0f;
+                    wb[hs] = 1111.0f;
+                }
+        }
+        if (stage) {
+            // the wave's waypoints are the prefix of its lanes with have set;
+            // their slots are one contiguous run of as and of bs
+            const int64_t w0 = base + (int64_t)wid * 64;
+            const int64_t left = g_end - w0;
+            const int nw = left <= 0 ? 0 : left < 64 ? (int)left : 64;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            float *ga = as + w0 * max_hs * 3;
+            float *gb = bs + w0 * max_hs;
+            for (int i = lane; i < nw * max_hs * 3; i += 64)
+                ga[i] = sa[i];
+            for (int i = lane; i < nw * max_hs; i += 64)
+                gb[i] = sb[i];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
     if (edge_mask) {
@@ -146,21 +270,27 @@ __global__ __launch_bounds__(kHsBlock) void k_cache_hs(
     int64_t n_edges, const float *__restrict__ disc, const int64_t *__restrict__ wp_offsets,
     int w_fixed, const uint8_t *__restrict__ is_free, int n_obs, int max_hs,
     const float *__restrict__ obstacles, float *__restrict__ as, float *__restrict__ bs,
-    int *__restrict__ count_stored, float maxd2, uint64_t *__restrict__ edge_mask)
+    int *__restrict__ count_stored, float maxd2, uint64_t *__restrict__ edge_mask, int stage_hs)
 {
+    extern __shared__ __align__(16) char smem[];
     const HsJob J{n_edges, disc, wp_offsets, w_fixed, is_free, n_obs, max_hs,
                   obstacles, as, bs, count_stored, maxd2, edge_mask};
-    hs_block<kSentinels>(J, (int64_t)blockIdx.x * kEdgesPerBlock);
+    hs_block<kSentinels>(J, (int64_t)blockIdx.x * kEdgesPerBlock, smem, hs_obs_cap(n_obs),
+                         stage_hs);
 }
 
-// the half-space cache of many instances in one launch (with sentinel rows)
+// the half-space cache of many instances in one launch (with sentinel rows);
+// LDS holds obs_cap boxes and stage_hs slots per waypoint (the largest of the
+// instances, or 0)
 __global__ __launch_bounds__(kHsBlock) void k_cache_hs_many(const HsJob *__restrict__ jobs,
                                                             const int64_t *__restrict__ prefix,
-                                                            int n_inst)
+                                                            int n_inst, int obs_cap, int stage_hs)
 {
+    extern __shared__ __align__(16) char smem[];
     const int k = instance_of(prefix, n_inst, blockIdx.x);
     const HsJob J = jobs[k];
-    hs_block<true>(J, ((int64_t)blockIdx.x - prefix[k]) * kEdgesPerBlock);
+    hs_block<true>(J, ((int64_t)blockIdx.x - prefix[k]) * kEdgesPerBlock, smem, obs_cap,
+                   stage_hs);
 }
 static_assert(kEdgesPerBlock == kHsEdgesPerBlock, "edges per half-space cache block");
 
@@ -786,29 +916,34 @@ extern "C" int pump_cache_halfspaces(int64_t n_edges, const float *d_disc_motion
     int64_t blocks = (n_edges + kEdgesPerBlock - 1) / kEdgesPerBlock;
     if (blocks > 0x7fffffff)
         return PUMP_E_RANGE;
+    const int stage_hs = (write_sentinels && max_hs <= kStageHs) ? max_hs : 0;
+    const size_t lds = (size_t)hs_lds(hs_obs_cap(n_obs), stage_hs).bytes;
     if (write_sentinels)
-        hipLaunchKernelGGL(k_cache_hs<true>, dim3((unsigned)blocks), dim3(kHsBlock), 0,
+        hipLaunchKernelGGL(k_cache_hs<true>, dim3((unsigned)blocks), dim3(kHsBlock), lds,
                            as_stream(stream), n_edges, d_disc_motions, d_wp_offsets,
                            waypoints_count, d_is_free_edges, n_obs, max_hs, d_obstacles, d_as,
-                           d_bs, d_count_stored, maxd2, d_edge_hs_mask);
+                           d_bs, d_count_stored, maxd2, d_edge_hs_mask, stage_hs);
     else
-        hipLaunchKernelGGL(k_cache_hs<false>, dim3((unsigned)blocks), dim3(kHsBlock), 0,
+        hipLaunchKernelGGL(k_cache_hs<false>, dim3((unsigned)blocks), dim3(kHsBlock), lds,
                            as_stream(stream), n_edges, d_disc_motions, d_wp_offsets,
                            waypoints_count, d_is_free_edges, n_obs, max_hs, d_obstacles, d_as,
-                           d_bs, d_count_stored, maxd2, d_edge_hs_mask);
+                           d_bs, d_count_stored, maxd2, d_edge_hs_mask, 0);
     PUMP_LAUNCHED();
     return PUMP_OK;
 }
 
 int pump::cache_hs_many(const HsJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
-                       int64_t n_blocks, hipStream_t s)
+                       int64_t n_blocks, int max_obs, int max_hs, hipStream_t s)
 {
     PUMP_REQUIRE(d_jobs && d_block_prefix && n_inst > 0 && n_blocks >= 0 &&
-                 n_blocks <= 0x7fffffff);
+                 n_blocks <= 0x7fffffff && max_obs >= 0 && max_hs >= 0);
     if (n_blocks == 0)
         return PUMP_OK;
-    hipLaunchKernelGGL(k_cache_hs_many, dim3((unsigned)n_blocks), dim3(kHsBlock), 0, s, d_jobs,
-                       d_block_prefix, n_inst);
+    const int cap = hs_obs_cap(max_obs);
+    const int stage_hs = max_hs <= kStageHs ? max_hs : 0;
+    const size_t lds = (size_t)hs_lds(cap, stage_hs).bytes;
+    hipLaunchKernelGGL(k_cache_hs_many, dim3((unsigned)n_blocks), dim3(kHsBlock), lds, s, d_jobs,
+                       d_block_prefix, n_inst, cap, stage_hs);
     PUMP_LAUNCHED();
     return PUMP_OK;
 }

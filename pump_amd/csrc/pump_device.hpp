@@ -427,5 +427,39 @@ __device__ __forceinline__ int instance_of(const int64_t *__restrict__ prefix, i
     return lo;
 }
 
+// DPP lane move: lane i of the result reads src from the lane that kCtrl
+// names (wave_shl:1 = 0x130: from lane i + 1; row_ror:n = 0x120 + n within
+// each row of 16); lanes without a source keep 0.
+template <int kCtrl> __device__ __forceinline__ float dpp_move(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ float read_lane(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// min / max over the 64 lanes of a wave (all lanes active): rotations
+// within each row of 16, then the four row results
+__device__ __forceinline__ float wave_min(float x)
+{
+    x = fminf(x, dpp_move<0x121>(x));
+    x = fminf(x, dpp_move<0x122>(x));
+    x = fminf(x, dpp_move<0x124>(x));
+    x = fminf(x, dpp_move<0x128>(x));
+    return fminf(fminf(read_lane(x, 0), read_lane(x, 16)), fminf(read_lane(x, 32), read_lane(x, 48)));
+}
+
+__device__ __forceinline__ float wave_max(float x)
+{
+    x = fmaxf(x, dpp_move<0x121>(x));
+    x = fmaxf(x, dpp_move<0x122>(x));
+    x = fmaxf(x, dpp_move<0x124>(x));
+    x = fmaxf(x, dpp_move<0x128>(x));
+    return fmaxf(fmaxf(read_lane(x, 0), read_lane(x, 16)), fmaxf(read_lane(x, 32), read_lane(x, 48)));
+}
+
 } // namespace pump
+
 

@@ -86,9 +86,10 @@ int fill_pairs_many(const PairJob *d_jobs, const int64_t *d_prefix, int n_inst, 
 // (max_obs: the largest box count of the instances)
 int edges_many(const EdgeJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
                int64_t n_blocks, int edges_per_block, int max_obs, hipStream_t s);
-constexpr int kHsEdgesPerBlock = 32;
+constexpr int kHsEdgesPerBlock = 128;
+// (max_obs, max_hs: the largest box count and slot count of the instances)
 int cache_hs_many(const HsJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
-                  int64_t n_blocks, hipStream_t s);
+                  int64_t n_blocks, int max_obs, int max_hs, hipStream_t s);
 
 // Grid size for a flat launch over n items, grid-stride beyond the cap.
 inline unsigned grid_for(int64_t n, int block, int64_t cap = 1 << 20)

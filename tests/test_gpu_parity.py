@@ -469,8 +469,9 @@ def test_free_edges_and_discretize_entry_points():
     cnt = dev(np.full(n_wp, -7, np.int32))
     emask = dev(np.full(E, 12345, np.uint64))
     P.check("setup_as_bs", P.lib.pump_setup_as_bs(hs, ptr(bs), ptr(as_), None))
+    gfree = dev(orc["is_free_edges"])  # held until the launch has run
     P.check("cache_hs", P.lib.pump_cache_halfspaces(
-        E, ptr(gdisc), ptr(dev(orc["is_free_edges"])), o.shape[0], max_hs, ptr(go), 0, ptr(goff),
+        E, ptr(gdisc), ptr(gfree), o.shape[0], max_hs, ptr(go), 0, ptr(goff),
         ptr(as_), ptr(bs), ptr(cnt), orc["maxd2"], 0, ptr(emask), None))
     assert np.array_equal(host(cnt), orc["count_stored"])
     assert np.array_equal(host(bs), orc["bs"][:hs])
@@ -479,6 +480,52 @@ def test_free_edges_and_discretize_entry_points():
     assert ref_mask.any()
     assert np.array_equal(host(emask), ref_mask)
     rm.close()
+
+
+@pytest.mark.parametrize("maxd2", [0.01, 0.05, float("nan"), -1.0])
+def test_cache_halfspaces_many_boxes(maxd2):
+    """cacheHalfspaces (hsmc.cu:10-99) on 600 boxes (past the 512 staged in
+    LDS, ten candidate words), waypoints scattered over the unit cube in
+    edges of 2-70 waypoints, some edges blocked, against the oracle: a
+    neighbourhood holding many boxes (the first max_hs by index are kept), a
+    NaN radius (every box passes the d2 > maxd2 skip) and a negative one."""
+    rng = np.random.default_rng(23)
+    n_obs, max_hs = 600, 5
+    c = rng.uniform(0, 1, (n_obs, 3))
+    h = rng.uniform(0.005, 0.04, (n_obs, 3))
+    obs = np.zeros((n_obs, 12), np.float32)
+    obs[:, :3], obs[:, 6:9] = c - h, c + h
+    obs[:, 3:6], obs[:, 9:12] = -3.1, 3.1
+    W = rng.integers(2, 71, 300)
+    off = np.concatenate([[0], np.cumsum(W)]).astype(np.int64)
+    n_wp, E = int(off[-1]), W.shape[0]
+    disc = np.concatenate([rng.uniform(0, 1, (n_wp, 3)), rng.uniform(-0.5, 0.5, (n_wp, 3))],
+                          1).astype(np.float32)
+    disc[:40, 3:] = 0
+    free = (rng.uniform(size=E) < 0.8).astype(np.uint8)
+    md2 = np.float32(maxd2)
+    hs = n_wp * max_hs
+    ref_as, ref_bs = np.zeros((hs, 3), np.float32), np.zeros(hs, np.float32)
+    ref_cnt = np.zeros(n_wp, np.int32)
+    O.po_setup_as_bs(hs, oracle.fp(ref_bs), oracle.fp(ref_as))
+    O.po_cache_halfspaces(E, oracle.fp(disc), oracle.lp(off), oracle.bp(free), n_obs, max_hs,
+                          oracle.fp(obs), oracle.fp(ref_as), oracle.fp(ref_bs),
+                          oracle.ip(ref_cnt), md2, 4)
+    as_, bs = dev(np.zeros((hs, 3), np.float32)), dev(np.zeros(hs, np.float32))
+    cnt, emask = dev(np.full(n_wp, -7, np.int32)), dev(np.zeros(E, np.uint64))
+    gdisc, gfree, gobs, goff = dev(disc), dev(free), dev(obs), dev(off)
+    P.check("cache_hs", P.lib.pump_cache_halfspaces(
+        E, ptr(gdisc), ptr(gfree), n_obs, max_hs, ptr(gobs), 0, ptr(goff),
+        ptr(as_), ptr(bs), ptr(cnt), md2, 1, ptr(emask), None))
+    got_cnt = host(cnt)
+    assert np.array_equal(got_cnt, ref_cnt)
+    assert np.array_equal(host(bs).view(np.uint32), ref_bs.view(np.uint32))
+    assert np.array_equal(host(as_).view(np.uint32), ref_as.view(np.uint32))
+    assert np.array_equal(host(emask), edge_mask_host(off, ref_bs, max_hs))
+    if maxd2 == 0.05 or maxd2 != maxd2:
+        assert (got_cnt == max_hs).sum() > 100   # truncation by box index exercised
+    if maxd2 < 0:
+        assert not got_cnt.any()
 
 
 def test_setup_valid_particles():
