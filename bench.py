@@ -171,7 +171,7 @@ def main():
                    "n_p": int(prm.n_p), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": "k_edges<true,true> (pump_edges_discretize_check)",
+                     "kernel": "k_edges<true,true,128> (pump_edges_discretize_check)",
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "hsmc": hsmc,
