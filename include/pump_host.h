@@ -104,6 +104,7 @@ typedef struct pump_params {
     int polish;          /* also compute FP64-polished arrival times (chosen form) */
     int path_cap;        /* rows of a candidate path for full MC */
     int mccp_batch;      /* goal candidates evaluated per full-MC launch */
+    int search_device;   /* 1: search bookkeeping on the device, 0: on the host */
 } pump_params;
 
 void pump_params_defaults(pump_params *p);

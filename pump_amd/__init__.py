@@ -72,7 +72,7 @@ class Params(ctypes.Structure):
         ("eps_cp", c_float), ("cp_target", c_float), ("cp_factor", c_float), ("max_itrs", c_int),
         ("max_path_count", c_int64), ("validity_bits", c_int), ("precision", c_int),
         ("fused_edges", c_int), ("keep_pairs", c_int), ("polish", c_int), ("path_cap", c_int),
-        ("mccp_batch", c_int),
+        ("mccp_batch", c_int), ("search_device", c_int),
     ]
 
 

@@ -31,6 +31,7 @@ SOURCES = [
     "mccp.hip",
     "host_gen.hip",
     "host_pipeline.hip",
+    "search_device.hip",
     "runtime.hip",
 ]
 

@@ -293,6 +293,7 @@ extern "C" void pump_params_defaults(pump_params *p)
     p->polish = 0;
     p->path_cap = 4096;
     p->mccp_batch = 16;
+    p->search_device = 1;
 }
 
 extern "C" int pump_roadmap_build(const float *samples_h, int num, const float *obstacles_h,
@@ -573,40 +574,93 @@ extern "C" int pump_find_optimal_path(float dt, const float *xs, int n_wp, float
 // ---------------------------------------------------------------------------
 // The PUMP search consumer (PUMP.cu:3-405)
 // ---------------------------------------------------------------------------
+// Goal-path selection (PUMP.cu:354-405): the goal paths sorted by (cost,
+// index), then the first whose full-MC CP is below the target.  status is 0
+// on success, 1 when no goal path qualifies, negative on a capacity error.
+int pump::rmb::select_goal_path(pump_roadmap *rm, const pump_params &P,
+                                const std::vector<int> &goal_list, const PathTable &T,
+                                pump_search_result *res, int &status)
+{
+    hipStream_t s = rm->stream;
+    // goal-path selection (PUMP.cu:354-405): cheapest first, first whose
+    // full-MC CP is below the target
+    std::vector<std::pair<float, int>> goal_paths;
+    for (int pidx : goal_list)
+        goal_paths.emplace_back(T.cost[pidx], pidx);
+    std::sort(goal_paths.begin(), goal_paths.end());
+    res->num_goal_paths = (int)goal_paths.size();
+    status = 1;
+    const int cap = P.path_cap;
+    const int batch = P.mccp_batch > 0 ? P.mccp_batch : 1;
+    DevBuf d_paths, d_T, d_hits;
+    std::vector<float> hpaths, xs;
+    std::vector<int> hT, hits;
+    std::vector<std::vector<int>> nodes_of;
+    for (size_t k0 = 0; k0 < goal_paths.size() && status == 1; k0 += batch) {
+        const int nk = (int)std::min<size_t>(batch, goal_paths.size() - k0);
+        hpaths.assign((size_t)nk * cap * 6, 0.0f);
+        hT.assign(nk, 0);
+        hits.assign(nk, 0);
+        nodes_of.assign(nk, {});
+        for (int q = 0; q < nk; ++q) {
+            int pidx = goal_paths[k0 + q].second;
+            std::vector<int> &nodes = nodes_of[q];
+            while (pidx != -2) {
+                nodes.push_back(T.node[pidx]);
+                pidx = T.prev[pidx];
+            }
+            std::reverse(nodes.begin(), nodes.end());
+            xs.resize(nodes.size() * 6);
+            for (size_t r = 0; r < nodes.size(); ++r)
+                std::memcpy(&xs[r * 6], &rm->samples_h[(size_t)nodes[r] * 6],
+                            sizeof(float) * 6);
+            float tt = 0;
+            int rows = pump_find_optimal_path(P.dt, xs.data(), (int)nodes.size(),
+                                              &hpaths[(size_t)q * cap * 6], cap, &tt);
+            if (rows < 0 || rows > rm->tmax_steps + 1) {
+                status = -5; // candidate path longer than the path / noise buffers
+                break;
+            }
+            hT[q] = rows;
+        }
+        if (status != 1)
+            break;
+        TRY(d_paths.ensure(sizeof(float) * hpaths.size()));
+        TRY(d_T.ensure(sizeof(int) * nk));
+        TRY(d_hits.ensure(sizeof(int) * nk));
+        TRYH(hipMemcpyAsync(d_paths.p, hpaths.data(), sizeof(float) * hpaths.size(),
+                            hipMemcpyHostToDevice, s));
+        TRYH(hipMemcpyAsync(d_T.p, hT.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
+        TRY(pump_collision_probability_many(rm->obstacles, rm->n_obs, rm->xcomb,
+                                            rm->tmax_steps, rm->n_mc, P.offset_mult,
+                                            (const float *)d_paths.p, cap, (const int *)d_T.p,
+                                            nk, (int *)d_hits.p, 1, s));
+        TRYH(hipMemcpyAsync(hits.data(), d_hits.p, sizeof(int) * nk, hipMemcpyDeviceToHost,
+                            s));
+        TRYH(hipStreamSynchronize(s));
+        for (int q = 0; q < nk; ++q) {
+            const int pidx = goal_paths[k0 + q].second;
+            const float cp = ((float)hits[q]) / ((float)rm->n_mc);
+            res->mccp_calls++;
+            res->soln_path_idx = pidx;
+            res->soln_cp = cp;
+            res->soln_cost = T.cost[pidx];
+            res->soln_time = T.time[pidx];
+            res->soln_hsmc_cp = T.cp[pidx];
+            res->soln_len = (int)std::min<size_t>(nodes_of[q].size(), 256);
+            for (int r = 0; r < res->soln_len; ++r)
+                res->soln_nodes[r] = nodes_of[q][r];
+            if (cp < P.cp_target) {
+                status = 0;
+                break;
+            }
+        }
+    }
+    return PUMP_OK;
+}
+
 namespace {
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    int ensure(size_t b)
-    {
-        if (b <= bytes)
-            return PUMP_OK;
-        if (p)
-            hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        size_t want = b + b / 2;
-        if (hipMalloc(&p, want) != hipSuccess) {
-            // the failed attempt leaves its error as the thread's last error:
-            // clear it, so that a later launch check does not report it
-            (void)hipGetLastError();
-            if (hipMalloc(&p, b) != hipSuccess) {
-                (void)hipGetLastError();
-                p = nullptr;
-                return PUMP_E_NOMEM;
-            }
-            want = b;
-        }
-        bytes = want;
-        return PUMP_OK;
-    }
-    ~DevBuf()
-    {
-        if (p)
-            hipFree(p);
-    }
-};
 
 struct SearchState {
     std::vector<int64_t> row_start;
@@ -828,6 +882,10 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
             float dcost = (path_cost[st] + S.h[node] - cost_threshold);
             dcost = std::max(0.0f, dcost);
             int bucket = (((int)(dcost / dr)) + gidx + 1) % nb;
+            if (bucket < 0 || bucket >= nb) {
+                status = -6; // bucket index outside [0, nb): negative rn
+                break;
+            }
             G[(size_t)bucket * maxp + sizeG[bucket]] = (int)st;
             sizeG[bucket]++;
             path_prev[st] = wf_prev[i];
@@ -880,80 +938,9 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
     res->soln_path_idx = -1;
 
     if (status == 0) {
-        // goal-path selection (PUMP.cu:354-405): cheapest first, first whose
-        // full-MC CP is below the target
-        std::vector<std::pair<float, int>> goal_paths;
-        for (int pidx : Pn[goal_idx])
-            goal_paths.emplace_back(path_cost[pidx], pidx);
-        std::sort(goal_paths.begin(), goal_paths.end());
-        res->num_goal_paths = (int)goal_paths.size();
-        status = 1;
-        const int cap = P.path_cap;
-        const int batch = P.mccp_batch > 0 ? P.mccp_batch : 1;
-        DevBuf d_paths, d_T, d_hits;
-        std::vector<float> hpaths, xs;
-        std::vector<int> hT, hits;
-        std::vector<std::vector<int>> nodes_of;
-        for (size_t k0 = 0; k0 < goal_paths.size() && status == 1; k0 += batch) {
-            const int nk = (int)std::min<size_t>(batch, goal_paths.size() - k0);
-            hpaths.assign((size_t)nk * cap * 6, 0.0f);
-            hT.assign(nk, 0);
-            hits.assign(nk, 0);
-            nodes_of.assign(nk, {});
-            for (int q = 0; q < nk; ++q) {
-                int pidx = goal_paths[k0 + q].second;
-                std::vector<int> &nodes = nodes_of[q];
-                while (pidx != -2) {
-                    nodes.push_back(path_node[pidx]);
-                    pidx = path_prev[pidx];
-                }
-                std::reverse(nodes.begin(), nodes.end());
-                xs.resize(nodes.size() * 6);
-                for (size_t r = 0; r < nodes.size(); ++r)
-                    std::memcpy(&xs[r * 6], &rm->samples_h[(size_t)nodes[r] * 6],
-                                sizeof(float) * 6);
-                float tt = 0;
-                int rows = pump_find_optimal_path(P.dt, xs.data(), (int)nodes.size(),
-                                                  &hpaths[(size_t)q * cap * 6], cap, &tt);
-                if (rows < 0 || rows > rm->tmax_steps + 1) {
-                    status = -5; // candidate path longer than the path / noise buffers
-                    break;
-                }
-                hT[q] = rows;
-            }
-            if (status != 1)
-                break;
-            TRY(d_paths.ensure(sizeof(float) * hpaths.size()));
-            TRY(d_T.ensure(sizeof(int) * nk));
-            TRY(d_hits.ensure(sizeof(int) * nk));
-            TRYH(hipMemcpyAsync(d_paths.p, hpaths.data(), sizeof(float) * hpaths.size(),
-                                hipMemcpyHostToDevice, s));
-            TRYH(hipMemcpyAsync(d_T.p, hT.data(), sizeof(int) * nk, hipMemcpyHostToDevice, s));
-            TRY(pump_collision_probability_many(rm->obstacles, rm->n_obs, rm->xcomb,
-                                                rm->tmax_steps, rm->n_mc, P.offset_mult,
-                                                (const float *)d_paths.p, cap, (const int *)d_T.p,
-                                                nk, (int *)d_hits.p, 1, s));
-            TRYH(hipMemcpyAsync(hits.data(), d_hits.p, sizeof(int) * nk, hipMemcpyDeviceToHost,
-                                s));
-            TRYH(hipStreamSynchronize(s));
-            for (int q = 0; q < nk; ++q) {
-                const int pidx = goal_paths[k0 + q].second;
-                const float cp = ((float)hits[q]) / ((float)rm->n_mc);
-                res->mccp_calls++;
-                res->soln_path_idx = pidx;
-                res->soln_cp = cp;
-                res->soln_cost = path_cost[pidx];
-                res->soln_time = path_time[pidx];
-                res->soln_hsmc_cp = path_cp[pidx];
-                res->soln_len = (int)std::min<size_t>(nodes_of[q].size(), 256);
-                for (int r = 0; r < res->soln_len; ++r)
-                    res->soln_nodes[r] = nodes_of[q][r];
-                if (cp < P.cp_target) {
-                    status = 0;
-                    break;
-                }
-            }
-        }
+        PathTable T{path_prev.data(), path_node.data(), path_cost.data(), path_time.data(),
+                    path_cp.data()};
+        TRY(select_goal_path(rm, P, Pn[goal_idx], T, res, status));
     }
     res->status = status;
     res->ms_total =
@@ -968,5 +955,7 @@ extern "C" int pump_search_run(pump_roadmap *rm, const pump_params *prm, pump_se
     if (!rm || !prm || !res)
         return PUMP_E_INVALID;
     std::memset(res, 0, sizeof(*res));
+    if (prm->search_device)
+        return pump::rmb::search_device(rm, *prm, res);
     return search(rm, *prm, res);
 }

@@ -81,6 +81,40 @@ inline int ensure_ws(pump_roadmap *rm, size_t bytes)
     return r;
 }
 
+// A device buffer that grows (by half again) on demand.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t b)
+    {
+        if (b <= bytes)
+            return PUMP_OK;
+        if (p)
+            hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t want = b + b / 2;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            // the failed attempt leaves its error as the thread's last error:
+            // clear it, so that a later launch check does not report it
+            (void)hipGetLastError();
+            if (hipMalloc(&p, b) != hipSuccess) {
+                (void)hipGetLastError();
+                p = nullptr;
+                return PUMP_E_NOMEM;
+            }
+            want = b;
+        }
+        bytes = want;
+        return PUMP_OK;
+    }
+    ~DevBuf()
+    {
+        if (p)
+            hipFree(p);
+    }
+};
+
 struct EventTimer {
     hipEvent_t a{}, b{};
     hipStream_t s;
@@ -134,6 +168,17 @@ int rm_halfspaces(pump_roadmap *rm);
 int rm_finish(pump_roadmap *rm, std::chrono::steady_clock::time_point t_begin);
 // frees the roadmap and what it owns
 void rm_release(pump_roadmap *rm);
+
+// the search's path table: parent path, node, and the float sums of cost,
+// time and the HSMC CP, indexed by path
+struct PathTable {
+    const int *prev, *node;
+    const float *cost, *time, *cp;
+};
+int select_goal_path(pump_roadmap *rm, const pump_params &P, const std::vector<int> &goal_list,
+                     const PathTable &T, pump_search_result *res, int &status);
+// the search with its bookkeeping on the device (search_device.hip)
+int search_device(pump_roadmap *rm, const pump_params &P, pump_search_result *res);
 
 } // namespace rmb
 } // namespace pump

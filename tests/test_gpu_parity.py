@@ -125,10 +125,14 @@ def test_nn_tables_parity():
 
 @pytest.mark.parametrize("case", CASES[:2] + CASES[3:])
 @pytest.mark.parametrize("bits", [0, 1])
-def test_search_parity(case, bits):
+@pytest.mark.parametrize("on_device", [1, 0])
+def test_search_parity(case, bits, on_device):
+    """The search with its bookkeeping on the device (default) and on the
+    host, against the oracle's restatement of PUMP.cu:3-405."""
     cfg, s, o, ot, xc, rm, orc = build_pair(case, validity_bits=bits)
     prm = cfg["params"]
-    got = P.search_result_dict(rm.search())
+    prm.search_device = on_device
+    got = P.search_result_dict(rm.search(prm))
     ref = OP.search(orc, s, s[-1], o, ot, xc, prm, nthreads=8, path_cap=prm.path_cap)
     print("gpu", got, "\noracle", ref, flush=True)
     for k in ("status", "itrs", "num_paths", "num_considered", "num_goal_paths", "soln_path_idx",
