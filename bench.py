@@ -131,6 +131,9 @@ def main():
     if not args.skip_hsmc:
         hsmc = hsmc_pass(P, rm, d, prm, info, stream, args.hsmc_reps)
 
+    # ---------------- the PUMP search, bookkeeping on the device and host --
+    search = search_pass(P, rm, prm)
+
     # ---------------- kernel (a): all-pairs 2pBVP --------------------------
     pairs = {"pair_solves_per_s": info.n_pairs / (info.ms_pairs / 1e3),
              "ms": info.ms_pairs, "n_pairs": int(info.n_pairs)}
@@ -176,6 +179,7 @@ def main():
         "cpu_baseline": cpu,
         "hsmc": hsmc,
         "pairs": pairs,
+        "search": search,
         "batch": batch,
         "pipeline_ms": {k: getattr(info, k) for k in (
             "ms_pairs", "ms_select", "ms_edges_build", "ms_nn", "ms_sample_free",
@@ -305,6 +309,45 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
     out["value"] = out[best]["particle_steps_per_s"]
     out["unit"] = "particle-steps/s"
     return out
+
+
+def search_pass(P, rm, prm):
+    """The whole PUMP search (PUMP.cu:3-405) on the roadmap, with its
+    bookkeeping on the device and on the host; both must return the same
+    path."""
+    import ctypes
+    out = {}
+    for name, dev in (("device", 1), ("host", 0)):
+        q = P.Params()
+        ctypes.memmove(ctypes.byref(q), ctypes.byref(prm), ctypes.sizeof(P.Params))
+        q.validity_bits = 1
+        q.search_device = dev
+        r = P.search_result_dict(rm.search(q))
+        out[name] = r
+    # smoothing and controller discretisation of the device search's result
+    sm = None
+    q = P.Params()
+    ctypes.memmove(ctypes.byref(q), ctypes.byref(prm), ctypes.sizeof(P.Params))
+    q.validity_bits, q.search_device = 1, 1
+    r = P.SearchResult()
+    P.check("pump_search_run", P.lib.pump_search_run(rm._h, ctypes.byref(q), ctypes.byref(r)))
+    if r.status == 0:
+        so, path, _, disc = rm.smooth(r, q)
+        sm = {"status": so.status, "itrs": so.itrs, "mccp_launches": so.mccp_launches,
+              "ms": so.ms_total, "cost_before": so.cost_before, "cost": so.cost, "cp": so.cp,
+              "path_rows": so.path_rows, "discretized_rows": so.discretized_rows}
+        log(f"smoothing: {so.itrs} steps in {so.mccp_launches} full-MC launches, "
+            f"{so.ms_total:.1f} ms, cost {so.cost_before:.2f} -> {so.cost:.2f}, cp {so.cp:.4f}")
+    d, h = out["device"], out["host"]
+    same = all(d[k] == h[k] for k in ("status", "itrs", "num_paths", "num_goal_paths",
+                                       "soln_nodes", "soln_cost", "soln_cp", "particle_steps"))
+    log(f"search: {d['ms_total']:.1f} ms with device bookkeeping, {h['ms_total']:.1f} ms on the "
+        f"host, {d['itrs']} iterations, {d['num_paths']} paths, same path: {same}")
+    return {"ms_device_bookkeeping": d["ms_total"], "ms_host_bookkeeping": h["ms_total"],
+            "ms_propagate": d["ms_propagate"], "itrs": d["itrs"], "num_paths": d["num_paths"],
+            "num_considered": d["num_considered"], "status": d["status"],
+            "soln_cost": d["soln_cost"], "soln_cp": d["soln_cp"], "same_result": same,
+            "smoothing": sm}
 
 
 def batch_pass(P, n_inst, reps):

@@ -210,6 +210,31 @@ typedef struct pump_search_result {
 
 int pump_search_run(pump_roadmap *rm, const pump_params *prm, pump_search_result *res);
 
+/* Smoothing of the search's solution path and its controller discretisation
+ * (PUMP.cu:406-568).  The solution's node states are blended towards the
+ * obstacle-free optimal path's states, alpha bisected on the full-MC CP (at
+ * most 30 steps, kSmoothDepth = 4 steps evaluated per batched full-MC
+ * launch, with the sequential loop's alphas and decisions); the kept path
+ * is the one with CP below the target closest to it.  Outputs: the kept path
+ * rows at dt (path.txt; path_cap rows), the last iterate's node states
+ * (n_nodes x 6), and the controller discretisation of the last iterate at
+ * 0.05 s (discretizedPath.txt; disc_cap rows).  NULL outputs are skipped. */
+typedef struct pump_smooth_result {
+    int status;           /* 0 smoothed; 1 the search has no solution; <0 a path exceeds the buffers */
+    int itrs;             /* bisection steps (full-MC evaluations of the sequential loop) */
+    int mccp_launches;    /* batched full-MC launches that evaluated them */
+    int n_nodes;          /* states of the solution path */
+    int path_rows;        /* rows of the kept path */
+    int discretized_rows; /* rows of the controller discretisation */
+    float cost_before, cost, cp;
+    float ms_total;
+    float trace[32][3];   /* alpha, CP, cost of each step */
+} pump_smooth_result;
+
+int pump_smooth_run(pump_roadmap *rm, const pump_params *prm, const pump_search_result *sres,
+                    pump_smooth_result *out, float *path_host, int path_cap,
+                    float *xs_smooth_host, float *discretized_host, int disc_cap);
+
 /* findOptimalPath (2pBVP.cu:5-36) on the host with the library's arithmetic;
  * returns rows written or -1 if more than cap rows would be needed. */
 int pump_find_optimal_path(float dt, const float *xs, int n_wp, float *split, int cap,

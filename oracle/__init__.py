@@ -84,6 +84,9 @@ _SIGS = {
     "po_mccp_hits": (c_int, [f32p, c_int, f32p, c_int, c_int, c_float, f32p, c_int, i32p]),
     "po_collision_probability": (c_float, [f32p, c_int, f32p, c_int, c_int, c_float, f32p, c_int]),
     "po_search": (c_int, [ctypes.POINTER(SearchIn), ctypes.POINTER(SearchOut)]),
+    "po_smooth": (c_int, [f32p, c_int, c_float, c_float, f32p, c_int, f32p, c_int, c_int, c_float,
+                          c_float, c_float, c_int, f32p, i32p, f32p, f32p, c_int, f32p, i32p,
+                          f32p, i32p]),
     "po64_cost": (c_double, [c_double, f32p, f32p]),
     "po64_topt": (c_double, [f32p, f32p, c_double]),
     "po64_topt_near": (c_double, [f32p, f32p, c_double, c_double]),

@@ -169,3 +169,37 @@ def fp64_path_cp(nodes, samples, obstacles, xcomb, offset_mult, dt=0.05, cap=819
     hits = O.po64_mccp_hits(fp(obs), obstacles.shape[0], fp(xc), tsteps, n_mc, offset_mult,
                             dp(path), T)
     return hits / n_mc, hits, n_mc, T
+
+
+def smooth(search_res, samples, obstacles, xcomb, prm, path_cap=4096, disc_cap=1 << 16):
+    """Oracle smoothing and controller discretisation (PUMP.cu:406-568) of a
+    search result (a dict with status, soln_nodes, soln_cost, soln_cp)."""
+    O = load()
+    nodes = list(search_res["soln_nodes"])
+    if search_res["status"] != 0 or len(nodes) < 2:
+        return {"status": 1}
+    samples = np.ascontiguousarray(samples, np.float32)
+    xs = np.ascontiguousarray(samples[nodes], np.float32)
+    obstacles = np.ascontiguousarray(obstacles, np.float32).reshape(-1, 12)
+    obs = obstacles if obstacles.shape[0] else np.zeros((1, 12), np.float32)
+    xc = np.ascontiguousarray(xcomb, np.float32)
+    n_mc, tmax_steps = xc.shape[0], xc.shape[1]
+    best = np.zeros((path_cap, 6), np.float32)
+    rows = np.zeros(1, np.int32)
+    cc = np.zeros(2, np.float32)
+    xs_s = np.zeros_like(xs)
+    disc = np.zeros((disc_cap, 6), np.float32)
+    drows = np.zeros(1, np.int32)
+    trace = np.zeros((30, 3), np.float32)
+    itrs = np.zeros(1, np.int32)
+    rc = O.po_smooth(fp(xs), xs.shape[0], np.float32(search_res["soln_cost"]),
+                     np.float32(search_res["soln_cp"]), fp(obs), obstacles.shape[0], fp(xc),
+                     tmax_steps, n_mc, prm.offset_mult, prm.dt, prm.cp_target, path_cap, fp(best),
+                     ip(rows), fp(cc), fp(xs_s), disc_cap, fp(disc), ip(drows), fp(trace),
+                     ip(itrs))
+    if rc != 0:
+        return {"status": rc}
+    k = int(itrs[0])
+    return {"status": 0, "itrs": k, "trace": trace[:k].copy(), "path": best[: rows[0]].copy(),
+            "cost": float(cc[0]), "cp": float(cc[1]), "xs_smooth": xs_s,
+            "discretized": disc[: drows[0]].copy()}

@@ -1240,3 +1240,152 @@ int po64_find_optimal_path(double dt, double *split, const float *xs, int n_wp, 
     }
     return rows;
 }
+
+/* ======================================================================
+ * Smoothing and controller discretisation
+ * ==================================================================== */
+
+/* Smoothing -- PUMP.cu:406-540: the solution's node states xs are blended
+ * towards the states that the obstacle-free optimal path from xs[0] to
+ * xs[n-1] (findOptimalPath at dt/10) passes at the same time fractions,
+ * xsSmooth = (1-alpha)*xs + alpha*xsOpt, with alpha bisected on the full-MC
+ * CP (alpha = 1 first; CP < target raises the lower bound, else the upper
+ * one), at most 30 iterations, stopping once target - CP lies in (0, 2e-4).
+ * The kept path is the one whose CP is below the target and closest to it
+ * (lower cost on a tie), starting from the solution itself.
+ * Controller discretisation -- PUMP.cu:543-568: each edge of the last
+ * iterate xsSmooth (the reference discretises the last iterate, not the
+ * kept one) gives (int)(tau/0.05 + 1) rows of findDiscretizedPath, with tau
+ * from a bisection with tmax = 2; the row count is accumulated as an int
+ * with float additions (discretizedPathLength += tau/controlDt + 1).
+ * Outputs: the kept path (path.txt) and its cost and CP; the last iterate;
+ * the discretised rows (discretizedPath.txt); a trace of alpha, CP and cost
+ * per iteration.  Returns 0, or -1 when a buffer capacity is exceeded (the
+ * reference writes past its fixed arrays there).
+ * Pinned by: an obstacle-free problem (every CP is 0, alpha stays 1, and the
+ * kept cost is min(solution cost, the alpha = 1 cost)); the kept path equals
+ * findOptimalPath of the kept iterate and the costs equal the sum of the
+ * edge costs recomputed from the primitives; the discretised rows equal the
+ * per-edge findDiscretizedPath rows. */
+int po_smooth(const float *xs, int n, float soln_cost, float soln_cp, const float *obstacles,
+              int n_obs, const float *xcomb, int tmax_steps, int n_mc, float offset_mult, float dt,
+              float cp_target, int path_cap, float *best_path, int *best_rows, float *best_cost_cp,
+              float *xs_smooth, int disc_cap, float *disc, int *disc_rows, float *trace, int *itrs)
+{
+    const int dt_mult = 10;
+    const int opt_cap = path_cap * dt_mult;
+    float *opt = (float *)malloc(sizeof(float) * (size_t)opt_cap * DIM);
+    float *path = (float *)malloc(sizeof(float) * (size_t)path_cap * DIM);
+    float *xs_opt = (float *)malloc(sizeof(float) * (size_t)n * DIM);
+    float *cum = (float *)malloc(sizeof(float) * (size_t)n);
+    int rc = 0;
+    int opt_len = 0, len = 0;
+    /* the kept path starts as the solution path */
+    po_find_optimal_path(dt, best_path, xs, n, &len, path_cap);
+    if (len < 0) {
+        rc = -1;
+        goto done;
+    }
+    *best_rows = len;
+    float best_cost = soln_cost, best_cp = soln_cp;
+    /* the obstacle-free optimal path from the first to the last state */
+    {
+        float ends[2 * DIM];
+        for (int d = 0; d < DIM; ++d) {
+            ends[d] = xs[d];
+            ends[DIM + d] = xs[(n - 1) * DIM + d];
+        }
+        po_find_optimal_path(dt / dt_mult, opt, ends, 2, &opt_len, opt_cap);
+        if (opt_len < 0) {
+            rc = -1;
+            goto done;
+        }
+    }
+    cum[0] = 0;
+    for (int i = 0; i < n - 1; ++i)
+        cum[i + 1] = cum[i] + po_topt_bisection(&xs[i * DIM], &xs[(i + 1) * DIM], 2);
+    for (int i = 0; i < n; ++i) {
+        int idx = (int)((float)(opt_len - 1) * cum[i] / cum[n - 1]);
+        for (int d = 0; d < DIM; ++d)
+            xs_opt[i * DIM + d] = opt[idx * DIM + d];
+    }
+    {
+        float alpha = 1.0f, alpha_u = 1.0f, alpha_l = 0.0f;
+        int k = 0;
+        for (k = 0; k < 30;) {
+            for (int i = 0; i < n * DIM; ++i)
+                xs_smooth[i] = (1 - alpha) * xs[i] + alpha * xs_opt[i];
+            po_find_optimal_path(dt, path, xs_smooth, n, &len, path_cap);
+            if (len < 0 || len > tmax_steps) {
+                rc = -1;
+                goto done;
+            }
+            float cost = 0;
+            for (int i = 0; i < n - 1; ++i) {
+                float tau = po_topt_bisection(&xs_smooth[i * DIM], &xs_smooth[(i + 1) * DIM], 2);
+                cost += po_cost(tau, &xs_smooth[i * DIM], &xs_smooth[(i + 1) * DIM]);
+            }
+            int hits = po_mccp_hits(obstacles, n_obs, xcomb, tmax_steps, n_mc, offset_mult, path,
+                                    len, 0);
+            float cp = ((float)hits) / ((float)n_mc);
+            trace[k * 3 + 0] = alpha;
+            trace[k * 3 + 1] = cp;
+            trace[k * 3 + 2] = cost;
+            if (cp < cp_target)
+                alpha_l = alpha;
+            if (cp >= cp_target)
+                alpha_u = alpha;
+            alpha = (alpha_l + alpha_u) / 2;
+            int satisfies = cp < cp_target;
+            int closer = cp > best_cp;
+            int equal_lower = (cp == best_cp && cost < best_cost);
+            if (satisfies && (closer || equal_lower)) {
+                memcpy(best_path, path, sizeof(float) * (size_t)len * DIM);
+                *best_rows = len;
+                best_cost = cost;
+                best_cp = cp;
+            }
+            if (fabsf(cp_target - cp) < 0.0002f && cp_target - cp > 0) {
+                ++k;
+                break;
+            }
+            ++k;
+        }
+        *itrs = k;
+    }
+    best_cost_cp[0] = best_cost;
+    best_cost_cp[1] = best_cp;
+    /* controller discretisation of the last iterate */
+    {
+        const float control_dt = 0.05f;
+        int rows = 1;
+        for (int i = 0; i < n - 1; ++i) {
+            float tau = po_topt_bisection(&xs_smooth[i * DIM], &xs_smooth[(i + 1) * DIM], 2);
+            rows += tau / control_dt + 1;
+        }
+        if (rows > disc_cap) {
+            rc = -1;
+            goto done;
+        }
+        memset(disc, 0, sizeof(float) * (size_t)rows * DIM);
+        int cur = 0;
+        for (int i = 0; i < n - 1; ++i) {
+            float tau = po_topt_bisection(&xs_smooth[i * DIM], &xs_smooth[(i + 1) * DIM], 2);
+            int nw = tau / control_dt + 1;
+            if (cur + nw + 1 > rows) {
+                rc = -1;
+                goto done;
+            }
+            po_find_discretized_path(&disc[(size_t)cur * DIM], &xs_smooth[i * DIM],
+                                     &xs_smooth[(i + 1) * DIM], nw);
+            cur += nw;
+        }
+        *disc_rows = rows;
+    }
+done:
+    free(opt);
+    free(path);
+    free(xs_opt);
+    free(cum);
+    return rc;
+}

@@ -156,6 +156,12 @@ int    po64_mccp_hits(const float *obstacles, int n_obs, const float *xcomb, int
 /* path sampled in FP64 at the same dt schedule as findOptimalPath */
 int    po64_find_optimal_path(double dt, double *split, const float *xs, int n_wp, int cap);
 
+/* smoothing + controller discretisation (PUMP.cu:406-568) */
+int po_smooth(const float *xs, int n, float soln_cost, float soln_cp, const float *obstacles,
+              int n_obs, const float *xcomb, int tmax_steps, int n_mc, float offset_mult, float dt,
+              float cp_target, int path_cap, float *best_path, int *best_rows, float *best_cost_cp,
+              float *xs_smooth, int disc_cap, float *disc, int *disc_rows, float *trace, int *itrs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -117,6 +117,13 @@ class BatchInfo(ctypes.Structure):
                 ("ms_total", c_float)]
 
 
+class SmoothResult(ctypes.Structure):
+    _fields_ = [("status", c_int), ("itrs", c_int), ("mccp_launches", c_int), ("n_nodes", c_int),
+                ("path_rows", c_int), ("discretized_rows", c_int), ("cost_before", c_float),
+                ("cost", c_float), ("cp", c_float), ("ms_total", c_float),
+                ("trace", (c_float * 3) * 32)]
+
+
 S = c_void_p  # stream / device pointer
 _SIGS = {
     # pump_abi.h
@@ -181,6 +188,8 @@ _SIGS = {
     "pump_roadmap_copy_halfspaces": (c_int, [c_void_p, f32p, f32p, i32p]),
     "pump_roadmap_device": (c_int, [c_void_p, ctypes.POINTER(RoadmapDev)]),
     "pump_search_run": (c_int, [c_void_p, ctypes.POINTER(Params), ctypes.POINTER(SearchResult)]),
+    "pump_smooth_run": (c_int, [c_void_p, ctypes.POINTER(Params), ctypes.POINTER(SearchResult),
+                                ctypes.POINTER(SmoothResult), f32p, c_int, f32p, f32p, c_int]),
     "pump_batch_build": (c_int, [c_int, ctypes.POINTER(BatchInstance), f32p, f32p, c_int, c_int,
                                  c_int, ctypes.POINTER(c_void_p)]),
     "pump_batch_info_get": (c_int, [c_void_p, ctypes.POINTER(BatchInfo)]),
@@ -404,6 +413,19 @@ class Roadmap:
         check("pump_search_run",
               lib.pump_search_run(self._h, ctypes.byref(prm or self.prm), ctypes.byref(r)))
         return r
+
+    def smooth(self, search_result, prm=None, path_cap=4096, disc_cap=1 << 16):
+        """Smoothing and controller discretisation (PUMP.cu:406-568) of a
+        search result: (SmoothResult, kept path rows, last iterate's node
+        states, discretised rows)."""
+        out = SmoothResult()
+        path = np.zeros((path_cap, 6), np.float32)
+        xs = np.zeros((max(search_result.soln_len, 1), 6), np.float32)
+        disc = np.zeros((disc_cap, 6), np.float32)
+        check("pump_smooth_run", lib.pump_smooth_run(
+            self._h, ctypes.byref(prm or self.prm), ctypes.byref(search_result),
+            ctypes.byref(out), fp(path), path_cap, fp(xs), fp(disc), disc_cap))
+        return out, path[: out.path_rows].copy(), xs, disc[: out.discretized_rows].copy()
 
 
 class Batch:

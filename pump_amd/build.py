@@ -32,6 +32,7 @@ SOURCES = [
     "host_gen.hip",
     "host_pipeline.hip",
     "search_device.hip",
+    "smooth.hip",
     "runtime.hip",
 ]
 
