@@ -60,6 +60,28 @@ void pump_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[
 
 int pump_gen_problem(const pump_problem_spec *spec, float *samples_host, float *obstacles_host);
 
+/* ---- the reference's problem files and sampler -------------------------
+ * pump_read_problem: the problem file of mainPUMP.cu:152-230 (the
+ * reference's input.txt / indoor.txt): comma-separated init, goal, lo, hi
+ * (6 values each), offsetMult, the obstacle count, then 12 values per
+ * obstacle (lowest corner, highest corner); the rest of the file is
+ * ignored.  Values parse as `stream >> float` (strtof).  A missing or
+ * unreadable value returns PUMP_E_INVALID (the reference keeps 0 there);
+ * more than obs_cap obstacles returns PUMP_E_CAPACITY.
+ * pump_halton_samples: createSamplesHalton (sampler.cu:20-57), NUM samples
+ * from the Halton bases bases[d + skip] (2, 3, 5, ...), sample 0 = init,
+ * sample num-1 = goal. */
+typedef struct pump_problem_file {
+    float init[6], goal[6], lo[6], hi[6];
+    float offset_mult;
+    int n_obs;
+} pump_problem_file;
+
+int pump_read_problem(const char *path, pump_problem_file *out, float *obstacles_host,
+                      int obs_cap);
+int pump_halton_samples(int num, int skip, const float *init, const float *goal, const float *lo,
+                        const float *hi, float *samples_host);
+
 /* ---- noise tables ------------------------------------------------------
  * Closed-loop double-integrator offsets from the nominal path: per axis
  * x_{t+1} = A_cl x_t + L z_t with A_cl the dt-discretised double integrator

@@ -84,6 +84,7 @@ _SIGS = {
     "po_mccp_hits": (c_int, [f32p, c_int, f32p, c_int, c_int, c_float, f32p, c_int, i32p]),
     "po_collision_probability": (c_float, [f32p, c_int, f32p, c_int, c_int, c_float, f32p, c_int]),
     "po_search": (c_int, [ctypes.POINTER(SearchIn), ctypes.POINTER(SearchOut)]),
+    "po_halton_samples": (c_int, [c_int, c_int, f32p, f32p, f32p, f32p, f32p]),
     "po_smooth": (c_int, [f32p, c_int, c_float, c_float, f32p, c_int, f32p, c_int, c_int, c_float,
                           c_float, c_float, c_int, f32p, i32p, f32p, f32p, c_int, f32p, i32p,
                           f32p, i32p]),

@@ -1389,3 +1389,37 @@ done:
     free(cum);
     return rc;
 }
+
+/* createSamplesHalton -- sampler.cu:20-57: dimension d of sample n is the
+ * radical inverse of n in base primes[d + skip], in the reference's float /
+ * int steps, scaled to [lo, hi]; sample 0 = init, sample num-1 = goal.
+ * Pinned by: base 2 is exact (the van der Corput values are dyadic), every
+ * other base within a few ulps of the exact radical inverse. */
+int po_halton_samples(int num, int skip, const float *init, const float *goal, const float *lo,
+                      const float *hi, float *samples)
+{
+    static const int primes[25] = {2,  3,  5,  7,  11, 13, 17, 19, 23, 29, 31, 37, 41,
+                                   43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97};
+    if (skip < 0 || skip + DIM > 25)
+        return -1;
+    for (int d = 0; d < DIM; ++d) {
+        const int b = primes[d + skip];
+        for (int n = 0; n < num; ++n) {
+            float hn = 0, f = 1 / ((float)b);
+            int n0 = n;
+            while (n0 > 0) {
+                float n1 = n0 / b;
+                int r = n0 - n1 * b;
+                hn += f * r;
+                f = f / b;
+                n0 = n1;
+            }
+            samples[n * DIM + d] = hn * (hi[d] - lo[d]) + lo[d];
+        }
+    }
+    for (int d = 0; d < DIM; ++d) {
+        samples[d] = init[d];
+        samples[(num - 1) * DIM + d] = goal[d];
+    }
+    return 0;
+}

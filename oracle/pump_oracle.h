@@ -156,6 +156,9 @@ int    po64_mccp_hits(const float *obstacles, int n_obs, const float *xcomb, int
 /* path sampled in FP64 at the same dt schedule as findOptimalPath */
 int    po64_find_optimal_path(double dt, double *split, const float *xs, int n_wp, int cap);
 
+/* createSamplesHalton (sampler.cu:20-57) */
+int po_halton_samples(int num, int skip, const float *init, const float *goal, const float *lo,
+                      const float *hi, float *samples);
 /* smoothing + controller discretisation (PUMP.cu:406-568) */
 int po_smooth(const float *xs, int n, float soln_cost, float soln_cp, const float *obstacles,
               int n_obs, const float *xcomb, int tmax_steps, int n_mc, float offset_mult, float dt,

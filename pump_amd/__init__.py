@@ -117,6 +117,11 @@ class BatchInfo(ctypes.Structure):
                 ("ms_total", c_float)]
 
 
+class ProblemFile(ctypes.Structure):
+    _fields_ = [("init", c_float * 6), ("goal", c_float * 6), ("lo", c_float * 6),
+                ("hi", c_float * 6), ("offset_mult", c_float), ("n_obs", c_int)]
+
+
 class SmoothResult(ctypes.Structure):
     _fields_ = [("status", c_int), ("itrs", c_int), ("mccp_launches", c_int), ("n_nodes", c_int),
                 ("path_rows", c_int), ("discretized_rows", c_int), ("cost_before", c_float),
@@ -188,6 +193,8 @@ _SIGS = {
     "pump_roadmap_copy_halfspaces": (c_int, [c_void_p, f32p, f32p, i32p]),
     "pump_roadmap_device": (c_int, [c_void_p, ctypes.POINTER(RoadmapDev)]),
     "pump_search_run": (c_int, [c_void_p, ctypes.POINTER(Params), ctypes.POINTER(SearchResult)]),
+    "pump_read_problem": (c_int, [ctypes.c_char_p, ctypes.POINTER(ProblemFile), f32p, c_int]),
+    "pump_halton_samples": (c_int, [c_int, c_int, f32p, f32p, f32p, f32p, f32p]),
     "pump_smooth_run": (c_int, [c_void_p, ctypes.POINTER(Params), ctypes.POINTER(SearchResult),
                                 ctypes.POINTER(SmoothResult), f32p, c_int, f32p, f32p, c_int]),
     "pump_batch_build": (c_int, [c_int, ctypes.POINTER(BatchInstance), f32p, f32p, c_int, c_int,
@@ -279,6 +286,27 @@ def gen_problem(spec):
     o = np.zeros((max(spec.n_obs, 1), 12), np.float32)
     check("pump_gen_problem", lib.pump_gen_problem(ctypes.byref(spec), fp(s), fp(o)))
     return s, o[: spec.n_obs].copy()
+
+
+def read_problem(path, obs_cap=4096):
+    """The reference's problem file (mainPUMP.cu:152-230): dict of init,
+    goal, lo, hi (float32[6]), offset_mult and obstacles [n_obs, 12]."""
+    pf = ProblemFile()
+    obs = np.zeros((obs_cap, 12), np.float32)
+    check("pump_read_problem", lib.pump_read_problem(os.fsencode(path), ctypes.byref(pf),
+                                                      fp(obs), obs_cap))
+    arr = lambda a: np.array(list(a), np.float32)
+    return dict(init=arr(pf.init), goal=arr(pf.goal), lo=arr(pf.lo), hi=arr(pf.hi),
+                offset_mult=np.float32(pf.offset_mult), obstacles=obs[: pf.n_obs].copy())
+
+
+def halton_samples(num, init, goal, lo, hi, skip=0):
+    """createSamplesHalton (sampler.cu:20-57): [num, 6] float32."""
+    out = np.zeros((num, 6), np.float32)
+    args = [np.ascontiguousarray(a, np.float32) for a in (init, goal, lo, hi)]
+    check("pump_halton_samples", lib.pump_halton_samples(num, skip, *[fp(a) for a in args],
+                                                          fp(out)))
+    return out
 
 
 def noise_spec(**kw):

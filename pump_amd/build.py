@@ -33,6 +33,7 @@ SOURCES = [
     "host_pipeline.hip",
     "search_device.hip",
     "smooth.hip",
+    "problem_io.hip",
     "runtime.hip",
 ]
 
