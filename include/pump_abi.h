@@ -278,6 +278,40 @@ int pump_collision_probability_many(const float *d_obstacles, int n_obs, const f
                                     const int *d_T, int n_paths, int *d_hits, int zero_hits,
                                     pump_stream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Group Marching Tree and Euclidean neighbours (not in the reference's
+ * build; listed API surface)
+ * ------------------------------------------------------------------------ */
+
+/* calculateNumberNN (precomp.cu:3-30): d_nn_sizes[i] = #{j != i :
+ * |x_j - x_i|^2 < r2} over all 6 state dimensions. */
+int pump_precomp_nn_count(const float *d_samples, int num, float r2, int *d_nn_sizes,
+                          pump_stream_t stream);
+/* setupPrecompNN + precompNN (precomp.cu:32-76): d_nn_edges/d_distances
+ * [num x nn_size], neighbours in sample order with their distance, -1 / 0
+ * past the last one (at most nn_size kept per node). */
+int pump_precomp_nn(const float *d_samples, int num, float r2, int nn_size, float *d_distances,
+                    int *d_nn_edges, pump_stream_t stream);
+
+/* GMTwavefront (GMT.cu:3-115, lambda = 1) over the padded neighbour tables
+ * of pump_nn_tables (or precomp): writes the tree's costs and parents
+ * (d_costs, d_edges [num]; -11 / -2 on blocked samples, 0 / -1 where not
+ * reached), the goal's cost (0 if not reached) and the iterations.  The
+ * connecting edge from parent to node is found through d_nn_idxs (dense
+ * [to*num + from], mainPUMP.cu:338) when given, else by the CSR rows
+ * d_row_start / d_edge_dst; its waypoints are d_disc at d_wp_offsets (or
+ * numDisc+1 per edge when NULL).  Node 0 starts the wavefront
+ * (GMT.cu:128-134).  Workspace: pump_gmt_workspace(num) bytes.
+ * Synchronises once per iteration. */
+size_t pump_gmt_workspace(int num);
+int pump_gmt_wavefront(const float *d_obstacles, int n_obs, const float *d_distances_come,
+                       const int *d_nn_go_edges, const int *d_nn_come_edges, int nn_size,
+                       const float *d_disc, const int64_t *d_wp_offsets, int num_disc,
+                       const int *d_nn_idxs, const int64_t *d_row_start, const int *d_edge_dst,
+                       const uint8_t *d_is_free_samples, int num, float *d_costs, int *d_edges,
+                       int goal_idx, void *d_ws, size_t ws_bytes, float *cost_goal_host,
+                       int *itrs_host, pump_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

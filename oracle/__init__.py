@@ -84,6 +84,9 @@ _SIGS = {
     "po_mccp_hits": (c_int, [f32p, c_int, f32p, c_int, c_int, c_float, f32p, c_int, i32p]),
     "po_collision_probability": (c_float, [f32p, c_int, f32p, c_int, c_int, c_float, f32p, c_int]),
     "po_search": (c_int, [ctypes.POINTER(SearchIn), ctypes.POINTER(SearchOut)]),
+    "po_precomp_nn": (None, [f32p, c_int, c_float, c_int, i32p, f32p, i32p]),
+    "po_gmt_wavefront": (c_int, [f32p, c_int, f32p, i32p, i32p, c_int, f32p, i64p, i64p, i32p,
+                                 u8p, c_int, f32p, i32p, c_int, f32p, i32p]),
     "po_halton_samples": (c_int, [c_int, c_int, f32p, f32p, f32p, f32p, f32p]),
     "po_smooth": (c_int, [f32p, c_int, c_float, c_float, f32p, c_int, f32p, c_int, c_int, c_float,
                           c_float, c_float, c_int, f32p, i32p, f32p, f32p, c_int, f32p, i32p,

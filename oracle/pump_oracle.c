@@ -1423,3 +1423,156 @@ int po_halton_samples(int num, int skip, const float *init, const float *goal, c
     }
     return 0;
 }
+
+/* ======================================================================
+ * Group Marching Tree and Euclidean neighbours (not in the reference's
+ * build; listed API surface)
+ * ==================================================================== */
+
+/* calculateNumberNN + setupPrecompNN + precompNN -- precomp.cu:3-76: the
+ * samples within squared distance r2 (all 6 dimensions, sum in dimension
+ * order), in sample order, with their distance sqrtf(d2); -1 / 0 past the
+ * last one.  nn_sizes receives every node's full count; at most nn_size are
+ * kept.  Pinned by a float32 brute force in the same summation order. */
+void po_precomp_nn(const float *samples, int num, float r2, int nn_size, int *nn_sizes,
+                   float *distances, int *nn_edges)
+{
+    for (int node = 0; node < num; ++node) {
+        int c = 0;
+        for (int k = 0; k < nn_size; ++k) {
+            nn_edges[(int64_t)node * nn_size + k] = -1;
+            distances[(int64_t)node * nn_size + k] = 0;
+        }
+        for (int s = 0; s < num; ++s) {
+            if (s == node)
+                continue;
+            float d2 = 0;
+            for (int d = 0; d < DIM; ++d) {
+                float diff = samples[node * DIM + d] - samples[s * DIM + d];
+                d2 += diff * diff;
+            }
+            if (d2 < r2) {
+                if (c < nn_size) {
+                    nn_edges[(int64_t)node * nn_size + c] = s;
+                    distances[(int64_t)node * nn_size + c] = sqrtf(d2);
+                }
+                ++c;
+            }
+        }
+        nn_sizes[node] = c;
+    }
+}
+
+/* GMTwavefront -- GMT.cu:3-261 with lambda = 1, one step at a time:
+ * setupArrays; then per iteration findWavefront, the new wavefront,
+ * findOptimalConnection (first minimum in come-edge order; a cost of 0 is
+ * "unset"), verifyExpansion + removeInvalidExpansions (the connecting edge's
+ * segments, found in the CSR rows), updateWavefront; until the goal has a
+ * non-zero cost, the new wavefront is empty, or 100 iterations.
+ * Pinned by: every reached node's cost is its parent's plus the come-edge
+ * distance and its parent was reached earlier; the goal cost is at least the
+ * shortest-path cost; on a chain the tree is the chain. */
+int po_gmt_wavefront(const float *obstacles, int n_obs, const float *distances_come,
+                     const int *nn_go, const int *nn_come, int nn_size, const float *disc,
+                     const int64_t *wp_offsets, const int64_t *row_start, const int *edge_dst,
+                     const uint8_t *is_free_samples, int num, float *costs, int *edges, int goal,
+                     float *cost_goal, int *itrs_out)
+{
+    uint8_t *wf = (uint8_t *)calloc((size_t)num, 1), *wf_new = (uint8_t *)calloc((size_t)num, 1);
+    uint8_t *was = (uint8_t *)calloc((size_t)num, 1), *unv = (uint8_t *)calloc((size_t)num, 1);
+    int rc = 0;
+    for (int n = 0; n < num; ++n) {
+        unv[n] = is_free_samples[n];
+        was[n] = !is_free_samples[n];
+        wf[n] = n == 0;
+        costs[n] = is_free_samples[n] ? 0.0f : -11.0f;
+        edges[n] = is_free_samples[n] ? -1 : -2;
+    }
+    float cg = 0;
+    int itrs = 0;
+    while (itrs < 100 && cg == 0) {
+        ++itrs;
+        for (int node = 0; node < num; ++node) {
+            if (!wf[node])
+                continue;
+            was[node] = 1;
+            for (int i = 0; i < nn_size; ++i) {
+                int nn = nn_go[(int64_t)node * nn_size + i];
+                if (nn == -1)
+                    break;
+                if (unv[nn] && !wf[nn])
+                    wf_new[nn] = 1;
+            }
+        }
+        int n_new = 0;
+        for (int node = 0; node < num; ++node)
+            n_new += wf_new[node];
+        if (n_new == 0)
+            break;
+        for (int node = 0; node < num; ++node) {
+            if (!wf_new[node])
+                continue;
+            for (int i = 0; i < nn_size; ++i) {
+                int nn = nn_come[(int64_t)node * nn_size + i];
+                if (nn == -1)
+                    break;
+                if (wf[nn]) {
+                    float c = costs[nn] + distances_come[(int64_t)node * nn_size + i];
+                    if (costs[node] == 0 || c < costs[node]) {
+                        costs[node] = c;
+                        edges[node] = nn;
+                    }
+                }
+            }
+            int from = edges[node];
+            int64_t e = -1;
+            for (int64_t q = from >= 0 ? row_start[from] : 0; from >= 0 && q < row_start[from + 1];
+                 ++q)
+                if (edge_dst[q] == node) {
+                    e = q;
+                    break;
+                }
+            if (e < 0) {
+                rc = -1;
+                goto done;
+            }
+            int collides = 0;
+            for (int64_t g = wp_offsets[e]; g + 1 < wp_offsets[e + 1] && !collides; ++g) {
+                float v[HALF], w[HALF], bmin[HALF], bmax[HALF];
+                for (int d = 0; d < HALF; ++d) {
+                    v[d] = disc[g * DIM + d];
+                    w[d] = disc[(g + 1) * DIM + d];
+                    bmin[d] = v[d] > w[d] ? w[d] : v[d];
+                    bmax[d] = v[d] > w[d] ? v[d] : w[d];
+                }
+                collides = !po_is_motion_valid(v, w, bmin, bmax, n_obs, obstacles);
+            }
+            if (collides) {
+                costs[node] = 0;
+                edges[node] = -1;
+                wf_new[node] = 0;
+            }
+        }
+        for (int node = 0; node < num; ++node) {
+            if (!unv[node])
+                continue;
+            if (was[node]) {
+                wf[node] = 0;
+                unv[node] = 0;
+            } else if (wf_new[node]) {
+                wf[node] = 1;
+                wf_new[node] = 0;
+                if (node == goal)
+                    cg = costs[node];
+            }
+        }
+    }
+    *cost_goal = cg;
+    *itrs_out = itrs;
+done:
+    free(wf);
+    free(wf_new);
+    free(was);
+    free(unv);
+    return rc;
+}

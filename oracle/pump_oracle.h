@@ -156,6 +156,14 @@ int    po64_mccp_hits(const float *obstacles, int n_obs, const float *xcomb, int
 /* path sampled in FP64 at the same dt schedule as findOptimalPath */
 int    po64_find_optimal_path(double dt, double *split, const float *xs, int n_wp, int cap);
 
+/* precomp.cu:3-76 and GMT.cu:3-261 */
+void po_precomp_nn(const float *samples, int num, float r2, int nn_size, int *nn_sizes,
+                   float *distances, int *nn_edges);
+int po_gmt_wavefront(const float *obstacles, int n_obs, const float *distances_come,
+                     const int *nn_go, const int *nn_come, int nn_size, const float *disc,
+                     const int64_t *wp_offsets, const int64_t *row_start, const int *edge_dst,
+                     const uint8_t *is_free_samples, int num, float *costs, int *edges, int goal,
+                     float *cost_goal, int *itrs_out);
 /* createSamplesHalton (sampler.cu:20-57) */
 int po_halton_samples(int num, int skip, const float *init, const float *goal, const float *lo,
                       const float *hi, float *samples);

@@ -174,6 +174,11 @@ _SIGS = {
     "pump_collision_probability": (c_int, [S, c_int, S, c_int, c_int, c_float, S, c_int, f32p, S]),
     "pump_collision_probability_many": (c_int, [S, c_int, S, c_int, c_int, c_float, S, c_int64, S,
                                                 c_int, S, c_int, S]),
+    "pump_precomp_nn_count": (c_int, [S, c_int, c_float, S, S]),
+    "pump_precomp_nn": (c_int, [S, c_int, c_float, c_int, S, S, S]),
+    "pump_gmt_workspace": (c_size_t, [c_int]),
+    "pump_gmt_wavefront": (c_int, [S, c_int, S, S, S, c_int, S, S, c_int, S, S, S, S, c_int, S, S,
+                                   c_int, S, c_size_t, f32p, i32p, S]),
     # pump_host.h
     "pump_philox4x32": (None, [u32p, u32p, u32p]),
     "pump_gen_problem": (c_int, [ctypes.POINTER(ProblemSpec), f32p, f32p]),
@@ -307,6 +312,59 @@ def halton_samples(num, init, goal, lo, hi, skip=0):
     check("pump_halton_samples", lib.pump_halton_samples(num, skip, *[fp(a) for a in args],
                                                           fp(out)))
     return out
+
+
+def precomp_nn(samples, r, nn_size=None):
+    """Euclidean neighbours within r (precomp.cu:3-76) on the device:
+    (nn_sizes [num], distances [num, nn_size], nn_edges [num, nn_size]).
+    nn_size defaults to the largest count; rows keep their first nn_size
+    neighbours in sample order and pad with -1 / 0."""
+    s = DeviceArray(np.ascontiguousarray(samples, np.float32))
+    num, r2 = s.shape[0], float(np.float32(r) * np.float32(r))
+    sizes = DeviceArray(shape=(num,), dtype=np.int32)
+    check("pump_precomp_nn_count", lib.pump_precomp_nn_count(s.p, num, r2, sizes.p, None))
+    counts = sizes.numpy()
+    if nn_size is None:
+        nn_size = int(counts.max()) if num else 0
+    dist = DeviceArray(shape=(num, nn_size), dtype=np.float32)
+    edges = DeviceArray(shape=(num, nn_size), dtype=np.int32)
+    check("pump_precomp_nn", lib.pump_precomp_nn(s.p, num, r2, nn_size, dist.p, edges.p, None))
+    return counts, dist.numpy(), edges.numpy()
+
+
+def gmt_wavefront(obstacles, nn_go_edges, nn_come_edges, distances_come, disc, is_free_samples,
+                  goal_idx, wp_offsets=None, num_disc=0, nn_idxs=None, row_start=None,
+                  edge_dst=None):
+    """GMTwavefront (GMT.cu:3-261) on the device from host arrays: returns
+    (costs [num], edges [num], cost_goal, itrs).  The connecting edge of a
+    node is looked up in nn_idxs ([num, num], nnIdxs[to*NUM + from]) when
+    given, else in the CSR rows (row_start, edge_dst); its waypoints are the
+    rows wp_offsets[e]..wp_offsets[e+1]-1 of disc, or num_disc+1 rows per
+    edge when wp_offsets is None."""
+    go = np.ascontiguousarray(nn_go_edges, np.int32)
+    num, m = go.shape
+    hold = []
+
+    def up(a, dt):
+        if a is None:
+            return None
+        d = DeviceArray(np.ascontiguousarray(a, dt))
+        hold.append(d)
+        return d.p
+
+    obs = np.ascontiguousarray(obstacles, np.float32).reshape(-1, 12)
+    costs = DeviceArray(shape=(num,), dtype=np.float32)
+    edges = DeviceArray(shape=(num,), dtype=np.int32)
+    ws = DeviceArray(shape=(lib.pump_gmt_workspace(num),), dtype=np.uint8)
+    cg, it = np.zeros(1, np.float32), np.zeros(1, np.int32)
+    check("pump_gmt_wavefront", lib.pump_gmt_wavefront(
+        up(obs, np.float32) if len(obs) else None, len(obs), up(distances_come, np.float32),
+        up(go, np.int32), up(nn_come_edges, np.int32), m, up(disc, np.float32),
+        up(wp_offsets, np.int64), num_disc, up(nn_idxs, np.int32), up(row_start, np.int64),
+        up(edge_dst, np.int32), up(is_free_samples, np.uint8), num, costs.p, edges.p, goal_idx,
+        ws.p, ws.nbytes, fp(cg), ip(it), None))
+    sync()
+    return costs.numpy(), edges.numpy(), float(cg[0]), int(it[0])
 
 
 def noise_spec(**kw):

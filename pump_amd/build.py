@@ -34,6 +34,7 @@ SOURCES = [
     "search_device.hip",
     "smooth.hip",
     "problem_io.hip",
+    "gmt.hip",
     "runtime.hip",
 ]
 
