@@ -268,6 +268,9 @@ float pump_host_cost(float tau, const float *x0, const float *x1);
 void pump_host_path_point(float t, float tau, const float *x0, const float *x1, float *x);
 /* the kernels' FP64 division a/b (reciprocal + residual steps), n pairs */
 void pump_host_div_rn(const double *a, const double *b, double *q, int64_t n);
+void pump_host_div_sq(const double *a, const float *tau, double *q, int64_t n);
+/* the edge pass's bounding box of an edge's waypoint positions (lo[3], hi[3]) */
+void pump_host_edge_bbox(const float *x0, const float *x1, float tau, float *lo, float *hi);
 
 #ifdef __cplusplus
 }

@@ -229,6 +229,8 @@ _SIGS = {
     "pump_host_cost": (c_float, [c_float, f32p, f32p]),
     "pump_host_path_point": (None, [c_float, c_float, f32p, f32p, f32p]),
     "pump_host_div_rn": (None, [S, S, S, c_int64]),
+    "pump_host_div_sq": (None, [S, S, S, c_int64]),
+    "pump_host_edge_bbox": (None, [f32p, f32p, c_float, f32p, f32p]),
 }
 
 for _name, (_res, _args) in _SIGS.items():

@@ -62,31 +62,6 @@ __host__ __device__ inline EdgeLds edge_lds(int epb, int obs_cap, int words_cap)
 __host__ __device__ inline int edge_obs_cap(int n_obs) { return n_obs < kEdgeObs ? n_obs : kEdgeObs; }
 __host__ __device__ inline int edge_words(int obs_cap) { return (obs_cap + 63) >> 6; }
 
-// Bounding box of the positions of every waypoint of an edge.  The position
-// part of pathPoint (2pBVP.cu:117-131) is the cubic Hermite segment through
-// (x0, v0) and (x1, v1) over [0, tau], which lies in the convex hull of its
-// Bezier control points x0, x0 + v0 tau/3, x1 - v1 tau/3, x1.  The waypoints
-// are that cubic evaluated with FP32 coefficients and FP32 results: their
-// distance from it is below 30 * 2^-24 * S, S = |p0| + |p1| + |v0 tau| +
-// |v1 tau|, and the margin 1e-5 (1 + S) covers it with room to spare.
-__device__ __forceinline__ void edge_bbox(const float *x0, const float *x1, float tau, float *lo,
-                                          float *hi)
-{
-    const double t3 = (double)tau / 3.0;
-    for (int d = 0; d < 3; ++d) {
-        const double p0 = x0[d], p3 = x1[d];
-        const double w0 = (double)x0[d + 3] * t3, w1 = (double)x1[d + 3] * t3;
-        const double p1 = p0 + w0, p2 = p3 - w1;
-        const double mn = fmin(fmin(p0, p1), fmin(p2, p3));
-        const double mx = fmax(fmax(p0, p1), fmax(p2, p3));
-        const double m = 1e-5 * (1.0 + fabs(p0) + fabs(p3) + 3.0 * (fabs(w0) + fabs(w1)));
-        const float l = (float)(mn - m), h = (float)(mx + m);
-        const bool finite = l == l && h == h && fabsf(l) < INFINITY && fabsf(h) < INFINITY;
-        lo[d] = finite ? l : -INFINITY;
-        hi[d] = finite ? h : INFINITY;
-    }
-}
-
 // index q of the edge with loff[q] <= g < loff[q + 1] (local waypoint index)
 __device__ __forceinline__ int find_edge_local(const int *loff, int ne, int g)
 {
@@ -145,57 +120,100 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
     const int n_masked = edge_obs_cap(n_obs); // staged boxes with candidate masks
     const int n_words = edge_words(n_masked);
     const int64_t g_begin = wp_offsets ? wp_offsets[e0] : e0 * (int64_t)J.w_fixed;
-    if (kCheck)
-        for (int q = t; q < n_masked * 3; q += kTile) {
-            const int b = q / 3, d = q - 3 * (q / 3);
-            olo[d * obs_cap + b] = obstacles[b * 12 + d];
-            ohi[d * obs_cap + b] = obstacles[b * 12 + 6 + d];
-        }
-    for (int q = t; q < ne * kDim; q += kTile) {
-        int le = q / kDim, d = q - kDim * (q / kDim);
-        x0s[le][d] = samples[(int64_t)edge_src[e0 + le] * kDim + d];
-        x1s[le][d] = samples[(int64_t)edge_dst[e0 + le] * kDim + d];
-    }
-    for (int q = t; q <= ne; q += kTile)
-        loff[q] = (int)((wp_offsets ? wp_offsets[e0 + q] : (e0 + q) * (int64_t)J.w_fixed) -
-                        g_begin);
-    if (kCheck)
-        for (int q = t; q < ne * n_words; q += kTile)
-            emask[q] = 0ull;
-    __syncthreads();
-    const int n_loc = loff[ne];
-    const bool mapped = n_loc <= kMapCap;
-    // kTile / EPB lanes per edge; part 0 forms the edge's constants, and
-    // every part sets the candidate bits of the boxes b = part (mod parts)
+    // kTile / EPB lanes per edge: lane t serves edge q = t % EPB as part
+    // t / EPB.  Every global read of the set-up is issued before the first
+    // barrier: the edge's arrival time, and each endpoint's state gathered
+    // once (part 0 the source, part 1 the destination; with one part, both).
     constexpr int kParts = kTile / EPB;
     const int q = t % EPB;
     const int part = __builtin_amdgcn_readfirstlane(t / EPB); // uniform within a wave
-    if (q < ne) {
-        const float tau = topts_edge[e0 + q];
-        if (part == 0) {
-            const TauPowers P = tau_powers(tau);
-            tps[q] = P;
-            polys[q] = edge_poly(x0s[q], x1s[q], P);
-            const int a = loff[q], W = loff[q + 1] - a;
-            dts[q] = tau / (float)(W - 1);
-            ok[q] = 1;
-            if (mapped)
-                for (int j = 0; j < W; ++j)
-                    wmap[a + j] = (uint8_t)q;
+    const bool own = q < ne;
+    const float tau = own ? topts_edge[e0 + q] : 0.0f;
+    if (own && part < 2) {
+        const int node = (part == 0 ? edge_src : edge_dst)[e0 + q];
+        float(*xs)[kDim] = part == 0 ? x0s : x1s;
+        for (int d = 0; d < kDim; ++d)
+            xs[q][d] = samples[(int64_t)node * kDim + d];
+    }
+    if (kParts == 1 && own) {
+        const int node = edge_dst[e0 + q];
+        for (int d = 0; d < kDim; ++d)
+            x1s[q][d] = samples[(int64_t)node * kDim + d];
+    }
+    if (kCheck)
+        for (int i = t; i < n_masked * 3; i += kTile) {
+            const int b = i / 3, d = i - 3 * (i / 3);
+            olo[d * obs_cap + b] = obstacles[b * 12 + d];
+            ohi[d * obs_cap + b] = obstacles[b * 12 + 6 + d];
         }
-        if (kCheck) {
-            float lo[3], hi[3];
+    for (int i = t; i <= ne; i += kTile)
+        loff[i] = (int)((wp_offsets ? wp_offsets[e0 + i] : (e0 + i) * (int64_t)J.w_fixed) -
+                        g_begin);
+    if (kCheck)
+        for (int i = t; i < ne * n_words; i += kTile)
+            emask[i] = 0ull;
+    __syncthreads();
+    const int n_loc = loff[ne];
+    const bool mapped = n_loc <= kMapCap;
+    // Part 0 forms the edge's constants; the other parts (or the only one)
+    // set the edge's candidate bits over the staged boxes.
+    if (own && part == 0) {
+        const TauPowers P = tau_powers(tau);
+        tps[q] = P;
+        polys[q] = edge_poly(x0s[q], x1s[q], P);
+        const int a = loff[q], W = loff[q + 1] - a;
+        dts[q] = tau / (float)(W - 1);
+        ok[q] = 1;
+        if (mapped)
+            for (int j = 0; j < W; ++j)
+                wmap[a + j] = (uint8_t)q;
+    }
+    constexpr int kMaskParts = kParts == 1 ? 1 : kParts - 1;
+    const int mpart = kParts == 1 ? 0 : part - 1;
+    if (kCheck && n_words > 0 && mpart >= 0) {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        if (own)
             edge_bbox(x0s[q], x1s[q], tau, lo, hi);
-            for (int wd = 0; wd < n_words; ++wd) {
-                unsigned long long m = 0ull;
-                const int b_end = n_masked < wd * 64 + 64 ? n_masked : wd * 64 + 64;
-                for (int b = wd * 64 + part; b < b_end; b += kParts) {
-                    const float *ob = obstacles + b * 12;
-                    const bool sep = (hi[0] <= ob[0]) | (ob[6] <= lo[0]) | (hi[1] <= ob[1]) |
-                                     (ob[7] <= lo[1]) | (hi[2] <= ob[2]) | (ob[8] <= lo[2]);
-                    m |= (unsigned long long)(!sep) << (b - wd * 64);
+        // The wave's union of its edges' boxes: a box separated from the
+        // union (the same <= test) is separated from each of them, so only
+        // the boxes that meet the union are tested edge by edge, with their
+        // bounds read from the lane that loaded them.
+        float ulo[3], uhi[3];
+        for (int d = 0; d < 3; ++d) {
+            ulo[d] = wave_min(lo[d]);
+            uhi[d] = wave_max(hi[d]);
+        }
+        unsigned long long pat = 0ull; // this part's boxes within a word
+        for (int i = mpart; i < 64; i += kMaskParts)
+            pat |= 1ull << i;
+        for (int wd = 0; wd < n_words; ++wd) {
+            const int bl = wd * 64 + lane;
+            float bx[6] = {0, 0, 0, 0, 0, 0}; // lo x, y, z, hi x, y, z of box bl
+            bool meets = false;
+            if (bl < n_masked) {
+                for (int d = 0; d < 3; ++d) {
+                    bx[d] = olo[d * obs_cap + bl];
+                    bx[3 + d] = ohi[d * obs_cap + bl];
                 }
-                if (m)
+                meets = !((uhi[0] <= bx[0]) | (bx[3] <= ulo[0]) | (uhi[1] <= bx[1]) |
+                          (bx[4] <= ulo[1]) | (uhi[2] <= bx[2]) | (bx[5] <= ulo[2]));
+            }
+            unsigned long long cand = __ballot(meets) & pat;
+            unsigned long long m = 0ull;
+            while (cand) {
+                const int i = __builtin_ctzll(cand);
+                cand &= cand - 1;
+                float c[6];
+                for (int d = 0; d < 6; ++d)
+                    c[d] = read_lane(bx[d], i);
+                const bool sep = (hi[0] <= c[0]) | (c[3] <= lo[0]) | (hi[1] <= c[1]) |
+                                 (c[4] <= lo[1]) | (hi[2] <= c[2]) | (c[5] <= lo[2]);
+                m |= (unsigned long long)(!sep) << i;
+            }
+            if (own) {
+                if (kMaskParts == 1)
+                    emask[q * n_words + wd] = m;
+                else if (m)
                     atomicOr(&emask[q * n_words + wd], m);
             }
         }

@@ -546,6 +546,20 @@ extern "C" void pump_host_div_rn(const double *a, const double *b, double *q, in
         q[i] = div_rn(a[i], b[i], 1.0 / b[i]);
 }
 
+extern "C" void pump_host_edge_bbox(const float *x0, const float *x1, float tau, float *lo,
+                                    float *hi)
+{
+    edge_bbox(x0, x1, tau, lo, hi);
+}
+
+extern "C" void pump_host_div_sq(const double *a, const float *tau, double *q, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        const double b = (double)tau[i] * (double)tau[i];
+        q[i] = div_sq(a[i], b, 1.0 / b);
+    }
+}
+
 extern "C" int pump_find_optimal_path(float dt, const float *xs, int n_wp, float *split, int cap,
                                       float *total_time)
 {

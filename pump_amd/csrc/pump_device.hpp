@@ -173,13 +173,35 @@ __host__ __device__ __forceinline__ TauPowers tau_powers(float tau)
 // 1 ulp, and a second step from a quotient within 1 ulp and a reciprocal
 // within 1/2 ulp rounds correctly.  Valid while a/b, a and b stay clear of
 // overflow and subnormals (|a| < 1e300, 1e-300 < |b| < 1e300 here).
+// The residuals are formed as b q - a and subtracted, which keeps the sign
+// of a zero quotient (a = -0 gives -0).
 __host__ __device__ __forceinline__ double div_rn(double a, double b, double r)
 {
     const double q0 = a * r;
-    const double e0 = fma(-b, q0, a);
-    const double q1 = fma(e0, r, q0);
-    const double e1 = fma(-b, q1, a);
-    return fma(e1, r, q1);
+    const double e0 = fma(b, q0, -a);
+    const double q1 = fma(-e0, r, q0);
+    const double e1 = fma(b, q1, -a);
+    return fma(-e1, r, q1);
+}
+
+// Correctly rounded a/b for a divisor b = tau^2, the exact square of a float,
+// from r = RN(1/b), in one residual step.  Write b = M^2 2^(2k) with M < 2^24
+// an integer, so b is a multiple of u = 2^(2k) >= 2^-48 b, and q = a/b.
+//   * q0 = RN(a r) is within 1.5 ulp(q) of q.  a - b q0 is a multiple of
+//     u ulp(q0) and at most 1.5 ulp(q) b < 2^49 u ulp(q) in magnitude, so the
+//     FMA forms it exactly.
+//   * q0 + (a - b q0) r = q + (q - q0)(b r - 1), |b r - 1| <= 2^-53: the
+//     exact value rounded by the second FMA is within 1.5 * 2^-53 ulp(q) of q.
+//   * A rounding midpoint m (odd multiple of ulp(q)/2) is not q: m b has at
+//     least 54 significant bits, a has 53.  And |q - m| = |a - m b| / b with
+//     a - m b a non-zero multiple of min(ulp(a), u ulp(q) / 2), so
+//     |q - m| >= 2^-49 ulp(q).
+// Hence the rounding lands on RN(q).  Same range conditions as div_rn.
+__host__ __device__ __forceinline__ double div_sq(double a, double b, double r)
+{
+    const double q0 = a * r;
+    const double e0 = fma(b, q0, -a);
+    return fma(-e0, r, q0);
 }
 
 // Per-edge constants of pathPoint (2pBVP.cu:117-131): every subexpression
@@ -233,10 +255,10 @@ __host__ __device__ __forceinline__ void path_point(float t, const TauPowers &P,
         const float p0 = x0[d], v0 = x0[d + 3];
         const float lin = t * v0 + p0;
         x[d] = (float)((double)lin + div_rn(s3 * E.c3[d], P.t3, P.r3) -
-                       div_rn(s2 * E.c2[d], P.t2, P.r2));
+                       div_sq(s2 * E.c2[d], P.t2, P.r2));
         const float f = -(2.0f * t * E.e[d]);
-        x[d + 3] = (float)(div_rn((double)f, P.t2, P.r2) + div_rn(3.0 * s2 * E.cv[d], P.t3, P.r3) +
-                           div_rn((double)v0 * q, P.t2, P.r2));
+        x[d + 3] = (float)(div_sq((double)f, P.t2, P.r2) + div_rn(3.0 * s2 * E.cv[d], P.t3, P.r3) +
+                           div_sq((double)v0 * q, P.t2, P.r2));
     }
     {
         const double a = (double)(t * x0[5]) * P.t3;
@@ -293,6 +315,33 @@ __host__ __device__ __forceinline__ void waypoint(int k, int W, const TauPowers 
                                          const float *x1, float *x)
 {
     waypoint_dt(k, W, P, P.tau / (float)(W - 1), x0, x1, x);
+}
+
+// Bounding box of the positions of every waypoint of an edge.  The position
+// part of pathPoint (2pBVP.cu:117-131) is the cubic Hermite segment through
+// (x0, v0) and (x1, v1) over [0, tau], which lies in the convex hull of its
+// Bezier control points x0, x0 + v0 tau/3, x1 - v1 tau/3, x1.  The waypoints
+// are that cubic with FP32-rounded coefficients, rounded to FP32: their
+// distance from it is below 30 * 2^-24 * S, S = |p0| + |p1| + |v0 tau| +
+// |v1 tau|.  The hull is formed here in FP32 (within 4 * 2^-24 * S of the
+// exact one, and lo/hi round by at most 2^-24 of their magnitude), so the
+// margin 1e-5 (1 + S) covers every deviation with room to spare.
+__host__ __device__ __forceinline__ void edge_bbox(const float *x0, const float *x1, float tau, float *lo,
+                                          float *hi)
+{
+    const float t3 = tau * (1.0f / 3.0f);
+    for (int d = 0; d < 3; ++d) {
+        const float p0 = x0[d], p3 = x1[d];
+        const float w0 = x0[d + 3] * t3, w1 = x1[d + 3] * t3;
+        const float p1 = p0 + w0, p2 = p3 - w1;
+        const float mn = fminf(fminf(p0, p1), fminf(p2, p3));
+        const float mx = fmaxf(fmaxf(p0, p1), fmaxf(p2, p3));
+        const float m = 1e-5f * (1.0f + fabsf(p0) + fabsf(p3) + 3.0f * (fabsf(w0) + fabsf(w1)));
+        const float l = mn - m, h = mx + m;
+        const bool finite = fabsf(l) < INFINITY && fabsf(h) < INFINITY; // false for NaN too
+        lo[d] = finite ? l : -INFINITY;
+        hi[d] = finite ? h : INFINITY;
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -206,6 +206,39 @@ def test_path_point_division_is_correctly_rounded():
     assert bad.size == 0, (bad[:5], got[bad[:2]], ref[bad[:2]])
 
 
+@pytest.mark.parametrize("scale", [1.0, 30.0])
+def test_edge_bbox_contains_every_waypoint(scale):
+    """The edge pass drops a box for a whole edge when it is separated from
+    this bounding box, so every waypoint position of the edge (interior
+    points at k * RN(tau / (W-1)), endpoints copied) must lie inside it:
+    random edges at the workload's magnitudes and at 30x larger states."""
+    rng = np.random.default_rng(int(scale))
+    n = 40000
+    lo_s = np.float32([0, 0, 0, -0.5, -0.5, -0.5]) * scale
+    hi_s = np.float32([1, 1, 1, 0.5, 0.5, 0.5]) * scale
+    x0 = rng.uniform(lo_s, hi_s, (n, 6)).astype(np.float32)
+    x1 = rng.uniform(lo_s, hi_s, (n, 6)).astype(np.float32)
+    tau = np.exp(rng.uniform(np.log(0.01), np.log(5.0), n)).astype(np.float32)
+    W = rng.integers(2, 65, n)
+    lo, hi = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+    lo_i, hi_i = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    for i in range(n):
+        P.lib.pump_host_edge_bbox(P.fp(x0[i]), P.fp(x1[i]), tau[i], P.fp(lo_i), P.fp(hi_i))
+        lo[i], hi[i] = lo_i, hi_i
+    assert np.all(lo <= x0[:, :3]) and np.all(x0[:, :3] <= hi)
+    assert np.all(lo <= x1[:, :3]) and np.all(x1[:, :3] <= hi)
+    rows = np.repeat(np.arange(n), W - 2)
+    k = np.concatenate([np.arange(1, w - 1) for w in W]).astype(np.float32)
+    dt = (tau / (W - 1).astype(np.float32)).astype(np.float32)
+    t = (dt[rows] * k).astype(np.float32)
+    pts = _path_point_ieee(t, tau[rows], x0[rows], x1[rows])[:, :3]
+    assert np.all(lo[rows] <= pts) and np.all(pts <= hi[rows])
+    # the box is the control-point hull plus a 1e-5 relative margin, not more
+    S = (np.abs(x0[:, :3]) + np.abs(x1[:, :3]) + (np.abs(x0[:, 3:]) + np.abs(x1[:, 3:]))
+         * tau[:, None])
+    assert np.all(hi - lo <= np.abs(x1[:, :3] - x0[:, :3]) + S + 3e-5 * (1 + S))
+
+
 def test_fp64_division_matches_ieee():
     """The kernels' FP64 division (correctly rounded reciprocal, then two
     residual steps with exact FMA) equals IEEE a/b: random operands across
@@ -224,6 +257,7 @@ def test_fp64_division_matches_ieee():
     a[sl] = q[sl] * b[sl]
     half = slice(n // 2, 3 * n // 4)
     a[half] = (q[half] + np.spacing(q[half]) / 2) * b[half]
+    a[3 * n // 4: 3 * n // 4 + 2] = [0.0, -0.0]
     a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
     out = np.zeros(n)
     P.lib.pump_host_div_rn(a.ctypes.data, b.ctypes.data, out.ctypes.data, n)
@@ -233,3 +267,32 @@ def test_fp64_division_matches_ieee():
     r = 1.0 / b
     q0 = a * r
     assert not np.array_equal(q0, ref)
+
+
+def test_fp64_division_by_a_float_square_matches_ieee():
+    """The one-step division used for the tau^2 divisors of pathPoint equals
+    IEEE a/b for b = tau*tau with tau a float: random quotients, quotients
+    one bit above and below a rounding midpoint, exact quotients, taus with
+    all-ones significands and the magnitudes of pathPoint's operands."""
+    rng = np.random.default_rng(11)
+    n = 2_000_000
+    tau = (rng.uniform(1, 2, n) * 2.0 ** rng.integers(-8, 5, n)).astype(np.float32)
+    tau[: n // 10] = np.ldexp(np.float32(2 - 2 ** -23), rng.integers(-8, 5, n // 10)).astype(
+        np.float32)
+    b = tau.astype(np.float64) ** 2
+    q = rng.uniform(1, 2, n) * 2.0 ** rng.integers(-30, 12, n) * rng.choice([-1.0, 1.0], n)
+    a = q * b
+    sl = slice(n // 4, n // 2)          # a / b just off a midpoint of q's binade
+    mid = q[sl] + np.spacing(q[sl]) / 2
+    a[sl] = mid * b[sl]
+    a[n // 2: 5 * n // 8] = np.nextafter(a[n // 2: 5 * n // 8], np.inf)
+    ex = slice(5 * n // 8, 3 * n // 4)  # exact quotients with short significands, signed zeros
+    a[ex] = np.round(q[ex] * 2.0 ** 20) * 2.0 ** -20 * b[ex]
+    assert np.any(a[ex] == 0) and np.any(np.signbit(a[ex]) & (a[ex] == 0))
+    a = np.ascontiguousarray(a)
+    out = np.zeros(n)
+    P.lib.pump_host_div_sq(a.ctypes.data, np.ascontiguousarray(tau).ctypes.data,
+                           out.ctypes.data, n)
+    ref = a / b
+    assert np.array_equal(out.view(np.uint64), ref.view(np.uint64))
+    assert not np.array_equal(a * (1.0 / b), ref)
