@@ -26,13 +26,21 @@ constexpr int kTile = 256;
 
 constexpr int kEdgeObs = 256;                  // boxes staged in LDS by the edge pass
 constexpr int kEdgeMaskWords = kEdgeObs / 64;  // candidate-mask words per edge
-constexpr int kMapCap = 3072;                  // block waypoints mapped to their edge in LDS
+constexpr int kMapCap = 2016;                  // block waypoints mapped to their edge in LDS
 
 // LDS of an edge-pass workgroup, sized at launch: the per-edge arrays for
 // EPB edges, candidate masks of words_cap words per edge and obs_cap staged
-// boxes (so a workload with few boxes keeps more workgroups resident).
+// boxes (so a workload with few boxes keeps more workgroups resident).  The
+// arrays are kept small: at 128 edges and a few dozen boxes seven
+// workgroups fit the 160 KiB of a CU.
 struct EdgeLds {
-    int tps, polys, emask, x0, x1, dts, loff, ok, obs, wmap, bytes;
+    int tps, taus, polys, emask, x0, x1, dts, loff, ok, obs, wmap, bytes;
+};
+
+// tau^3 and the reciprocals of tau^2, tau^3 (tau itself is stored apart, so
+// that the doubles pack without padding)
+struct TauLds {
+    double t3, r2, r3;
 };
 
 __host__ __device__ inline EdgeLds edge_lds(int epb, int obs_cap, int words_cap)
@@ -45,14 +53,15 @@ __host__ __device__ inline EdgeLds edge_lds(int epb, int obs_cap, int words_cap)
         o += bytes;
         return at;
     };
-    L.tps = take(epb * (int)sizeof(TauPowers), 8);
+    L.tps = take(epb * (int)sizeof(TauLds), 8);
+    L.taus = take(epb * 4, 4);
     L.polys = take(epb * (int)sizeof(EdgePoly), 8);
     L.emask = take(epb * words_cap * 8, 8);
     L.x0 = take(epb * kDim * 4, 4);
     L.x1 = take(epb * kDim * 4, 4);
     L.dts = take(epb * 4, 4);
     L.loff = take((epb + 1) * 4, 4);
-    L.ok = take(epb * 4, 4);
+    L.ok = take(epb, 4);
     L.obs = take(obs_cap * 6 * 4, 4);
     L.wmap = take(kMapCap, 4);
     L.bytes = (o + 15) / 16 * 16;
@@ -102,14 +111,15 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
     uint8_t *__restrict__ is_free = J.is_free;
     constexpr int kWaves = kTile / 64;
     const EdgeLds L = edge_lds(EPB, obs_cap, words_cap);
-    TauPowers *tps = reinterpret_cast<TauPowers *>(smem + L.tps);   // powers, reciprocals
+    TauLds *tps = reinterpret_cast<TauLds *>(smem + L.tps);         // powers, reciprocals
+    float *taus = reinterpret_cast<float *>(smem + L.taus);         // arrival times
     EdgePoly *polys = reinterpret_cast<EdgePoly *>(smem + L.polys); // pathPoint constants
     unsigned long long *emask = reinterpret_cast<unsigned long long *>(smem + L.emask);
-    float(*x0s)[kDim] = reinterpret_cast<float(*)[kDim]>(smem + L.x0);
+    float(*x0s)[kDim] = reinterpret_cast<float(*)[kDim]>(smem + L.x0); // end states
     float(*x1s)[kDim] = reinterpret_cast<float(*)[kDim]>(smem + L.x1);
     float *dts = reinterpret_cast<float *>(smem + L.dts);   // waypoint spacing
     int *loff = reinterpret_cast<int *>(smem + L.loff);      // first waypoint (block-local)
-    int *ok = reinterpret_cast<int *>(smem + L.ok);          // flag
+    uint8_t *ok = reinterpret_cast<uint8_t *>(smem + L.ok);  // flag
     float *olo = reinterpret_cast<float *>(smem + L.obs);    // staged boxes: lo[d][b], hi[d][b]
     float *ohi = olo + 3 * obs_cap;
     uint8_t *wmap = reinterpret_cast<uint8_t *>(smem + L.wmap); // edge of each local waypoint
@@ -120,25 +130,29 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
     const int n_masked = edge_obs_cap(n_obs); // staged boxes with candidate masks
     const int n_words = edge_words(n_masked);
     const int64_t g_begin = wp_offsets ? wp_offsets[e0] : e0 * (int64_t)J.w_fixed;
+    float *__restrict__ disc = J.disc ? J.disc + g_begin * kDim : nullptr;
     // kTile / EPB lanes per edge: lane t serves edge q = t % EPB as part
-    // t / EPB.  Every global read of the set-up is issued before the first
-    // barrier: the edge's arrival time, and each endpoint's state gathered
-    // once (part 0 the source, part 1 the destination; with one part, both).
+    // t / EPB; each part that needs them gathers both end states into
+    // registers.  All global reads of the set-up are issued before the first
+    // barrier.
     constexpr int kParts = kTile / EPB;
     const int q = t % EPB;
     const int part = __builtin_amdgcn_readfirstlane(t / EPB); // uniform within a wave
     const bool own = q < ne;
-    const float tau = own ? topts_edge[e0 + q] : 0.0f;
-    if (own && part < 2) {
-        const int node = (part == 0 ? edge_src : edge_dst)[e0 + q];
-        float(*xs)[kDim] = part == 0 ? x0s : x1s;
-        for (int d = 0; d < kDim; ++d)
-            xs[q][d] = samples[(int64_t)node * kDim + d];
-    }
-    if (kParts == 1 && own) {
-        const int node = edge_dst[e0 + q];
-        for (int d = 0; d < kDim; ++d)
-            x1s[q][d] = samples[(int64_t)node * kDim + d];
+    float tau = 1.0f;
+    float x0[kDim] = {0, 0, 0, 0, 0, 0}, x1[kDim] = {0, 0, 0, 0, 0, 0};
+    if (own && (kCheck || part == 0)) {
+        tau = topts_edge[e0 + q];
+        const int src = edge_src[e0 + q], dst = edge_dst[e0 + q];
+        for (int d = 0; d < kDim; ++d) {
+            x0[d] = samples[(int64_t)src * kDim + d];
+            x1[d] = samples[(int64_t)dst * kDim + d];
+        }
+        if (part == 0)
+            for (int d = 0; d < kDim; ++d) {
+                x0s[q][d] = x0[d];
+                x1s[q][d] = x1[d];
+            }
     }
     if (kCheck)
         for (int i = t; i < n_masked * 3; i += kTile) {
@@ -155,13 +169,14 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
     __syncthreads();
     const int n_loc = loff[ne];
     const bool mapped = n_loc <= kMapCap;
-    // Part 0 forms the edge's constants; the other parts (or the only one)
+    // Part 0 forms the edge's constants; the parts past 0 (or the only one)
     // set the edge's candidate bits over the staged boxes.
+    const int a = own ? loff[q] : 0, W = own ? loff[q + 1] - a : 0;
     if (own && part == 0) {
         const TauPowers P = tau_powers(tau);
-        tps[q] = P;
-        polys[q] = edge_poly(x0s[q], x1s[q], P);
-        const int a = loff[q], W = loff[q + 1] - a;
+        tps[q] = TauLds{P.t3, P.r2, P.r3};
+        taus[q] = tau;
+        polys[q] = edge_poly(x0, x1, P);
         dts[q] = tau / (float)(W - 1);
         ok[q] = 1;
         if (mapped)
@@ -173,7 +188,7 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
     if (kCheck && n_words > 0 && mpart >= 0) {
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         if (own)
-            edge_bbox(x0s[q], x1s[q], tau, lo, hi);
+            edge_bbox(x0, x1, tau, lo, hi);
         // The wave's union of its edges' boxes: a box separated from the
         // union (the same <= test) is separated from each of them, so only
         // the boxes that meet the union are tested edge by edge, with their
@@ -220,18 +235,19 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
     }
     __syncthreads();
 
-    float *__restrict__ disc = J.disc ? J.disc + g_begin * kDim : nullptr;
     // tiles start every 63 waypoints; tile j belongs to wave j % kWaves
     for (int base = wid * 63; base < n_loc; base += kWaves * 63) {
         const int g = base + lane;
         const bool have = g < n_loc;
-        int le = 0, k = 0, W = 0;
+        int le = 0, k = 0, Wq = 0;
         float x[kDim] = {0, 0, 0, 0, 0, 0};
         if (have) {
             le = mapped ? (int)wmap[g] : find_edge_local(loff, ne, g);
             k = g - loff[le];
-            W = loff[le + 1] - loff[le];
-            waypoint_dt(k, W, tps[le], polys[le], dts[le], x0s[le], x1s[le], x);
+            Wq = loff[le + 1] - loff[le];
+            const TauLds T = tps[le];
+            const TauPowers P{taus[le], T.t3, T.r2, T.r3};
+            waypoint_dt(k, Wq, P, polys[le], dts[le], x0s[le], x1s[le], x);
             // lane 63's waypoint is lane 0's of the next tile
             if (kWrite && (lane < 63 || g + 1 >= n_loc))
                 for (int d = 0; d < kDim; ++d)
@@ -243,7 +259,7 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
         float w[3];
         for (int d = 0; d < 3; ++d)
             w[d] = dpp_move<0x130>(x[d]); // from lane + 1
-        const bool test = have && lane < 63 && k < W - 1 && ok[le];
+        const bool test = have && lane < 63 && k < Wq - 1 && ok[le];
         if (test) {
             const float v[3] = {x[0], x[1], x[2]};
             float bmin[3], bmax[3];
@@ -274,8 +290,8 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
     }
     if (kCheck) {
         __syncthreads();
-        for (int q = t; q < ne; q += kTile)
-            is_free[e0 + q] = (uint8_t)ok[q];
+        for (int i = t; i < ne; i += kTile)
+            is_free[e0 + i] = ok[i];
     }
 }
 

@@ -152,17 +152,17 @@ __host__ __device__ __forceinline__ float topt_bisection(const CostTerms &c, flo
 // ---------------------------------------------------------------------------
 struct TauPowers {
     float tau;
-    double t2, t3; // tau^2 (exact), tau^3
-    double r2, r3; // correctly rounded 1/t2, 1/t3
+    double t3;     // tau^3 (tau^2 is exact and formed where it is used)
+    double r2, r3; // correctly rounded 1/tau^2, 1/tau^3
 };
 
 __host__ __device__ __forceinline__ TauPowers tau_powers(float tau)
 {
     TauPowers p;
+    const double t2 = (double)tau * (double)tau;
     p.tau = tau;
-    p.t2 = (double)tau * (double)tau;
-    p.t3 = p.t2 * (double)tau;
-    p.r2 = 1.0 / p.t2;
+    p.t3 = t2 * (double)tau;
+    p.r2 = 1.0 / t2;
     p.r3 = 1.0 / p.t3;
     return p;
 }
@@ -208,12 +208,14 @@ __host__ __device__ __forceinline__ double div_sq(double a, double b, double r)
 // that depends only on (x0, x1, tau), formed with the same operations in the
 // same order as the full expression, so a waypoint evaluated from them is
 // bit-identical to the expression evaluated from scratch.  Per waypoint only
-// the terms in t remain (about 100 FP64 operations instead of 165).
+// the terms in t remain (about 100 FP64 operations instead of 165).  The
+// products z0 tau^3 and vz0 tau^3 are one multiplication each and are
+// formed where they are used.
 struct EdgePoly {
-    double c3[2], c2[2], cv[2]; // x, y: cubic and quadratic position terms, velocity term
-    double bz, kz, dz;          // z position: z0 tau^3, bracket of s^2 tau, bracket of s^3
-    double av, bv, cz;          // z velocity: vz0 tau^3, bracket of t tau, bracket of s^2
-    float e[2];                 // x, y: velocity term of -(2 t e)
+    float c3[2], c2[2], cv[2]; // x, y: cubic and quadratic position terms, velocity term
+    float e[2];                // x, y: velocity term of -(2 t e)
+    double kz, dz;             // z position: bracket of s^2 tau, bracket of s^3
+    double bv, cz;             // z velocity: bracket of t tau, bracket of s^2
 };
 
 __host__ __device__ __forceinline__ EdgePoly edge_poly(const float *x0, const float *x1,
@@ -226,50 +228,51 @@ __host__ __device__ __forceinline__ EdgePoly edge_poly(const float *x0, const fl
         const float c3 = 2.0f * p0 - 2.0f * p1 + (v0 + v1) * tau;
         const float c2 = 3.0f * p0 - 3.0f * p1 + (2.0f * v0 + v1) * tau;
         const float c = 2.0f * p0 - 2.0f * p1 + v1 * tau;
-        E.c3[d] = (double)c3;
-        E.c2[d] = (double)c2;
-        E.cv[d] = (double)c;
+        E.c3[d] = c3;
+        E.c2[d] = c2;
+        E.cv[d] = c;
         E.e[d] = 3.0f * p0 - 3.0f * p1 + v1 * tau;
     }
     const double z0 = (double)x0[2], z1 = (double)x1[2];
     const double vz0 = (double)x0[5], vz1 = (double)x1[5], td = (double)tau;
-    E.bz = z0 * P.t3;
     E.kz = -3.0 * z0 + 3.0 * z1 - 2.0 * vz0 * td - 1.0 * vz1 * td;
     E.dz = 2.0 * z0 - 2.0 * z1 + (double)((x0[5] + x1[5]) * tau);
-    E.av = vz0 * P.t3;
     E.bv = -6.0 * z0 + 6.0 * z1 - 4.0 * vz0 * td - 2.0 * vz1 * td;
     E.cz = 6.0 * z0 - 6.0 * z1 + 3.0 * vz0 * td + 3.0 * vz1 * td;
     return E;
 }
 
 // pathPoint at time t from the edge's constants; x0 supplies p0, v0 of x, y
-// and the z velocity
+// and z0, vz0
 __host__ __device__ __forceinline__ void path_point(float t, const TauPowers &P, const EdgePoly &E,
                                                     const float *x0, float *x)
 {
     const float tau = P.tau;
+    const double t2 = (double)tau * (double)tau; // exact
     const double s2 = (double)t * (double)t;
     const double s3 = s2 * (double)t;
-    const double q = 3.0 * s2 - (double)(4.0f * t * tau) + P.t2;
+    const double q = 3.0 * s2 - (double)(4.0f * t * tau) + t2;
     for (int d = 0; d < 2; ++d) {
         const float p0 = x0[d], v0 = x0[d + 3];
         const float lin = t * v0 + p0;
-        x[d] = (float)((double)lin + div_rn(s3 * E.c3[d], P.t3, P.r3) -
-                       div_sq(s2 * E.c2[d], P.t2, P.r2));
+        x[d] = (float)((double)lin + div_rn(s3 * (double)E.c3[d], P.t3, P.r3) -
+                       div_sq(s2 * (double)E.c2[d], t2, P.r2));
         const float f = -(2.0f * t * E.e[d]);
-        x[d + 3] = (float)(div_sq((double)f, P.t2, P.r2) + div_rn(3.0 * s2 * E.cv[d], P.t3, P.r3) +
-                           div_sq((double)v0 * q, P.t2, P.r2));
+        x[d + 3] = (float)(div_sq((double)f, t2, P.r2) + div_rn(3.0 * s2 * (double)E.cv[d], P.t3, P.r3) +
+                           div_sq((double)v0 * q, t2, P.r2));
     }
     {
         const double a = (double)(t * x0[5]) * P.t3;
         const double c = s2 * (double)tau * E.kz;
         const double d = s3 * E.dz;
-        x[2] = (float)div_rn(a + E.bz + c + d, P.t3, P.r3);
+        const double bz = (double)x0[2] * P.t3;
+        x[2] = (float)div_rn(a + bz + c + d, P.t3, P.r3);
     }
     {
         const double b = (double)(t * tau) * E.bv;
         const double c = s2 * E.cz;
-        x[5] = (float)div_rn(E.av + b + c, P.t3, P.r3);
+        const double av = (double)x0[5] * P.t3;
+        x[5] = (float)div_rn(av + b + c, P.t3, P.r3);
     }
 }
 
