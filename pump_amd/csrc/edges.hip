@@ -28,19 +28,28 @@ constexpr int kEdgeObs = 256;                  // boxes staged in LDS by the edg
 constexpr int kEdgeMaskWords = kEdgeObs / 64;  // candidate-mask words per edge
 constexpr int kMapCap = 2016;                  // block waypoints mapped to their edge in LDS
 
-// LDS of an edge-pass workgroup, sized at launch: the per-edge arrays for
-// EPB edges, candidate masks of words_cap words per edge and obs_cap staged
-// boxes (so a workload with few boxes keeps more workgroups resident).  The
-// arrays are kept small: at 128 edges and a few dozen boxes seven
-// workgroups fit the 160 KiB of a CU.
-struct EdgeLds {
-    int tps, taus, polys, emask, x0, x1, dts, loff, ok, obs, wmap, bytes;
+// Per-edge record of an edge-pass workgroup in LDS: everything a lane needs
+// to evaluate any waypoint of the edge, read from one base address.
+struct alignas(16) EdgeRec {
+    EdgePoly E;         // pathPoint constants
+    double t3, r2, r3;  // tau^3, RN(1/tau^2), RN(1/tau^3)
+    float tau, dt;      // arrival time, waypoint spacing
+    float x0[kDim];     // start state
+    float x1[kDim];     // end state
+};
+static_assert(sizeof(EdgeRec) == 144, "edge record layout");
+
+// A staged box: lower corner, upper corner (positions).
+struct alignas(8) BoxRec {
+    float lo[3], hi[3];
 };
 
-// tau^3 and the reciprocals of tau^2, tau^3 (tau itself is stored apart, so
-// that the doubles pack without padding)
-struct TauLds {
-    double t3, r2, r3;
+// LDS of an edge-pass workgroup, sized at launch: EPB edge records,
+// candidate masks of words_cap words per edge and obs_cap staged boxes (so a
+// workload with few boxes keeps more workgroups resident).  At 128 edges and
+// a few dozen boxes seven workgroups fit the 160 KiB of a CU.
+struct EdgeLds {
+    int recs, emask, loff, ok, obs, wmap, bytes;
 };
 
 __host__ __device__ inline EdgeLds edge_lds(int epb, int obs_cap, int words_cap)
@@ -53,16 +62,11 @@ __host__ __device__ inline EdgeLds edge_lds(int epb, int obs_cap, int words_cap)
         o += bytes;
         return at;
     };
-    L.tps = take(epb * (int)sizeof(TauLds), 8);
-    L.taus = take(epb * 4, 4);
-    L.polys = take(epb * (int)sizeof(EdgePoly), 8);
+    L.recs = take(epb * (int)sizeof(EdgeRec), 16);
+    L.obs = take(obs_cap * (int)sizeof(BoxRec), 8);
     L.emask = take(epb * words_cap * 8, 8);
-    L.x0 = take(epb * kDim * 4, 4);
-    L.x1 = take(epb * kDim * 4, 4);
-    L.dts = take(epb * 4, 4);
     L.loff = take((epb + 1) * 4, 4);
     L.ok = take(epb, 4);
-    L.obs = take(obs_cap * 6 * 4, 4);
     L.wmap = take(kMapCap, 4);
     L.bytes = (o + 15) / 16 * 16;
     return L;
@@ -111,18 +115,12 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
     uint8_t *__restrict__ is_free = J.is_free;
     constexpr int kWaves = kTile / 64;
     const EdgeLds L = edge_lds(EPB, obs_cap, words_cap);
-    TauLds *tps = reinterpret_cast<TauLds *>(smem + L.tps);         // powers, reciprocals
-    float *taus = reinterpret_cast<float *>(smem + L.taus);         // arrival times
-    EdgePoly *polys = reinterpret_cast<EdgePoly *>(smem + L.polys); // pathPoint constants
+    EdgeRec *recs = reinterpret_cast<EdgeRec *>(smem + L.recs);   // per-edge records
+    BoxRec *boxes = reinterpret_cast<BoxRec *>(smem + L.obs);     // staged boxes
     unsigned long long *emask = reinterpret_cast<unsigned long long *>(smem + L.emask);
-    float(*x0s)[kDim] = reinterpret_cast<float(*)[kDim]>(smem + L.x0); // end states
-    float(*x1s)[kDim] = reinterpret_cast<float(*)[kDim]>(smem + L.x1);
-    float *dts = reinterpret_cast<float *>(smem + L.dts);   // waypoint spacing
-    int *loff = reinterpret_cast<int *>(smem + L.loff);      // first waypoint (block-local)
-    uint8_t *ok = reinterpret_cast<uint8_t *>(smem + L.ok);  // flag
-    float *olo = reinterpret_cast<float *>(smem + L.obs);    // staged boxes: lo[d][b], hi[d][b]
-    float *ohi = olo + 3 * obs_cap;
-    uint8_t *wmap = reinterpret_cast<uint8_t *>(smem + L.wmap); // edge of each local waypoint
+    int *loff = reinterpret_cast<int *>(smem + L.loff);           // first waypoint (block-local)
+    uint8_t *ok = reinterpret_cast<uint8_t *>(smem + L.ok);       // flag
+    uint8_t *wmap = reinterpret_cast<uint8_t *>(smem + L.wmap);   // edge of each local waypoint
 
     const int ne = (int)((J.n_edges - e0) < EPB ? (J.n_edges - e0) : EPB);
     const int t = threadIdx.x, lane = t & 63;
@@ -150,15 +148,15 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
         }
         if (part == 0)
             for (int d = 0; d < kDim; ++d) {
-                x0s[q][d] = x0[d];
-                x1s[q][d] = x1[d];
+                recs[q].x0[d] = x0[d];
+                recs[q].x1[d] = x1[d];
             }
     }
     if (kCheck)
         for (int i = t; i < n_masked * 3; i += kTile) {
             const int b = i / 3, d = i - 3 * (i / 3);
-            olo[d * obs_cap + b] = obstacles[b * 12 + d];
-            ohi[d * obs_cap + b] = obstacles[b * 12 + 6 + d];
+            boxes[b].lo[d] = obstacles[b * 12 + d];
+            boxes[b].hi[d] = obstacles[b * 12 + 6 + d];
         }
     for (int i = t; i <= ne; i += kTile)
         loff[i] = (int)((wp_offsets ? wp_offsets[e0 + i] : (e0 + i) * (int64_t)J.w_fixed) -
@@ -174,10 +172,13 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
     const int a = own ? loff[q] : 0, W = own ? loff[q + 1] - a : 0;
     if (own && part == 0) {
         const TauPowers P = tau_powers(tau);
-        tps[q] = TauLds{P.t3, P.r2, P.r3};
-        taus[q] = tau;
-        polys[q] = edge_poly(x0, x1, P);
-        dts[q] = tau / (float)(W - 1);
+        EdgeRec &R = recs[q];
+        R.E = edge_poly(x0, x1, P);
+        R.t3 = P.t3;
+        R.r2 = P.r2;
+        R.r3 = P.r3;
+        R.tau = tau;
+        R.dt = tau / (float)(W - 1);
         ok[q] = 1;
         if (mapped)
             for (int j = 0; j < W; ++j)
@@ -207,8 +208,8 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
             bool meets = false;
             if (bl < n_masked) {
                 for (int d = 0; d < 3; ++d) {
-                    bx[d] = olo[d * obs_cap + bl];
-                    bx[3 + d] = ohi[d * obs_cap + bl];
+                    bx[d] = boxes[bl].lo[d];
+                    bx[3 + d] = boxes[bl].hi[d];
                 }
                 meets = !((uhi[0] <= bx[0]) | (bx[3] <= ulo[0]) | (uhi[1] <= bx[1]) |
                           (bx[4] <= ulo[1]) | (uhi[2] <= bx[2]) | (bx[5] <= ulo[2]));
@@ -243,11 +244,12 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
         float x[kDim] = {0, 0, 0, 0, 0, 0};
         if (have) {
             le = mapped ? (int)wmap[g] : find_edge_local(loff, ne, g);
-            k = g - loff[le];
-            Wq = loff[le + 1] - loff[le];
-            const TauLds T = tps[le];
-            const TauPowers P{taus[le], T.t3, T.r2, T.r3};
-            waypoint_dt(k, Wq, P, polys[le], dts[le], x0s[le], x1s[le], x);
+            const EdgeRec &R = recs[le];
+            const int a0 = loff[le];
+            k = g - a0;
+            Wq = loff[le + 1] - a0;
+            const TauPowers P{R.tau, R.t3, R.r2, R.r3};
+            waypoint_dt(k, Wq, P, R.E, R.dt, R.x0, R.x1, x);
             // lane 63's waypoint is lane 0's of the next tile
             if (kWrite && (lane < 63 || g + 1 >= n_loc))
                 for (int d = 0; d < kDim; ++d)
@@ -270,9 +272,8 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
                 while (m && !hit) {
                     const int b = wd * 64 + __builtin_ctzll(m);
                     m &= m - 1;
-                    const float lo[3] = {olo[b], olo[obs_cap + b], olo[2 * obs_cap + b]};
-                    const float hi[3] = {ohi[b], ohi[obs_cap + b], ohi[2 * obs_cap + b]};
-                    hit = segment_hits_box(v, w, bmin, bmax, lo, hi);
+                    const BoxRec B = boxes[b];
+                    hit = segment_hits_box(v, w, bmin, bmax, B.lo, B.hi);
                 }
             }
             // boxes past the staged ones: a uniform loop, so that their
