@@ -216,15 +216,22 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
             }
             unsigned long long cand = __ballot(meets) & pat;
             unsigned long long m = 0ull;
+            // four candidates per step (the last step repeats its first
+            // index where fewer remain), so that their LDS reads and tests
+            // overlap
             while (cand) {
-                const int i = __builtin_ctzll(cand);
-                cand &= cand - 1;
-                float c[6];
-                for (int d = 0; d < 6; ++d)
-                    c[d] = read_lane(bx[d], i);
-                const bool sep = (hi[0] <= c[0]) | (c[3] <= lo[0]) | (hi[1] <= c[1]) |
-                                 (c[4] <= lo[1]) | (hi[2] <= c[2]) | (c[5] <= lo[2]);
-                m |= (unsigned long long)(!sep) << i;
+                int idx[4];
+                for (int u = 0; u < 4; ++u) {
+                    idx[u] = cand ? __builtin_ctzll(cand) : idx[0];
+                    cand &= cand - 1;
+                }
+                for (int u = 0; u < 4; ++u) {
+                    const BoxRec B = boxes[wd * 64 + idx[u]]; // the same address in every lane
+                    const float c[6] = {B.lo[0], B.lo[1], B.lo[2], B.hi[0], B.hi[1], B.hi[2]};
+                    const bool sep = (hi[0] <= c[0]) | (c[3] <= lo[0]) | (hi[1] <= c[1]) |
+                                     (c[4] <= lo[1]) | (hi[2] <= c[2]) | (c[5] <= lo[2]);
+                    m |= (unsigned long long)(!sep) << idx[u];
+                }
             }
             if (own) {
                 if (kMaskParts == 1)
