@@ -14,6 +14,9 @@ Step (timed K times, after W warmups): one edge pass -- kernel (b), waypoint
 generation fused with the segment/AABB test -- over every candidate edge.
     value = edge-checks/s = E / (seconds per step)
 Also measured in the same run, each over its own timed repetitions:
+    * the same edge pass through the reference-shaped pair of calls
+      (pump_discretize_edges, then pump_free_edges over the stored
+      waypoints), with its flags compared to the fused pass's;
     * kernel (c): one HSMC propagate + CP pass in which every free edge is
       expanded once from a synthetic parent path (particle-steps/s), in the
       byte-validity form (plain FP32, the reference layout) and the bit-mask
@@ -119,6 +122,30 @@ def main():
     log(f"edge pass: {ms_step:.3f} ms/step, {value:.3e} edge-checks/s, "
         f"{achieved / 1e9:.1f} GB/s algorithmic, wall {wall:.3f} s for {args.steps} steps")
 
+    # the same pass through the reference-shaped pair of calls (discretize
+    # every edge, then freeEdges over the stored waypoints): the plain path
+    fused_flags = rm.flags()["is_free_edges"]
+
+    def plain_pass():
+        P.check("pump_discretize_edges", P.lib.pump_discretize_edges(
+            d.samples, d.edge_src, d.edge_dst, d.topts_edge, E, d.wp_offsets, 0, d.disc,
+            stream.h))
+        P.check("pump_free_edges", P.lib.pump_free_edges(
+            d.obstacles, n_obs, d.samples, d.is_free_samples, 0, d.wp_offsets, d.disc,
+            d.is_free_edges, E, stream.h))
+
+    plain_pass()
+    stream.sync()
+    timer.start()
+    for _ in range(args.steps):
+        plain_pass()
+    ms_plain = timer.stop() / args.steps
+    plain = {"ms": ms_plain, "edge_checks_per_s": E / (ms_plain / 1e3),
+             "fused_vs_plain": ms_plain / ms_step,
+             "flags_equal": bool(np.array_equal(rm.flags()["is_free_edges"], fused_flags))}
+    log(f"plain edge pass (discretize + freeEdges): {ms_plain:.3f} ms, "
+        f"fused is {plain['fused_vs_plain']:.2f}x, flags equal {plain['flags_equal']}")
+
     # a plain device copy of known size: calibrates the HBM byte counters of a
     # profiled run (FETCH_SIZE / WRITE_SIZE of the copy vs its 2 x 256 MiB)
     calib = P.DeviceArray(shape=(2, 256 << 20), dtype=np.uint8, fill=7)
@@ -177,6 +204,7 @@ def main():
                      "kernel": "k_edges<true,true,128> (pump_edges_discretize_check)",
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "edges_plain": plain,
         "hsmc": hsmc,
         "pairs": pairs,
         "search": search,

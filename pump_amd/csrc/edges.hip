@@ -130,28 +130,19 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
     const int64_t g_begin = wp_offsets ? wp_offsets[e0] : e0 * (int64_t)J.w_fixed;
     float *__restrict__ disc = J.disc ? J.disc + g_begin * kDim : nullptr;
     // kTile / EPB lanes per edge: lane t serves edge q = t % EPB as part
-    // t / EPB; each part that needs them gathers both end states into
-    // registers.  All global reads of the set-up are issued before the first
-    // barrier.
+    // t / EPB.  Part 0 forms the edge's record (constants, end states), its
+    // flag and its entries of the waypoint map; the mask part (part 1, or
+    // the only one) its candidate boxes.  Both work from their own global
+    // reads, so the whole set-up needs a single barrier.
     constexpr int kParts = kTile / EPB;
+    constexpr int kMaskPart = kParts == 1 ? 0 : 1;
     const int q = t % EPB;
     const int part = __builtin_amdgcn_readfirstlane(t / EPB); // uniform within a wave
     const bool own = q < ne;
-    float tau = 1.0f;
-    float x0[kDim] = {0, 0, 0, 0, 0, 0}, x1[kDim] = {0, 0, 0, 0, 0, 0};
-    if (own && (kCheck || part == 0)) {
-        tau = topts_edge[e0 + q];
-        const int src = edge_src[e0 + q], dst = edge_dst[e0 + q];
-        for (int d = 0; d < kDim; ++d) {
-            x0[d] = samples[(int64_t)src * kDim + d];
-            x1[d] = samples[(int64_t)dst * kDim + d];
-        }
-        if (part == 0)
-            for (int d = 0; d < kDim; ++d) {
-                recs[q].x0[d] = x0[d];
-                recs[q].x1[d] = x1[d];
-            }
-    }
+    const int64_t W_fixed = J.w_fixed;
+    const int n_loc =
+        (int)((wp_offsets ? wp_offsets[e0 + ne] : (e0 + ne) * W_fixed) - g_begin);
+    const bool mapped = n_loc <= kMapCap;
     if (kCheck)
         for (int i = t; i < n_masked * 3; i += kTile) {
             const int b = i / 3, d = i - 3 * (i / 3);
@@ -159,17 +150,24 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
             boxes[b].hi[d] = obstacles[b * 12 + 6 + d];
         }
     for (int i = t; i <= ne; i += kTile)
-        loff[i] = (int)((wp_offsets ? wp_offsets[e0 + i] : (e0 + i) * (int64_t)J.w_fixed) -
-                        g_begin);
-    if (kCheck)
-        for (int i = t; i < ne * n_words; i += kTile)
-            emask[i] = 0ull;
-    __syncthreads();
-    const int n_loc = loff[ne];
-    const bool mapped = n_loc <= kMapCap;
-    // Part 0 forms the edge's constants; the parts past 0 (or the only one)
-    // set the edge's candidate bits over the staged boxes.
-    const int a = own ? loff[q] : 0, W = own ? loff[q + 1] - a : 0;
+        loff[i] = (int)((wp_offsets ? wp_offsets[e0 + i] : (e0 + i) * W_fixed) - g_begin);
+    const bool works = part == 0 || (kCheck && part == kMaskPart);
+    float tau = 1.0f;
+    float x0[kDim] = {0, 0, 0, 0, 0, 0}, x1[kDim] = {0, 0, 0, 0, 0, 0};
+    int a = 0, W = 0;
+    if (own && works) {
+        const int64_t e = e0 + q;
+        tau = topts_edge[e];
+        const int src = edge_src[e], dst = edge_dst[e];
+        const int64_t ga = wp_offsets ? wp_offsets[e] : e * W_fixed;
+        const int64_t gb = wp_offsets ? wp_offsets[e + 1] : (e + 1) * W_fixed;
+        a = (int)(ga - g_begin);
+        W = (int)(gb - ga);
+        for (int d = 0; d < kDim; ++d) {
+            x0[d] = samples[(int64_t)src * kDim + d];
+            x1[d] = samples[(int64_t)dst * kDim + d];
+        }
+    }
     if (own && part == 0) {
         const TauPowers P = tau_powers(tau);
         EdgeRec &R = recs[q];
@@ -179,14 +177,16 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
         R.r3 = P.r3;
         R.tau = tau;
         R.dt = tau / (float)(W - 1);
+        for (int d = 0; d < kDim; ++d) {
+            R.x0[d] = x0[d];
+            R.x1[d] = x1[d];
+        }
         ok[q] = 1;
         if (mapped)
             for (int j = 0; j < W; ++j)
                 wmap[a + j] = (uint8_t)q;
     }
-    constexpr int kMaskParts = kParts == 1 ? 1 : kParts - 1;
-    const int mpart = kParts == 1 ? 0 : part - 1;
-    if (kCheck && n_words > 0 && mpart >= 0) {
+    if (kCheck && n_words > 0 && part == kMaskPart) {
         float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         if (own)
             edge_bbox(x0, x1, tau, lo, hi);
@@ -199,26 +199,23 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
             ulo[d] = wave_min(lo[d]);
             uhi[d] = wave_max(hi[d]);
         }
-        unsigned long long pat = 0ull; // this part's boxes within a word
-        for (int i = mpart; i < 64; i += kMaskParts)
-            pat |= 1ull << i;
         for (int wd = 0; wd < n_words; ++wd) {
             const int bl = wd * 64 + lane;
             float bx[6] = {0, 0, 0, 0, 0, 0}; // lo x, y, z, hi x, y, z of box bl
             bool meets = false;
             if (bl < n_masked) {
                 for (int d = 0; d < 3; ++d) {
-                    bx[d] = boxes[bl].lo[d];
-                    bx[3 + d] = boxes[bl].hi[d];
+                    bx[d] = obstacles[bl * 12 + d];
+                    bx[3 + d] = obstacles[bl * 12 + 6 + d];
                 }
                 meets = !((uhi[0] <= bx[0]) | (bx[3] <= ulo[0]) | (uhi[1] <= bx[1]) |
                           (bx[4] <= ulo[1]) | (uhi[2] <= bx[2]) | (bx[5] <= ulo[2]));
             }
-            unsigned long long cand = __ballot(meets) & pat;
+            unsigned long long cand = __ballot(meets);
             unsigned long long m = 0ull;
             // four candidates per step (the last step repeats its first
-            // index where fewer remain), so that their LDS reads and tests
-            // overlap
+            // index where fewer remain), so that their cross-lane reads and
+            // tests overlap
             while (cand) {
                 int idx[4];
                 for (int u = 0; u < 4; ++u) {
@@ -226,19 +223,16 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
                     cand &= cand - 1;
                 }
                 for (int u = 0; u < 4; ++u) {
-                    const BoxRec B = boxes[wd * 64 + idx[u]]; // the same address in every lane
-                    const float c[6] = {B.lo[0], B.lo[1], B.lo[2], B.hi[0], B.hi[1], B.hi[2]};
+                    float c[6];
+                    for (int d = 0; d < 6; ++d)
+                        c[d] = read_lane(bx[d], idx[u]);
                     const bool sep = (hi[0] <= c[0]) | (c[3] <= lo[0]) | (hi[1] <= c[1]) |
                                      (c[4] <= lo[1]) | (hi[2] <= c[2]) | (c[5] <= lo[2]);
                     m |= (unsigned long long)(!sep) << idx[u];
                 }
             }
-            if (own) {
-                if (kMaskParts == 1)
-                    emask[q * n_words + wd] = m;
-                else if (m)
-                    atomicOr(&emask[q * n_words + wd], m);
-            }
+            if (own)
+                emask[q * n_words + wd] = m;
         }
     }
     __syncthreads();
