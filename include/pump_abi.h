@@ -110,7 +110,8 @@ int pump_heuristic(const float *d_samples, int num, const float *goal_host, floa
 
 /* Waypoints per edge: W_e = w_fixed if w_fixed > 0 (the reference's
  * numDisc+1, mainPUMP.cu:57), else clamp(ceil(tau_e/wp_dt), 2, w_cap).
- * Every edge entry point takes W_e < 2^23.
+ * Every edge entry point takes 2 <= W_e < 2^23 (the reference's numDisc >= 1;
+ * an edge with fewer waypoints has no defined discretisation).
  * Writes d_wp_offsets[n_edges+1] and *total_host (synchronises).
  * Workspace: pump_scan_workspace(n_edges+1). */
 int pump_waypoint_offsets(const float *d_topts_edge, int64_t n_edges, int w_fixed, float wp_dt,
