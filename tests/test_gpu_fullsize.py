@@ -12,6 +12,9 @@
   sets of the sampled rows, waypoints, flags and half-spaces of sampled edges
   against the oracle, and the returned path re-derived (chain of free edges,
   cost and time sums, full-MC CP and HSMC CP recomputed by the oracle).
+* Against the FP64 oracle (every returned path, NUM=4000 included): the same
+  node sequence sampled from FP64 arrival times in FP64, its full-MC CP with
+  the same noise draws within one MC standard error of the returned CP.
 """
 import os
 
@@ -79,6 +82,8 @@ def test_full_num4000_parity(bits):
         assert got[k] == ref[k], k
     for k in ("soln_cost", "soln_cp", "soln_time", "soln_hsmc_cp"):
         assert np.float32(got[k]) == np.float32(ref[k]), k
+    if got["soln_len"] > 0:
+        rederive_path(cfg, s, o, ot, xc, rm, got, e, f, hs)
     rm.close()
 
 
@@ -131,6 +136,28 @@ def rederive_path(cfg, s, o, ot, xc, rm, got, e, f, hs):
                                     prm.offset_mult, oracle.fp(split), int(ln[0]))
     assert np.float32(got["soln_cp"]) == np.float32(cp)
     assert (got["soln_cp"] < prm.cp_target) == (got["status"] == 0)
+    # against the FP64 oracle: the same nodes' path sampled from FP64 arrival
+    # times in FP64 and its full-MC CP with the same noise draws must lie
+    # within one MC standard error
+    split64 = np.zeros((cap, 6), np.float64)
+    rows64 = O.po64_find_optimal_path(prm.dt, oracle.dp(split64), oracle.fp(xs),
+                                      len(nodes), cap)
+    assert rows64 > 0
+    hits64 = O.po64_mccp_hits(oracle.fp(o), o.shape[0], oracle.fp(xc), T, n_mc,
+                              float(np.float32(prm.offset_mult)),
+                              oracle.dp(split64), rows64)
+    cp64 = hits64 / n_mc
+    se = max(np.sqrt(cp64 * (1 - cp64) / n_mc), 1.0 / n_mc)
+    cost64 = sum(O.po64_cost(O.po64_topt(oracle.fp(xs[k]), oracle.fp(xs[k + 1]), 5.0),
+                             oracle.fp(xs[k]), oracle.fp(xs[k + 1])) for k in range(len(nodes) - 1))
+    print(f"FP64 oracle: path CP {cp:.6f} vs {cp64:.6f} ({abs(cp - cp64) / se:.2f} MC standard "
+          f"errors), path cost rel err {abs(float(got['soln_cost']) - cost64) / cost64:.2e}",
+          flush=True)
+    assert abs(cp - cp64) <= se
+    # the reference's FP32 bisection of a finite-difference derivative stops
+    # short of the FP64 minimum (per-edge errors up to tens of percent in tau,
+    # well under that in cost); the path cost stays within a few percent
+    assert abs(float(got["soln_cost"]) - cost64) <= 0.05 * cost64
     # HSMC CP along the path: one propagate step per edge, oracle arithmetic
     n_p = prm.n_p
     wo = f["wp_offsets"]
