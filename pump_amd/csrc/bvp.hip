@@ -55,10 +55,16 @@ __global__ __launch_bounds__(256) void k_fill_pairs(const float *__restrict__ sa
 }
 
 // Bisection as a per-lane state machine so that a lane that finishes takes
-// the next pair while its wave-mates are still iterating.  Each step does
-// exactly one dcost evaluation; the sequence of evaluations per pair is the
-// one toptBisection performs, so the results are identical.
-enum : int { ST_TMAX = 0, ST_LOW = 1, ST_BIS = 2 };
+// the next pair while its wave-mates are still iterating.  Every step
+// evaluates cost at two points per lane: the two of one dcost (tau +- h/2,
+// as dcost forms them), or, in the final phase, cost(topt) for the output.
+// The sequence of dcost evaluations per pair is the one toptBisection
+// performs, so the results are identical.  Idle lanes are refilled only
+// once kRefill of them have gathered (or none is busy), so that the
+// wave-wide cost of setting up new pairs is paid once for many lanes.
+enum : int { ST_TMAX = 0, ST_LOW = 1, ST_BIS = 2, ST_FIN = 3 };
+constexpr int kRefill = 16;
+constexpr int kChunk = 2048; // pairs per claim of the work counter
 
 // Pair sources: map a global pair index to the pair's cost terms, tmax and
 // output slots.  One instance (idx in the reference's output order) ...
@@ -119,7 +125,8 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
                                                  unsigned long long *__restrict__ next_chunk)
 {
     const int lane = threadIdx.x & 63;
-    // each wave claims 64 pairs at a time from the global counter
+    // each wave claims kChunk pairs at a time from the global counter (one
+    // counter for the whole launch: small claims would serialise on it)
 
     int64_t my = -1;     // current pair of this lane, -1 when idle
     int64_t pool = 0;    // next unclaimed pair index in the wave's chunk
@@ -134,11 +141,12 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
     while (true) {
         // refill idle lanes from the wave's chunk, claiming a new chunk as needed
         unsigned long long idle = __ballot(my < 0);
-        while (idle != 0 && !done_all) {
+        const bool refill = __popcll(idle) >= kRefill || idle == ~0ull;
+        while (refill && idle != 0 && !done_all) {
             if (pool >= pool_end) {
                 unsigned long long claimed = 0;
                 if (lane == 0)
-                    claimed = atomicAdd(next_chunk, 64ull);
+                    claimed = atomicAdd(next_chunk, (unsigned long long)kChunk);
                 // every lane is active here, so the first active lane is lane 0
                 const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)claimed);
                 const uint32_t hi =
@@ -149,7 +157,7 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
                     break;
                 }
                 pool = b;
-                pool_end = b + 64 < n_pairs ? b + 64 : n_pairs;
+                pool_end = b + kChunk < n_pairs ? b + kChunk : n_pairs;
             }
             // rank of this lane among idle lanes
             int rank = __popcll(idle & ((1ull << lane) - 1ull));
@@ -170,49 +178,55 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
             pool += took;
             idle = __ballot(my < 0);
         }
-        if (__ballot(my >= 0) == 0)
-            break;
+        if (__ballot(my >= 0) == 0) {
+            if (done_all)
+                break;
+            continue;
+        }
         if (my >= 0) {
-            // one dcost evaluation of the current phase
-            float x = (state == ST_TMAX) ? tmax : (state == ST_LOW) ? tl : (tu + tl) / 2.0f;
-            float dc = dcost_eval(c, x);
-            bool finish = false;
-            float result = 0.0f;
-            if (state == ST_TMAX) {
-                if (dc < 0.0f) {
-                    finish = true;
-                    result = tmax;
-                } else {
-                    state = ST_LOW;
-                }
-            } else if (state == ST_LOW) {
-                if (dc > 0.0f && halvings < 400) {
-                    tl = tl / 2.0f;
-                    ++halvings;
-                } else {
-                    state = ST_BIS;
-                    if (!((double)fabsf(dcval) > 0.0001 && itr < 50)) {
-                        finish = true;
-                        result = topt;
-                    }
-                }
-            } else {
-                ++itr;
-                topt = x;
-                dcval = dc;
-                if (dcval > 0.0f)
-                    tu = topt;
-                else
-                    tl = topt;
-                if (!((double)fabsf(dcval) > 0.0001 && itr < 50)) {
-                    finish = true;
-                    result = topt;
-                }
-            }
-            if (finish) {
-                ot[oi] = result;
-                oc[oi] = cost_eval(c, result);
+            // the step's two cost evaluations
+            const float x = (state == ST_FIN)    ? topt
+                            : (state == ST_TMAX) ? tmax
+                            : (state == ST_LOW)  ? tl
+                                                 : (tu + tl) / 2.0f;
+            const float h = 0.000001f;
+            const float xa = state == ST_FIN ? x : x + h / 2.0f;
+            const float xb = state == ST_FIN ? x : x - h / 2.0f;
+            const float ca = cost_eval(c, xa);
+            const float cb = cost_eval(c, xb);
+            if (state == ST_FIN) {
+                ot[oi] = topt;
+                oc[oi] = ca;
                 my = -1;
+            } else {
+                const float dc = (ca - cb) / h; // dcost (2pBVP.cu:109-115)
+                if (state == ST_TMAX) {
+                    if (dc < 0.0f) {
+                        state = ST_FIN;
+                        topt = tmax;
+                    } else {
+                        state = ST_LOW;
+                    }
+                } else if (state == ST_LOW) {
+                    if (dc > 0.0f && halvings < 400) {
+                        tl = tl / 2.0f;
+                        ++halvings;
+                    } else {
+                        state = ST_BIS;
+                        if (!((double)fabsf(dcval) > 0.0001 && itr < 50))
+                            state = ST_FIN; // topt stays 0, as the reference returns it
+                    }
+                } else {
+                    ++itr;
+                    topt = x;
+                    dcval = dc;
+                    if (dcval > 0.0f)
+                        tu = topt;
+                    else
+                        tl = topt;
+                    if (!((double)fabsf(dcval) > 0.0001 && itr < 50))
+                        state = ST_FIN;
+                }
             }
         }
     }
