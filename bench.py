@@ -75,8 +75,8 @@ def edge_pass(P, d, n_obs, E, stream):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the data-parallel "
-              "split across GPUs is described in DESIGN.md and not run here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the split of the "
+              "work across GPUs is not run here.", file=sys.stderr)
         sys.exit(2)
     import pump_amd as P
     from pump_amd import configs
