@@ -2,9 +2,10 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
 
-Workload: the 1-GPU configuration (config 1: NUM=16000 samples, 30 boxes,
-~80 neighbours per sample, W_e = clamp(ceil(tau/0.05), 2, 64) waypoints per
-edge, 2000 particles, noise multiplier 0.03), synthetic seeded inputs.
+Workload (the default, --config 1 of pump_amd.configs): NUM=16000 samples,
+30 boxes, ~80 neighbours per sample, W_e = clamp(ceil(tau/0.05), 2, 64)
+waypoints per edge, 2000 particles, noise multiplier 0.03; synthetic seeded
+inputs.
 
 Setup (untimed): problem + noise tables, then the roadmap precompute on the
 device (all-pairs 2pBVP, rn selection, edge numbering, heuristic,
