@@ -80,6 +80,26 @@ __host__ __device__ inline HsLds hs_lds(int obs_cap, int stage_hs)
 
 __host__ __device__ inline int hs_obs_cap(int n_obs) { return n_obs < kHsObsLds ? n_obs : kHsObsLds; }
 
+// One wave writes a run of cnt floats from LDS to global memory: the elements
+// before the first 16-byte boundary singly, the body as 16-byte stores (one
+// KiB per wave instruction), the tail singly.
+__device__ __forceinline__ void copy_run(float *__restrict__ g, const float *l, int cnt, int lane)
+{
+    const int mis = (int)((reinterpret_cast<uintptr_t>(g) >> 2) & 3);
+    const int head = mis ? (4 - mis < cnt ? 4 - mis : cnt) : 0;
+    if (lane < head)
+        g[lane] = l[lane];
+    const int n4 = (cnt - head) >> 2;
+    float4 *g4 = reinterpret_cast<float4 *>(g + head);
+    for (int i = lane; i < n4; i += 64) {
+        const float *s = l + head + 4 * i;
+        g4[i] = make_float4(s[0], s[1], s[2], s[3]);
+    }
+    const int done = head + 4 * n4;
+    if (lane < cnt - done)
+        g[done + lane] = l[done + lane];
+}
+
 // cacheHalfspaces (hsmc.cu:10-99) for the waypoints of kEdgesPerBlock
 // consecutive edges, one lane per waypoint; with kSentinels the unfilled
 // slots get setupAsBs's values (hsmc.cu:201-211) in the same pass.
@@ -247,12 +267,8 @@ __device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0, char *smem,
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            float *ga = as + w0 * max_hs * 3;
-            float *gb = bs + w0 * max_hs;
-            for (int i = lane; i < nw * max_hs * 3; i += 64)
-                ga[i] = sa[i];
-            for (int i = lane; i < nw * max_hs; i += 64)
-                gb[i] = sb[i];
+            copy_run(as + w0 * max_hs * 3, sa, nw * max_hs * 3, lane);
+            copy_run(bs + w0 * max_hs, sb, nw * max_hs, lane);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
