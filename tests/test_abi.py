@@ -43,6 +43,12 @@ def test_invalid_arguments_rejected_without_gpu():
     assert P.lib.pump_propagate_cp(-1, 9, None, None, None, None, None, None, None, None, 0.05, 5,
                                    None, 10, 1.0, 64, None, None, 0, 64, None, 0, None, None, 0,
                                    None) == -1
+    cg, it = np.zeros(1, np.float32), np.zeros(1, np.int32)
+    assert P.lib.pump_gmt_wavefront(None, 0, None, None, None, 4, None, None, 0, None, None, None,
+                                    None, 100, None, None, 99, None, 0, P.fp(cg), P.ip(it),
+                                    None) == -1
+    assert P.lib.pump_precomp_nn(None, 10, 0.25, 4, None, None, None) == -1
+    assert P.lib.pump_gmt_workspace(0) == 0
 
 
 def test_philox_known_answers():
