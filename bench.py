@@ -343,10 +343,13 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
 def search_pass(P, rm, prm):
     """The whole PUMP search (PUMP.cu:3-405) on the roadmap, with its
     bookkeeping on the device and on the host; both must return the same
-    path."""
+    path. The roadmap keeps the search's device buffers between searches:
+    the first search grows them (hipMalloc/hipFree inside the call), the
+    ones after it reuse them. Both are reported; the second is the figure
+    for a planner that queries the same roadmap again."""
     import ctypes
     out = {}
-    for name, dev in (("device", 1), ("host", 0)):
+    for name, dev in (("device_first", 1), ("device", 1), ("host", 0)):
         q = P.Params()
         ctypes.memmove(ctypes.byref(q), ctypes.byref(prm), ctypes.sizeof(P.Params))
         q.validity_bits = 1
@@ -368,11 +371,15 @@ def search_pass(P, rm, prm):
         log(f"smoothing: {so.itrs} steps in {so.mccp_launches} full-MC launches, "
             f"{so.ms_total:.1f} ms, cost {so.cost_before:.2f} -> {so.cost:.2f}, cp {so.cp:.4f}")
     d, h = out["device"], out["host"]
-    same = all(d[k] == h[k] for k in ("status", "itrs", "num_paths", "num_goal_paths",
-                                       "soln_nodes", "soln_cost", "soln_cp", "particle_steps"))
-    log(f"search: {d['ms_total']:.1f} ms with device bookkeeping, {h['ms_total']:.1f} ms on the "
+    keys = ("status", "itrs", "num_paths", "num_goal_paths", "soln_nodes", "soln_cost",
+            "soln_cp", "particle_steps")
+    same = all(d[k] == h[k] == out["device_first"][k] for k in keys)
+    log(f"search: {d['ms_total']:.1f} ms with device bookkeeping (first search "
+        f"{out['device_first']['ms_total']:.1f} ms), {h['ms_total']:.1f} ms on the "
         f"host, {d['itrs']} iterations, {d['num_paths']} paths, same path: {same}")
-    return {"ms_device_bookkeeping": d["ms_total"], "ms_host_bookkeeping": h["ms_total"],
+    return {"ms_device_bookkeeping": d["ms_total"],
+            "ms_device_first_search": out["device_first"]["ms_total"],
+            "ms_host_bookkeeping": h["ms_total"],
             "ms_propagate": d["ms_propagate"], "itrs": d["itrs"], "num_paths": d["num_paths"],
             "num_considered": d["num_considered"], "status": d["status"],
             "soln_cost": d["soln_cost"], "soln_cp": d["soln_cp"], "same_result": same,

@@ -368,6 +368,8 @@ void pump::rmb::rm_release(pump_roadmap *rm)
         dfree(rm->xcomb);
     }
     dfree(rm->ws);
+    delete rm->search_ws;
+    rm->search_ws = nullptr;
     if (rm->stream && rm->owns_stream)
         hipStreamDestroy(rm->stream);
     delete rm;
@@ -606,10 +608,14 @@ int pump::rmb::select_goal_path(pump_roadmap *rm, const pump_params &P,
     status = 1;
     const int cap = P.path_cap;
     const int batch = P.mccp_batch > 0 ? P.mccp_batch : 1;
-    DevBuf d_paths, d_T, d_hits;
+    SearchWs *W = search_ws(rm);
+    if (!W)
+        return PUMP_E_NOMEM;
+    DevBuf &d_paths = W->sel_paths, &d_T = W->sel_T, &d_hits = W->sel_hits;
     std::vector<float> hpaths, xs;
     std::vector<int> hT, hits;
     std::vector<std::vector<int>> nodes_of;
+    StreamDrain drain{s};
     for (size_t k0 = 0; k0 < goal_paths.size() && status == 1; k0 += batch) {
         const int nk = (int)std::min<size_t>(batch, goal_paths.size() - k0);
         hpaths.assign((size_t)nk * cap * 6, 0.0f);
@@ -717,7 +723,12 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
 
     // device search state: validity rows of stored paths, their times, and
     // staging for the current wavefront
-    DevBuf valid, stage, d_time, d_cp, d_prev, d_edge, d_rows, d_clamp, d_band, d_ws;
+    SearchWs *W = search_ws(rm);
+    if (!W)
+        return PUMP_E_NOMEM;
+    DevBuf &valid = W->valid, &stage = W->stage, &d_time = W->time, &d_cp = W->cp,
+           &d_prev = W->wf_prev, &d_edge = W->wf_edge, &d_rows = W->src_rows, &d_clamp = W->clamp,
+           &d_ws = W->prop;
     TRY(valid.ensure((size_t)maxp * row_bytes));
     TRY(d_time.ensure(sizeof(float) * (size_t)maxp));
     TRY(d_clamp.ensure(2 * sizeof(unsigned long long)));
@@ -737,6 +748,7 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
     std::vector<std::vector<int>> Pn((size_t)num);
     std::vector<int> pnew((size_t)num, 0);
     std::vector<int> touched;
+    StreamDrain drain{s};
 
     const float dr = P.lambda * rm->rn;
     const float cp_min_soln = P.cp_target / P.cp_factor;

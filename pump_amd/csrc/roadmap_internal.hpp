@@ -4,10 +4,17 @@
 #pragma once
 
 #include <chrono>
+#include <new>
 #include <vector>
 
 #include "../../include/pump_host.h"
 #include "pump_internal.hpp"
+
+namespace pump {
+namespace rmb {
+struct SearchWs;
+}
+} // namespace pump
 
 struct pump_roadmap {
     int num = 0, n_obs = 0, n_mc = 0, tmax_steps = 0;
@@ -40,6 +47,8 @@ struct pump_roadmap {
     uint16_t *offsets_split = nullptr; // particle vectors of the split-BF16 test
     void *ws = nullptr;
     size_t ws_bytes = 0;
+    // device buffers of the search, kept between searches (allocated on first use)
+    pump::rmb::SearchWs *search_ws = nullptr;
     pump_roadmap_info info{};
     // a roadmap of a batch borrows the batch's stream and noise tables
     bool owns_stream = true, owns_noise = true;
@@ -113,6 +122,41 @@ struct DevBuf {
         if (p)
             hipFree(p);
     }
+};
+
+// The search's device buffers. The roadmap keeps them from one search to the
+// next: a repeated query on the same roadmap then makes no hipMalloc/hipFree
+// once the buffers have grown to its sizes (each hipFree of a grown buffer
+// also waits for the device). Every search initialises what it reads, so
+// the contents a previous search left behind are never used. The host- and
+// device-bookkeeping searches share the buffers whose meaning is the same;
+// searches on one roadmap run one at a time (its one stream).
+struct SearchWs {
+    // free out-edges to free nodes per node (offsets, targets), scan workspace
+    DevBuf voff, vedge, scan;
+    // path table, per-node path lists, bucket lists, validity rows
+    DevBuf prev, node, cost, time, cpv, next, head, open, valid, flags, steps, clamp;
+    // per-iteration wavefront state
+    DevBuf cnt, wf_prev, wf_edge, wf_next, stage, cp, new_time, new_cost, keep, node_cnt,
+        node_cur, grp, surv, src_rows, bkt, bflag, prop;
+    // goal-path selection: candidate paths, their lengths and MC hit counts
+    DevBuf sel_paths, sel_T, sel_hits;
+};
+
+// the roadmap's search buffers, created on first use (nullptr when out of host memory)
+inline SearchWs *search_ws(pump_roadmap *rm)
+{
+    if (!rm->search_ws)
+        rm->search_ws = new (std::nothrow) SearchWs();
+    return rm->search_ws;
+}
+
+// Waits for the stream when it leaves scope. Declared after the host buffers
+// that queued copies read or write, it keeps them alive until those copies
+// are done on every return path, the error returns included.
+struct StreamDrain {
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
 };
 
 struct EventTimer {

@@ -310,8 +310,11 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
     const float cp_min_soln = P.cp_target / P.cp_factor;
     const float cp_cutoff = P.cp_target * P.cp_factor;
 
+    SearchWs *W = search_ws(rm);
+    if (!W)
+        return PUMP_E_NOMEM;
     // free out-edges to free nodes per node
-    DevBuf voff, vedge, ws;
+    DevBuf &voff = W->voff, &vedge = W->vedge, &ws = W->scan;
     TRY(voff.ensure(sizeof(int64_t) * ((size_t)num + 1)));
     TRY(ws.ensure(pump_scan_workspace(std::max<int64_t>(num, maxp) + 1)));
     hipLaunchKernelGGL(k_valid_count, dim3(blocks_for(num)), dim3(kBlock), 0, s, rm->row_start,
@@ -330,7 +333,9 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
     PUMP_LAUNCHED();
 
     // the path table, per-node path lists, bucket lists, validity rows
-    DevBuf prev, node, cost, time, cpv, next, head, open, valid, flags, steps, clamp;
+    DevBuf &prev = W->prev, &node = W->node, &cost = W->cost, &time = W->time, &cpv = W->cpv,
+           &next = W->next, &head = W->head, &open = W->open, &valid = W->valid,
+           &flags = W->flags, &steps = W->steps, &clamp = W->clamp;
     TRY(prev.ensure(sizeof(int) * (size_t)maxp));
     TRY(node.ensure(sizeof(int) * (size_t)maxp));
     TRY(cost.ensure(sizeof(float) * (size_t)maxp));
@@ -363,8 +368,11 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
 
     std::vector<int64_t> sizeG(nb, 0);
     sizeG[0] = 1;
-    DevBuf cnt, wf_prev, wf_edge, wf_next, stage, d_cp, new_time, new_cost, keep, node_cnt,
-        node_cur, grp, surv, src_rows, bkt, bflag, d_ws;
+    DevBuf &cnt = W->cnt, &wf_prev = W->wf_prev, &wf_edge = W->wf_edge, &wf_next = W->wf_next,
+           &stage = W->stage, &d_cp = W->cp, &new_time = W->new_time, &new_cost = W->new_cost,
+           &keep = W->keep, &node_cnt = W->node_cnt, &node_cur = W->node_cur, &grp = W->grp,
+           &surv = W->surv, &src_rows = W->src_rows, &bkt = W->bkt, &bflag = W->bflag,
+           &d_ws = W->prop;
     TRY(node_cnt.ensure(sizeof(int64_t) * ((size_t)num + 1)));
     TRY(node_cur.ensure(sizeof(int) * (size_t)num));
 
@@ -493,20 +501,28 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
         }
         if (n_valid > 0) {
             std::vector<int64_t> add(nb, 0);
-            for (int b = 0; b < nb; ++b) {
-                hipLaunchKernelGGL(k_bucket_flag, dim3(blocks_for(n_valid)), dim3(kBlock), 0, s,
-                                   n_valid, as<int>(bkt), b, as<int64_t>(bflag));
-                PUMP_LAUNCHED();
-                TRY(scan_exclusive_i64(as<int64_t>(bflag), n_valid, ws.p, ws.bytes, s));
-                hipLaunchKernelGGL(k_bucket_place, dim3(blocks_for(n_valid)), dim3(kBlock), 0, s,
-                                   n_valid, num_paths, as<int64_t>(bflag),
-                                   as<int>(open) + (size_t)b * maxp, sizeG[b]);
-                PUMP_LAUNCHED();
-                TRYH(hipMemcpyAsync(&add[b], as<int64_t>(bflag) + n_valid, sizeof(int64_t),
-                                    hipMemcpyDeviceToHost, s));
-                // the next bucket's flags reuse bflag: wait for this copy
-                TRYH(hipStreamSynchronize(s));
-            }
+            auto place = [&]() -> int {
+                for (int b = 0; b < nb; ++b) {
+                    hipLaunchKernelGGL(k_bucket_flag, dim3(blocks_for(n_valid)), dim3(kBlock), 0,
+                                       s, n_valid, as<int>(bkt), b, as<int64_t>(bflag));
+                    PUMP_LAUNCHED();
+                    TRY(scan_exclusive_i64(as<int64_t>(bflag), n_valid, ws.p, ws.bytes, s));
+                    hipLaunchKernelGGL(k_bucket_place, dim3(blocks_for(n_valid)), dim3(kBlock), 0,
+                                       s, n_valid, num_paths, as<int64_t>(bflag),
+                                       as<int>(open) + (size_t)b * maxp, sizeG[b]);
+                    PUMP_LAUNCHED();
+                    // the next bucket's flags reuse bflag: the stream orders
+                    // this copy before them
+                    TRYH(hipMemcpyAsync(&add[b], as<int64_t>(bflag) + n_valid, sizeof(int64_t),
+                                        hipMemcpyDeviceToHost, s));
+                }
+                return PUMP_OK;
+            };
+            const int placed = place();
+            // one wait for every bucket's count; also on an error, since
+            // copies into add may still be queued
+            TRYH(hipStreamSynchronize(s));
+            TRY(placed);
             for (int b = 0; b < nb; ++b)
                 sizeG[b] += add[b];
             TRY(pump_gather_rows(as<uint8_t>(stage), as<int>(src_rows), (int)n_valid,

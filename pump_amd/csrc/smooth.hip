@@ -120,13 +120,19 @@ int smooth(pump_roadmap *rm, const pump_params &P, const pump_search_result &sre
 
     // the alpha bisection (PUMP.cu:475-537), kSmoothDepth steps per launch
     const int n_tree = (1 << kSmoothDepth) - 1;
-    DevBuf d_paths, d_T, d_hits;
+    // candidate paths, lengths and hit counts: the roadmap's goal-path
+    // selection buffers, kept between calls
+    SearchWs *W = search_ws(rm);
+    if (!W)
+        return PUMP_E_NOMEM;
+    DevBuf &d_paths = W->sel_paths, &d_T = W->sel_T, &d_hits = W->sel_hits;
     TRY(d_paths.ensure(sizeof(float) * (size_t)n_tree * cap * 6));
     TRY(d_T.ensure(sizeof(int) * n_tree));
     TRY(d_hits.ensure(sizeof(int) * n_tree));
     std::vector<Candidate> tree(n_tree);
     std::vector<float> hpaths((size_t)n_tree * cap * 6);
     std::vector<int> hT(n_tree), hits(n_tree);
+    StreamDrain drain{s};
     std::vector<float> last_xs = xs;
     float alpha = 1.0f, alpha_u = 1.0f, alpha_l = 0.0f;
     int k = 0;

@@ -143,6 +143,49 @@ def test_search_parity(case, bits, on_device):
     rm.close()
 
 
+def test_search_repeated_on_one_roadmap():
+    """The roadmap keeps the search's device buffers from one search to the
+    next. Searches in a row on one roadmap (validity bits, then bytes with
+    their wider rows, the host bookkeeping, a smaller particle count, and
+    smoothing between them) each return the oracle's result for their own
+    parameters, as a search on a fresh roadmap does."""
+    import ctypes
+    cfg, s, o, ot, xc, rm, orc = build_pair(CASES[1])
+    base = cfg["params"]
+
+    def params(**kw):
+        q = P.Params()
+        ctypes.memmove(ctypes.byref(q), ctypes.byref(base), ctypes.sizeof(P.Params))
+        for k, v in kw.items():
+            setattr(q, k, v)
+        return q
+
+    refs = {}
+    for step, kw in enumerate([dict(validity_bits=1, search_device=1),
+                               dict(validity_bits=0, search_device=1),
+                               dict(validity_bits=1, search_device=0),
+                               dict(validity_bits=1, search_device=1, n_p=97),
+                               dict(validity_bits=0, search_device=1, n_p=97),
+                               dict(validity_bits=1, search_device=1)]):
+        q = params(**kw)
+        r = rm.search(q)
+        got = P.search_result_dict(r)
+        if q.n_p not in refs:
+            refs[q.n_p] = OP.search(orc, s, s[-1], o, ot, xc, q, nthreads=8,
+                                    path_cap=q.path_cap)
+        ref = refs[q.n_p]
+        print("step", step, kw, got["status"], got["num_paths"], flush=True)
+        for k in ("status", "itrs", "num_paths", "num_considered", "num_goal_paths",
+                  "soln_path_idx", "soln_nodes", "particle_steps", "clamped_time_idx"):
+            assert got[k] == ref[k], (step, k)
+        for k in ("soln_cost", "soln_cp", "soln_time", "soln_hsmc_cp"):
+            assert np.float32(got[k]) == np.float32(ref[k]), (step, k)
+        if r.status == 0 and step in (0, 3):
+            rm.smooth(r, q)
+    assert len(refs) == 2
+    rm.close()
+
+
 # ------------------------------------------- entry points called directly ----
 def random_hsmc_inputs(seed, n_edges=60, W=7, max_hs=4, n_p=333, n_mc=512, T=80, n_new=50,
                        n_parent=9, variable=False, w_range=(2, 40)):
