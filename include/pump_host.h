@@ -230,6 +230,8 @@ typedef struct pump_search_result {
     float ms_total;        /* wall time of the search */
 } pump_search_result;
 
+/* The roadmap keeps the search's device buffers from its first search on
+ * (released by pump_roadmap_free); searches on one roadmap must not overlap. */
 int pump_search_run(pump_roadmap *rm, const pump_params *prm, pump_search_result *res);
 
 /* Smoothing of the search's solution path and its controller discretisation
