@@ -131,9 +131,8 @@ def main():
         P.check("pump_discretize_edges", P.lib.pump_discretize_edges(
             d.samples, d.edge_src, d.edge_dst, d.topts_edge, E, d.wp_offsets, 0, d.disc,
             stream.h))
-        P.check("pump_free_edges", P.lib.pump_free_edges(
-            d.obstacles, n_obs, d.samples, d.is_free_samples, 0, d.wp_offsets, d.disc,
-            d.is_free_edges, E, stream.h))
+        P.check("pump_free_edges_var", P.lib.pump_free_edges_var(
+            d.obstacles, n_obs, d.wp_offsets, d.disc, d.is_free_edges, E, stream.h))
 
     plain_pass()
     stream.sync()

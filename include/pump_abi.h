@@ -134,12 +134,22 @@ int pump_discretize_edges(const float *d_samples, const int *d_edge_src, const i
 int pump_sample_free(const float *d_obstacles, int n_obs, const float *d_samples, int num,
                      uint8_t *d_is_free_samples, pump_stream_t stream);
 
-/* freeEdges -- collisionCheck.cu:5-37, launched at mainPUMP.cu:444-452.
- * numDisc = waypoints_count-1 segments per edge (or wp_offsets). */
+/* freeEdges -- collisionCheck.cu:5-37 (signature collisionCheck.cu:6-8),
+ * launched at mainPUMP.cu:444-452.  The reference's arguments in its order:
+ * num_disc segments per edge, edge e's num_disc + 1 waypoints at rows
+ * e*(num_disc+1) .. of discMotions (collisionCheck.cu:20-23).  samples and
+ * isFreeSamples are accepted and unused, as in the reference; debugOutput is
+ * dropped and the stream added.  num_disc = 0 checks no segment (every edge
+ * free), as the reference's loop does. */
 int pump_free_edges(const float *d_obstacles, int n_obs, const float *d_samples,
-                    const uint8_t *d_is_free_samples, int waypoints_count,
-                    const int64_t *d_wp_offsets, const float *d_disc_motions,
+                    const uint8_t *d_is_free_samples, int num_disc, const float *d_disc_motions,
                     uint8_t *d_is_free_edges, int64_t n_edges, pump_stream_t stream);
+
+/* freeEdges over a variable number of waypoints per edge: edge e's
+ * waypoints are rows wp_offsets[e] .. wp_offsets[e+1]-1 (at least one). */
+int pump_free_edges_var(const float *d_obstacles, int n_obs, const int64_t *d_wp_offsets,
+                        const float *d_disc_motions, uint8_t *d_is_free_edges, int64_t n_edges,
+                        pump_stream_t stream);
 
 /* Fused edge pass (kernel b): waypoint generation (findDiscretizedPath) and
  * the segment/AABB test (freeEdges) in one launch; writes discMotions and

@@ -149,6 +149,37 @@ int pump_roadmap_build(const float *samples_host, int num, const float *obstacle
 int pump_roadmap_info_get(const pump_roadmap *rm, pump_roadmap_info *info);
 void pump_roadmap_free(pump_roadmap *rm);
 
+/* A roadmap from explicit tables instead of a precompute: the arrays the
+ * search consumer reads (PUMP.cu:3-405; mainPUMP.cu:313-504 builds them), in
+ * the reference's numbering (edges row-major by source, destinations
+ * ascending within a row, out-edge nn of node i = edge row_start[i] + nn).
+ * Sample 0 is the init, sample num-1 the goal.  Half-space slots are
+ * as[(wp*max_hs + h)*3 + d], bs[wp*max_hs + h] with wp the global waypoint
+ * index of wp_offsets; b >= 1000 is the empty sentinel.  The per-edge
+ * waypoint summary is derived from bs.  Waypoint rows are not held, so
+ * pump_roadmap_copy_waypoints and the edge passes do not apply. */
+typedef struct pump_roadmap_tables {
+    int num;
+    const float *samples;         /* [num*6] */
+    const float *obstacles;       /* [n_obs*12] */
+    int n_obs;
+    int64_t n_edges;
+    const int64_t *row_start;     /* [num+1] */
+    const int *edge_dst;          /* [E] */
+    const float *copts_edge, *topts_edge; /* [E] */
+    const float *h;               /* [num] */
+    const uint8_t *is_free_edges; /* [E] */
+    const uint8_t *is_free_samples; /* [num] */
+    const int64_t *wp_offsets;    /* [E+1] */
+    int max_hs;
+    const float *as, *bs;         /* [n_wp*max_hs*3], [n_wp*max_hs] */
+    float rn;
+} pump_roadmap_tables;
+
+int pump_roadmap_from_tables(const pump_roadmap_tables *t, const float *offsets_time_host,
+                             const float *xcomb_host, int n_mc, int tmax_steps,
+                             const pump_params *prm, pump_roadmap **out);
+
 /* host copies of the roadmap arrays (NULL pointers are skipped) */
 int pump_roadmap_copy_pairs(const pump_roadmap *rm, float *copts, float *topts);
 int pump_roadmap_copy_polished(const pump_roadmap *rm, double *topts64);
@@ -233,6 +264,13 @@ typedef struct pump_search_result {
 /* The roadmap keeps the search's device buffers from its first search on
  * (released by pump_roadmap_free); searches on one roadmap must not overlap. */
 int pump_search_run(pump_roadmap *rm, const pump_params *prm, pump_search_result *res);
+
+/* The stored path table of the roadmap's last search, rows [0, n) with n <=
+ * its num_paths: pathPrev, pathNode, pathCost, pathTime and pathCP of
+ * PUMP.cu (path 0 is the init path, prev -2).  NULL outputs are skipped;
+ * PUMP_E_RANGE if n exceeds the rows the search stored. */
+int pump_search_copy_paths(const pump_roadmap *rm, int64_t n, int *prev, int *node, float *cost,
+                           float *time, float *cp);
 
 /* Smoothing of the search's solution path and its controller discretisation
  * (PUMP.cu:406-568).  The solution's node states are blended towards the

@@ -27,6 +27,17 @@ u8p = ctypes.POINTER(ctypes.c_uint8)
 c_int, c_float, c_double, c_int64 = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_int64
 
 
+class SearchTrace(ctypes.Structure):
+    _fields_ = [
+        ("path_cap", c_int64), ("path_prev", i32p), ("path_node", i32p), ("path_bucket", i32p),
+        ("path_cost", f32p), ("path_time", f32p), ("path_cp", f32p),
+        ("itr_cap", c_int), ("itr_gidx", i32p), ("itr_n_new", i32p), ("itr_n_valid", i32p),
+        ("itr_size_g", i32p), ("itr_threshold", f32p),
+        ("wf_cap", c_int64), ("wf_prev", i32p), ("wf_edge", i32p), ("wf_fate", i32p),
+        ("wf_len", c_int64),
+    ]
+
+
 class SearchIn(ctypes.Structure):
     _fields_ = [
         ("num", c_int), ("samples", f32p), ("goal", f32p), ("init_idx", c_int), ("goal_idx", c_int),
@@ -37,7 +48,7 @@ class SearchIn(ctypes.Structure):
         ("n_obs", c_int), ("rn", c_float), ("lambda_", c_float), ("dt", c_float),
         ("cp_target", c_float), ("cp_factor", c_float), ("eps_cost", c_float), ("eps_cp", c_float),
         ("num_buckets", c_int), ("n_p", c_int), ("max_itrs", c_int), ("max_path_count", c_int64),
-        ("path_cap", c_int), ("nthreads", c_int),
+        ("path_cap", c_int), ("nthreads", c_int), ("trace", ctypes.POINTER(SearchTrace)),
     ]
 
 
@@ -96,6 +107,8 @@ _SIGS = {
     "po64_topt_near": (c_double, [f32p, f32p, c_double, c_double]),
     "po64_mccp_hits": (c_int, [f32p, c_int, f32p, c_int, c_int, c_double, f64p, c_int]),
     "po64_find_optimal_path": (c_int, [c_double, f64p, f32p, c_int, c_int]),
+    "po64_rows_cost_error": (None, [f32p, c_int, i32p, c_int, c_float, f32p, f32p, f64p, f64p,
+                                    c_int]),
 }
 
 _REF_SIGS = {
@@ -126,6 +139,12 @@ def load():
             raise RuntimeError("oracle/libpump_oracle.so is not built (run make -C oracle)")
         _LIB = _bind(ctypes.CDLL(path), _SIGS)
     return _LIB
+
+
+def load_from(path):
+    """A build of the restatement at another path (tests compile mutated
+    copies of it to show that the pins fail them)."""
+    return _bind(ctypes.CDLL(path), _SIGS)
 
 
 def load_ref():

@@ -9,7 +9,7 @@ import time
 
 import numpy as np
 
-from . import (SearchIn, SearchOut, bp, dp, fp, ip, load, lp)
+from . import (SearchIn, SearchOut, SearchTrace, bp, dp, fp, ip, load, lp)
 
 DT_WP = np.float32(0.05)
 
@@ -94,9 +94,16 @@ def roadmap(samples, obstacles, goal, prm, nthreads=0, pairs=None, timings=None)
                 n_edges=n_edges, n_wp=n_wp)
 
 
-def search(rm, samples, goal, obstacles, offsets_time, xcomb, prm, nthreads=0, path_cap=4096):
-    """Oracle PUMP search over the oracle roadmap rm."""
-    O = load()
+def search(rm, samples, goal, obstacles, offsets_time, xcomb, prm, nthreads=0, path_cap=4096,
+           trace=False, wf_cap=1 << 16, lib=None):
+    """Oracle PUMP search over the oracle roadmap rm.
+
+    With trace=True the result also holds the search record: the stored path
+    table (path_prev, path_node, path_bucket, path_cost, path_time, path_cp
+    over num_paths rows), per-iteration gidx, n_new, n_valid, size_g and
+    threshold, and the first wf_cap wavefront entries (wf_prev, wf_edge,
+    wf_fate: stored index, -1 CP cutoff, -2 dominated, -3 node 0)."""
+    O = lib if lib is not None else load()
     samples = np.ascontiguousarray(samples, np.float32)
     goal = np.ascontiguousarray(goal, np.float32)
     obstacles = np.ascontiguousarray(obstacles, np.float32).reshape(-1, 12)
@@ -148,10 +155,44 @@ def search(rm, samples, goal, obstacles, offsets_time, xcomb, prm, nthreads=0, p
         (2 * num) * np.sqrt(float(num)))
     si.path_cap = path_cap
     si.nthreads = nthreads
+    tr = None
+    if trace:
+        pcap = int(si.max_path_count)
+        nb = int(prm.num_buckets)
+        icap = max(int(prm.max_itrs), 1)
+        arr = dict(path_prev=np.full(pcap, -9, np.int32), path_node=np.full(pcap, -9, np.int32),
+                   path_bucket=np.full(pcap, -9, np.int32), path_cost=np.zeros(pcap, np.float32),
+                   path_time=np.zeros(pcap, np.float32), path_cp=np.zeros(pcap, np.float32),
+                   itr_gidx=np.zeros(icap, np.int32), itr_n_new=np.zeros(icap, np.int32),
+                   itr_n_valid=np.zeros(icap, np.int32), itr_size_g=np.zeros((icap, nb), np.int32),
+                   itr_threshold=np.zeros(icap, np.float32),
+                   wf_prev=np.zeros(max(wf_cap, 1), np.int32),
+                   wf_edge=np.zeros(max(wf_cap, 1), np.int32),
+                   wf_fate=np.zeros(max(wf_cap, 1), np.int32))
+        keep.append(arr)
+        tr = SearchTrace()
+        tr.path_cap = pcap
+        tr.itr_cap = icap
+        tr.wf_cap = wf_cap
+        for k, a in arr.items():
+            setattr(tr, k, fp(a) if a.dtype == np.float32 else ip(a))
+        si.trace = ctypes.pointer(tr)
     so = SearchOut()
     O.po_search(ctypes.byref(si), ctypes.byref(so))
     d = {k: getattr(so, k) for k, _ in SearchOut._fields_ if k != "soln_nodes"}
     d["soln_nodes"] = list(so.soln_nodes)[: so.soln_len]
+    if tr is not None:
+        n = int(so.num_paths)
+        # iterations that reached the compaction (one that found no new path stops before it)
+        ni = int(np.count_nonzero(arr["itr_n_new"]))
+        for k in ("path_prev", "path_node", "path_bucket", "path_cost", "path_time", "path_cp"):
+            d[k] = arr[k][:n].copy()
+        for k in ("itr_gidx", "itr_n_new", "itr_n_valid", "itr_size_g", "itr_threshold"):
+            d[k] = arr[k][:ni].copy()
+        d["wf_len"] = int(tr.wf_len)
+        m = min(int(tr.wf_len), wf_cap)
+        for k in ("wf_prev", "wf_edge", "wf_fate"):
+            d[k] = arr[k][:m].copy()
     return d
 
 

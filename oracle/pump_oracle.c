@@ -287,21 +287,44 @@ int64_t po_rn_index(int64_t num, float rn_perc)
     return (int64_t)prod;
 }
 
-static int cmp_float(const void *pa, const void *pb)
+/* rn -- mainPUMP.cu:303-304: rn = sort(copts)[rnIdx].
+ * Pinned by the order-statistic definition (numpy's full sort). */
+
+/* order-preserving key of a non-NaN float (-0 orders just below +0) */
+static uint32_t po_key(float f)
 {
-    float a = *(const float *)pa, b = *(const float *)pb;
-    return (a > b) - (a < b);
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-/* rn -- mainPUMP.cu:303-304: rn = sort(copts)[rnIdx].
- * Pinned by the order-statistic definition (full sort). */
+/* sorted(copts)[rn_idx] (mainPUMP.cu:303-304: thrust::sort, then the
+ * rnIdx-th entry) as an exact selection: the rank falls in one bin of the
+ * keys' upper 16 bits, then in one value of the lower 16 bits among that
+ * bin's entries; two passes instead of a full sort. */
 float po_select_rn(const float *copts, int64_t n_pairs, int64_t rn_idx)
 {
-    float *tmp = (float *)malloc(sizeof(float) * (size_t)n_pairs);
-    memcpy(tmp, copts, sizeof(float) * (size_t)n_pairs);
-    qsort(tmp, (size_t)n_pairs, sizeof(float), cmp_float);
-    float rn = tmp[rn_idx];
-    free(tmp);
+    int64_t *hist = (int64_t *)calloc(65536, sizeof(int64_t));
+    for (int64_t q = 0; q < n_pairs; ++q)
+        hist[po_key(copts[q]) >> 16]++;
+    int64_t below = 0;
+    uint32_t hi = 0;
+    while (below + hist[hi] <= rn_idx)
+        below += hist[hi++];
+    memset(hist, 0, 65536 * sizeof(int64_t));
+    for (int64_t q = 0; q < n_pairs; ++q) {
+        uint32_t k = po_key(copts[q]);
+        if ((k >> 16) == hi)
+            hist[k & 0xffffu]++;
+    }
+    uint32_t lo = 0;
+    while (below + hist[lo] <= rn_idx)
+        below += hist[lo++];
+    free(hist);
+    uint32_t key = (hi << 16) | lo;
+    uint32_t u = (key & 0x80000000u) ? (key & 0x7fffffffu) : ~key;
+    float rn;
+    memcpy(&rn, &u, 4);
     return rn;
 }
 
@@ -799,9 +822,15 @@ static int cmp_goal(const void *pa, const void *pb)
  * (P, nnIdxs) are held as per-node lists / CSR rows with identical order:
  * out-edge nn of node i is edge row_start[i] + nn (destinations ascending),
  * which is exactly nnGoEdges[i][nn] and nnIdxs[j][i].
- * Pinned by: invariants (CP monotone along paths, returned path is a chain
- * of free roadmap edges from init to goal, pathCost = sum of edge costs,
- * the selected path is the cheapest goal path whose MCCP is below target).
+ * Validity rows are held for the paths that exist (stored paths plus the
+ * current wavefront) instead of maxPathCount rows up front: every row is
+ * written (copied from its parent by propagate) before it is read, as in the
+ * reference, so only the allocation differs.
+ * Pinned by: hand-derived searches on small roadmaps (tests/
+ * test_oracle_search.py: per-iteration wavefronts, fates, buckets, the
+ * stored path table and the returned path, each rule of PUMP.cu:178-405
+ * exercised), and invariants (CP monotone along paths, returned path is a
+ * chain of free roadmap edges, pathCost = sum of edge costs).
  */
 int po_search(const po_search_in *in, po_search_out *out)
 {
@@ -819,7 +848,8 @@ int po_search(const po_search_in *in, po_search_out *out)
     float *path_cp = (float *)malloc(sizeof(float) * (size_t)maxp);
     float *d_path_cp = (float *)malloc(sizeof(float) * (size_t)maxp);   /* device mirror */
     float *d_path_time = (float *)malloc(sizeof(float) * (size_t)maxp); /* device mirror */
-    uint8_t *valid = (uint8_t *)malloc((size_t)(maxp * n_p));
+    int64_t vcap = 0; /* validity rows allocated */
+    uint8_t *valid = NULL;
     int *G = (int *)malloc(sizeof(int) * (size_t)(nb * maxp));
     int *sizeG = (int *)calloc((size_t)nb, sizeof(int));
     int *wf_prev = (int *)malloc(sizeof(int) * (size_t)maxp);
@@ -827,9 +857,21 @@ int po_search(const po_search_in *in, po_search_out *out)
     int *wf_edge = (int *)malloc(sizeof(int) * (size_t)maxp);
     ivec *P = (ivec *)calloc((size_t)num, sizeof(ivec));
     int *pnew = (int *)calloc((size_t)num, sizeof(int));
+    int *fate = (int *)malloc(sizeof(int) * (size_t)maxp);
+    po_search_trace *tr = in->trace;
     int status = 0;
 
-    po_setup_valid_particles(valid, n_p, maxp, d_path_cp, d_path_time);
+    /* setupValidParticles (PUMP.cu:766-777): every row true, CP 1, time 0;
+     * rows beyond path 0 are allocated when a wavefront needs them */
+    vcap = 1;
+    valid = (uint8_t *)malloc((size_t)n_p);
+    memset(valid, 1, (size_t)n_p);
+    for (int64_t k = 0; k < maxp; ++k) {
+        d_path_cp[k] = 1;
+        d_path_time[k] = 0;
+    }
+    if (tr)
+        tr->wf_len = 0;
     for (int64_t k = 0; k < nb * maxp; ++k)
         G[k] = -1;
 
@@ -850,6 +892,8 @@ int po_search(const po_search_in *in, po_search_out *out)
     G[0] = 0;
     float cost_threshold = in->h[in->init_idx];
     int itrs = 0;
+    if (tr && tr->path_cap > 0 && tr->path_bucket)
+        tr->path_bucket[0] = -1;
 
     while (itrs < in->max_itrs && !goal_cond && !empty_open) {
         ++itrs;
@@ -887,6 +931,21 @@ int po_search(const po_search_in *in, po_search_out *out)
         }
         sizeG[gidx] = 0;
         out->num_considered += n_new;
+        if (num_paths + n_new > vcap) {
+            int64_t want = num_paths + n_new;
+            want += want / 2;
+            if (want > maxp)
+                want = maxp;
+            uint8_t *nv = (uint8_t *)realloc(valid, (size_t)(want * n_p));
+            if (!nv) {
+                status = -7;
+                goto done;
+            }
+            valid = nv;
+            vcap = want;
+        }
+        for (int i = 0; i < n_new; ++i)
+            fate[i] = 0;
         for (int w = 0; w < n_new; ++w)
             out->particle_steps +=
                 (int64_t)n_p * (in->wp_offsets[wf_edge[w] + 1] - in->wp_offsets[wf_edge[w]]);
@@ -909,6 +968,7 @@ int po_search(const po_search_in *in, po_search_out *out)
             int64_t pidx = num_paths + i;
             if (path_cp[pidx] > cp_cutoff) {
                 wf_next[i] = -1;
+                fate[i] = -1;
                 continue;
             }
             ivec_push(&P[node], (int)pidx);
@@ -929,6 +989,7 @@ int po_search(const po_search_in *in, po_search_out *out)
                         path_cost[cmp] + in->eps_cost >= path_cost[pidx] && pidx < cmp)
                         continue;
                     wf_next[i] = -1;
+                    fate[i] = -2;
                     break;
                 }
             }
@@ -947,10 +1008,13 @@ int po_search(const po_search_in *in, po_search_out *out)
             int node = wf_next[i];
             int64_t pidx = num_paths + i;
             if (node == -1 || node == 0) {
+                if (node == 0)
+                    fate[i] = -3;
                 wf_next[i] = -1;
                 continue;
             }
             int64_t st = num_paths + n_valid;
+            fate[i] = (int)st;
             path_cp[st] = path_cp[pidx];
             path_time[st] = path_time[pidx];
             path_cost[st] = path_cost[pidx];
@@ -958,8 +1022,14 @@ int po_search(const po_search_in *in, po_search_out *out)
             float dcost = (path_cost[st] + in->h[node] - cost_threshold);
             dcost = dcost > 0 ? dcost : 0;
             int bucket = (((int)(dcost / dr)) + gidx + 1) % nb;
+            if (bucket < 0 || bucket >= nb) { /* negative rn: the reference indexes out of G */
+                status = -6;
+                goto done;
+            }
             G[bucket * maxp + sizeG[bucket]] = (int)st;
             sizeG[bucket]++;
+            if (tr && st < tr->path_cap && tr->path_bucket)
+                tr->path_bucket[st] = bucket;
             path_prev[st] = wf_prev[i];
             path_node[st] = node;
             ivec_push(&P[node], (int)st);
@@ -969,6 +1039,34 @@ int po_search(const po_search_in *in, po_search_out *out)
         memcpy(&d_path_cp[num_paths], &path_cp[num_paths], sizeof(float) * (size_t)n_valid);
         memcpy(&d_path_time[num_paths], &path_time[num_paths], sizeof(float) * (size_t)n_valid);
         num_paths += n_valid;
+        if (tr) {
+            int it = itrs - 1;
+            if (it < tr->itr_cap) {
+                if (tr->itr_gidx)
+                    tr->itr_gidx[it] = gidx;
+                if (tr->itr_n_new)
+                    tr->itr_n_new[it] = n_new;
+                if (tr->itr_n_valid)
+                    tr->itr_n_valid[it] = n_valid;
+                if (tr->itr_threshold)
+                    tr->itr_threshold[it] = cost_threshold;
+                if (tr->itr_size_g)
+                    for (int b = 0; b < nb; ++b)
+                        tr->itr_size_g[it * nb + b] = sizeG[b];
+            }
+            for (int i = 0; i < n_new; ++i) {
+                int64_t k = tr->wf_len + i;
+                if (k >= tr->wf_cap)
+                    break;
+                if (tr->wf_prev)
+                    tr->wf_prev[k] = wf_prev[i];
+                if (tr->wf_edge)
+                    tr->wf_edge[k] = wf_edge[i];
+                if (tr->wf_fate)
+                    tr->wf_fate[k] = fate[i];
+            }
+            tr->wf_len += n_new;
+        }
 
         for (int i = 0; i < P[in->goal_idx].n; ++i) {
             int pidx = P[in->goal_idx].v[i];
@@ -1049,6 +1147,21 @@ done:
     out->status = status;
     out->itrs = itrs;
     out->num_paths = num_paths;
+    if (tr && tr->path_cap > 0) {
+        int64_t n = num_paths < tr->path_cap ? num_paths : tr->path_cap;
+        for (int64_t k = 0; k < n; ++k) {
+            if (tr->path_prev)
+                tr->path_prev[k] = path_prev[k];
+            if (tr->path_node)
+                tr->path_node[k] = path_node[k];
+            if (tr->path_cost)
+                tr->path_cost[k] = path_cost[k];
+            if (tr->path_time)
+                tr->path_time[k] = path_time[k];
+            if (tr->path_cp)
+                tr->path_cp[k] = path_cp[k];
+        }
+    }
     for (int n = 0; n < num; ++n)
         free(P[n].v);
     free(P);
@@ -1066,6 +1179,7 @@ done:
     free(wf_prev);
     free(wf_next);
     free(wf_edge);
+    free(fate);
     return status;
 }
 
@@ -1148,6 +1262,58 @@ double po64_topt_near(const float *x0, const float *x1, double t_guess, double t
             lo = m;
     }
     return 0.5 * (lo + hi);
+}
+
+/* Cost errors of sampled rows against the FP64 oracle.  For each pair (i, j)
+ * of the rows, tau64 = po64_topt_near(x_i, x_j, topts32, tmax) is the FP64
+ * stationary point at the local minimum the bisection landed on and c64 =
+ * po64_cost(tau64).  stats[0..2]: max |c - c64| / max(|c64|, 1e-2) for
+ * c = po64_cost(topts_pol) (FP64-polished arrival time), c = copts32 (the
+ * FP32 cost table) and c = po64_cost(topts32); stats[3..4]: max |t - tau64| /
+ * tau64 for topts_pol and topts32; stats[5]: pairs compared.  Tables are
+ * [nrows][num-1] in the row layout of fillCoptsTopts. */
+void po64_rows_cost_error(const float *samples, int num, const int *rows, int nrows, float tmax,
+                          const float *topts32, const float *copts32, const double *topts_pol,
+                          double *stats, int nthreads)
+{
+    double m[5] = {0, 0, 0, 0, 0};
+    const int64_t n = (int64_t)nrows * (num - 1);
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#endif
+    double m0 = 0, m1 = 0, m2 = 0, m3 = 0, m4 = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) reduction(max : m0, m1, m2, m3, m4) if (nthreads != 1)
+#endif
+    for (int64_t q = 0; q < n; ++q) {
+        const int i = rows[q / (num - 1)];
+        const int k = (int)(q % (num - 1));
+        const int j = k + (k >= i);
+        const float *a = &samples[(int64_t)i * DIM], *b = &samples[(int64_t)j * DIM];
+        const double t64 = po64_topt_near(a, b, (double)topts32[q], (double)tmax);
+        const double c64 = po64_cost(t64, a, b);
+        const double den = fabs(c64) > 1e-2 ? fabs(c64) : 1e-2;
+        const double e0 = fabs(po64_cost(topts_pol[q], a, b) - c64) / den;
+        const double e1 = fabs((double)copts32[q] - c64) / den;
+        const double e2 = fabs(po64_cost((double)topts32[q], a, b) - c64) / den;
+        const double e3 = fabs(topts_pol[q] - t64) / t64;
+        const double e4 = fabs((double)topts32[q] - t64) / t64;
+        m0 = e0 > m0 ? e0 : m0;
+        m1 = e1 > m1 ? e1 : m1;
+        m2 = e2 > m2 ? e2 : m2;
+        m3 = e3 > m3 ? e3 : m3;
+        m4 = e4 > m4 ? e4 : m4;
+    }
+    m[0] = m0;
+    m[1] = m1;
+    m[2] = m2;
+    m[3] = m3;
+    m[4] = m4;
+    for (int k = 0; k < 5; ++k)
+        stats[k] = m[k];
+    stats[5] = (double)n;
+    (void)nthreads;
 }
 
 static int po64_face(const double *v, const double *w, double lambda, int j, const float *lo,

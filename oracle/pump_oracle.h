@@ -100,6 +100,26 @@ float po_collision_probability(const float *obstacles, int n_obs, const float *x
                                const float *path, int T);
 
 /* ---- PUMP search consumer (PUMP.cu:3-405) ------------------------------ */
+
+/* Optional record of a search, for tests: the stored path table, each
+ * iteration's counters, and the fate of every wavefront entry.  A zero cap
+ * (or a NULL array) skips that part; entries past a cap are not written. */
+typedef struct po_search_trace {
+    int64_t path_cap;               /* rows of the path arrays */
+    int *path_prev, *path_node;     /* pathPrev, pathNode */
+    int *path_bucket;               /* bucket a stored path was appended to (-1 for path 0) */
+    float *path_cost, *path_time, *path_cp;
+    int itr_cap;                    /* rows of the per-iteration arrays */
+    int *itr_gidx;                  /* bucket expanded (Gidx) */
+    int *itr_n_new, *itr_n_valid;   /* numNewPaths, numValidNewPaths */
+    int *itr_size_g;                /* [itr][num_buckets] sizeG after the iteration */
+    float *itr_threshold;           /* costThreshold used by the iteration */
+    int64_t wf_cap;                 /* entries of the wavefront log */
+    int *wf_prev, *wf_edge;         /* wavefrontPathPrev, wavefrontEdge, iterations concatenated */
+    int *wf_fate;                   /* stored index, or -1 CP cutoff, -2 dominated, -3 node 0 */
+    int64_t wf_len;                 /* out: entries the search produced (may exceed wf_cap) */
+} po_search_trace;
+
 typedef struct po_search_in {
     int num;
     const float *samples;           /* [num*6] */
@@ -125,6 +145,7 @@ typedef struct po_search_in {
     int64_t max_path_count;
     int path_cap;                   /* row capacity of the MCCP path buffer */
     int nthreads;
+    po_search_trace *trace;         /* optional (NULL) */
 } po_search_in;
 
 typedef struct po_search_out {
@@ -146,6 +167,9 @@ int po_search(const po_search_in *in, po_search_out *out);
 
 /* ---- FP64 oracle ------------------------------------------------------ */
 double po64_cost(double tau, const float *x0, const float *x1);
+void po64_rows_cost_error(const float *samples, int num, const int *rows, int nrows, float tmax,
+                          const float *topts32, const float *copts32, const double *topts_pol,
+                          double *stats, int nthreads);
 /* tau* by exact-arithmetic restatement of the bisection bracket in FP64 on
  * the quartic (1+g^2)tau^4 - 4R tau^2 + 24 S tau - 36 D = 0 */
 double po64_topt(const float *x0, const float *x1, double tmax);

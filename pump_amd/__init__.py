@@ -105,6 +105,14 @@ class SearchResult(ctypes.Structure):
     ]
 
 
+class RoadmapTables(ctypes.Structure):
+    _fields_ = [("num", c_int), ("samples", f32p), ("obstacles", f32p), ("n_obs", c_int),
+                ("n_edges", c_int64), ("row_start", i64p), ("edge_dst", i32p),
+                ("copts_edge", f32p), ("topts_edge", f32p), ("h", f32p),
+                ("is_free_edges", u8p), ("is_free_samples", u8p), ("wp_offsets", i64p),
+                ("max_hs", c_int), ("as_", f32p), ("bs", f32p), ("rn", c_float)]
+
+
 class BatchInstance(ctypes.Structure):
     _fields_ = [("samples", f32p), ("num", c_int), ("obstacles", f32p), ("n_obs", c_int),
                 ("goal", f32p), ("prm", Params)]
@@ -149,7 +157,8 @@ _SIGS = {
     "pump_waypoint_offsets": (c_int, [S, c_int64, c_int, c_float, c_int, S, i64p, S, c_size_t, S]),
     "pump_discretize_edges": (c_int, [S, S, S, S, c_int64, S, c_int, S, S]),
     "pump_sample_free": (c_int, [S, c_int, S, c_int, S, S]),
-    "pump_free_edges": (c_int, [S, c_int, S, S, c_int, S, S, S, c_int64, S]),
+    "pump_free_edges": (c_int, [S, c_int, S, S, c_int, S, S, c_int64, S]),
+    "pump_free_edges_var": (c_int, [S, c_int, S, S, S, c_int64, S]),
     "pump_edges_discretize_check": (c_int, [S, S, S, S, c_int64, S, c_int, S, c_int, S, S, S]),
     "pump_setup_as_bs": (c_int, [c_int64, S, S, S]),
     "pump_maxd2": (c_float, [c_float]),
@@ -187,6 +196,10 @@ _SIGS = {
     "pump_params_defaults": (None, [ctypes.POINTER(Params)]),
     "pump_roadmap_build": (c_int, [f32p, c_int, f32p, c_int, f32p, f32p, f32p, c_int, c_int,
                                    ctypes.POINTER(Params), ctypes.POINTER(c_void_p)]),
+    "pump_roadmap_from_tables": (c_int, [ctypes.POINTER(RoadmapTables), f32p, f32p, c_int,
+                                         c_int, ctypes.POINTER(Params),
+                                         ctypes.POINTER(c_void_p)]),
+    "pump_search_copy_paths": (c_int, [c_void_p, c_int64, i32p, i32p, f32p, f32p, f32p]),
     "pump_roadmap_info_get": (c_int, [c_void_p, ctypes.POINTER(RoadmapInfo)]),
     "pump_roadmap_free": (None, [c_void_p]),
     "pump_roadmap_copy_pairs": (c_int, [c_void_p, f32p, f32p]),
@@ -419,6 +432,46 @@ class Roadmap:
         check("pump_roadmap_info_get", lib.pump_roadmap_info_get(self._h, ctypes.byref(self.info)))
 
     @classmethod
+    def from_tables(cls, tables, samples, obstacles, offsets_time, xcomb, prm):
+        """A roadmap from explicit search tables (pump_roadmap_from_tables):
+        tables holds row_start, edge_dst, copts_edge, topts_edge, h,
+        is_free_edges, is_free_samples, wp_offsets, as_, bs, max_hs, rn."""
+        rm = cls.__new__(cls)
+        rm.samples = np.ascontiguousarray(samples, np.float32)
+        rm.obstacles = np.ascontiguousarray(obstacles, np.float32).reshape(-1, 12)
+        rm.goal = rm.samples[-1].copy()
+        rm.prm = prm
+        keep = {k: np.ascontiguousarray(tables[k], dt) for k, dt in (
+            ("row_start", np.int64), ("edge_dst", np.int32), ("copts_edge", np.float32),
+            ("topts_edge", np.float32), ("h", np.float32), ("is_free_edges", np.uint8),
+            ("is_free_samples", np.uint8), ("wp_offsets", np.int64), ("as_", np.float32),
+            ("bs", np.float32))}
+        obs = rm.obstacles if rm.obstacles.size else np.zeros((1, 12), np.float32)
+        t = RoadmapTables()
+        t.num = rm.samples.shape[0]
+        t.samples = fp(rm.samples)
+        t.obstacles = fp(obs)
+        t.n_obs = rm.obstacles.shape[0]
+        t.n_edges = keep["edge_dst"].shape[0]
+        t.row_start, t.edge_dst = lp(keep["row_start"]), ip(keep["edge_dst"])
+        t.copts_edge, t.topts_edge = fp(keep["copts_edge"]), fp(keep["topts_edge"])
+        t.h = fp(keep["h"])
+        t.is_free_edges, t.is_free_samples = bp(keep["is_free_edges"]), bp(keep["is_free_samples"])
+        t.wp_offsets = lp(keep["wp_offsets"])
+        t.max_hs = int(tables["max_hs"])
+        t.as_, t.bs = fp(keep["as_"]), fp(keep["bs"])
+        t.rn = float(tables["rn"])
+        ot = np.ascontiguousarray(offsets_time, np.float32)
+        xc = np.ascontiguousarray(xcomb, np.float32)
+        rm._h = c_void_p()
+        check("pump_roadmap_from_tables", lib.pump_roadmap_from_tables(
+            ctypes.byref(t), fp(ot), fp(xc), ot.shape[1], ot.shape[0], ctypes.byref(prm),
+            ctypes.byref(rm._h)))
+        rm.info = RoadmapInfo()
+        check("pump_roadmap_info_get", lib.pump_roadmap_info_get(rm._h, ctypes.byref(rm.info)))
+        return rm
+
+    @classmethod
     def _borrowed(cls, handle, owner, samples, obstacles, goal, prm):
         """A roadmap of a batch (freed with the batch)."""
         rm = cls.__new__(cls)
@@ -501,6 +554,14 @@ class Roadmap:
         check("pump_search_run",
               lib.pump_search_run(self._h, ctypes.byref(prm or self.prm), ctypes.byref(r)))
         return r
+
+    def paths(self, n):
+        """The last search's stored path table, rows [0, n)."""
+        prev, node = np.empty(n, np.int32), np.empty(n, np.int32)
+        cost, tm, cp = np.empty(n, np.float32), np.empty(n, np.float32), np.empty(n, np.float32)
+        check("pump_search_copy_paths", lib.pump_search_copy_paths(
+            self._h, n, ip(prev), ip(node), fp(cost), fp(tm), fp(cp)))
+        return dict(path_prev=prev, path_node=node, path_cost=cost, path_time=tm, path_cp=cp)
 
     def smooth(self, search_result, prm=None, path_cap=4096, disc_cap=1 << 16):
         """Smoothing and controller discretisation (PUMP.cu:406-568) of a

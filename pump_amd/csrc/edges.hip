@@ -510,18 +510,12 @@ extern "C" int pump_edges_discretize_check(const float *d_samples, const int *d_
                                          nullptr, d_is_free_edges, as_stream(stream));
 }
 
-extern "C" int pump_free_edges(const float *d_obstacles, int n_obs, const float *d_samples,
-                               const uint8_t *d_is_free_samples, int waypoints_count,
-                               const int64_t *d_wp_offsets, const float *d_disc_motions,
-                               uint8_t *d_is_free_edges, int64_t n_edges, pump_stream_t stream)
+namespace {
+
+int free_edges_launch(const float *d_obstacles, int n_obs, int waypoints_count,
+                      const int64_t *d_wp_offsets, const float *d_disc_motions,
+                      uint8_t *d_is_free_edges, int64_t n_edges, pump_stream_t stream)
 {
-    (void)d_samples;          // passed but unused, as in the reference
-    (void)d_is_free_samples;  // idem (collisionCheck.cu:5-37)
-    PUMP_REQUIRE(n_edges >= 0 && n_obs >= 0);
-    if (n_edges == 0)
-        return PUMP_OK;
-    PUMP_REQUIRE(d_disc_motions && d_is_free_edges && (n_obs == 0 || d_obstacles));
-    PUMP_REQUIRE(d_wp_offsets || waypoints_count >= 1);
     unsigned blocks;
     int r = edge_blocks(n_edges, blocks);
     if (r)
@@ -531,6 +525,37 @@ extern "C" int pump_free_edges(const float *d_obstacles, int n_obs, const float 
                        n_edges);
     PUMP_LAUNCHED();
     return PUMP_OK;
+}
+
+} // namespace
+
+extern "C" int pump_free_edges(const float *d_obstacles, int n_obs, const float *d_samples,
+                               const uint8_t *d_is_free_samples, int num_disc,
+                               const float *d_disc_motions, uint8_t *d_is_free_edges,
+                               int64_t n_edges, pump_stream_t stream)
+{
+    (void)d_samples;          // passed but unused, as in the reference
+    (void)d_is_free_samples;  // idem (collisionCheck.cu:5-37)
+    PUMP_REQUIRE(n_edges >= 0 && n_obs >= 0 && num_disc >= 0);
+    PUMP_REQUIRE(num_disc < kMaxWaypointsPerEdge);
+    if (n_edges == 0)
+        return PUMP_OK;
+    PUMP_REQUIRE(d_disc_motions && d_is_free_edges && (n_obs == 0 || d_obstacles));
+    // numDisc segments per edge, its waypoints (numDisc + 1 rows) contiguous
+    return free_edges_launch(d_obstacles, n_obs, num_disc + 1, nullptr, d_disc_motions,
+                             d_is_free_edges, n_edges, stream);
+}
+
+extern "C" int pump_free_edges_var(const float *d_obstacles, int n_obs,
+                                   const int64_t *d_wp_offsets, const float *d_disc_motions,
+                                   uint8_t *d_is_free_edges, int64_t n_edges, pump_stream_t stream)
+{
+    PUMP_REQUIRE(n_edges >= 0 && n_obs >= 0);
+    if (n_edges == 0)
+        return PUMP_OK;
+    PUMP_REQUIRE(d_wp_offsets && d_disc_motions && d_is_free_edges && (n_obs == 0 || d_obstacles));
+    return free_edges_launch(d_obstacles, n_obs, 0, d_wp_offsets, d_disc_motions, d_is_free_edges,
+                             n_edges, stream);
 }
 
 extern "C" int pump_sample_free(const float *d_obstacles, int n_obs, const float *d_samples,

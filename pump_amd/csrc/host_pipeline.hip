@@ -149,8 +149,8 @@ int rm_edge_pass(pump_roadmap *rm)
     } else {
         TRY(pump_discretize_edges(rm->samples, rm->edge_src, rm->edge_dst, rm->topts_edge,
                                   rm->n_edges, rm->wp_offsets, 0, rm->disc, s));
-        TRY(pump_free_edges(rm->obstacles, rm->n_obs, rm->samples, rm->is_free_samples, 0,
-                            rm->wp_offsets, rm->disc, rm->is_free_edges, rm->n_edges, s));
+        TRY(pump_free_edges_var(rm->obstacles, rm->n_obs, rm->wp_offsets, rm->disc,
+                                rm->is_free_edges, rm->n_edges, s));
     }
     rm->info.ms_edge_pass = tm.stop();
     return PUMP_OK;
@@ -322,6 +322,124 @@ extern "C" int pump_roadmap_build(const float *samples_h, int num, const float *
     if (r != PUMP_OK) {
         pump_roadmap_free(rm);
         *out = nullptr;
+        return r;
+    }
+    *out = rm;
+    return PUMP_OK;
+}
+
+namespace {
+
+template <typename T> int upload_n(T **dst, const T *src, size_t n, hipStream_t s)
+{
+    TRY(dalloc(dst, n));
+    if (n)
+        TRYH(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, s));
+    return PUMP_OK;
+}
+
+int from_tables(pump_roadmap *rm, const pump_roadmap_tables &t, const float *offsets_time_h,
+                const float *xcomb_h)
+{
+    const auto t_begin = std::chrono::steady_clock::now();
+    hipStream_t s = rm->stream;
+    const int num = t.num;
+    const size_t E = (size_t)t.n_edges;
+    // the host tables are checked before anything is uploaded
+    if (t.row_start[0] != 0 || t.row_start[num] != t.n_edges || t.wp_offsets[0] != 0)
+        return PUMP_E_INVALID;
+    std::vector<int> src(E);
+    for (int i = 0; i < num; ++i) {
+        if (t.row_start[i + 1] < t.row_start[i])
+            return PUMP_E_INVALID;
+        for (int64_t e = t.row_start[i]; e < t.row_start[i + 1]; ++e) {
+            const int d = t.edge_dst[e];
+            if (d < 0 || d >= num || d == i || (e > t.row_start[i] && t.edge_dst[e - 1] >= d))
+                return PUMP_E_INVALID;
+            src[(size_t)e] = i;
+        }
+    }
+    for (size_t e = 0; e < E; ++e)
+        if (t.wp_offsets[e + 1] - t.wp_offsets[e] < 2)
+            return PUMP_E_INVALID;
+    const int64_t n_wp = t.wp_offsets[E];
+    const int mh = t.max_hs;
+    // the per-edge waypoint summary the batched HSMC pass reads (bit min(k, 63):
+    // waypoint k stores a half-space whose b is not >= 1000) and the stored counts
+    std::vector<uint64_t> mask(E, 0);
+    std::vector<int> count((size_t)n_wp, 0);
+    for (size_t e = 0; e < E; ++e)
+        for (int64_t g = t.wp_offsets[e]; g < t.wp_offsets[e + 1]; ++g) {
+            int live = 0;
+            for (int h = 0; h < mh; ++h)
+                live += !(t.bs[g * mh + h] >= 1000.0f);
+            count[(size_t)g] = live;
+            if (live) {
+                const int64_t k = g - t.wp_offsets[e];
+                mask[e] |= 1ull << (k < 63 ? k : 63);
+            }
+        }
+    TRY(rm_upload(rm, t.samples, t.obstacles, offsets_time_h, xcomb_h));
+    dfree(rm->copts); // no pair tables: the roadmap starts from its edges
+    dfree(rm->topts);
+    rm->n_pairs = 0;
+    rm->rn = t.rn;
+    rm->n_edges = t.n_edges;
+    rm->n_wp = n_wp;
+    rm->max_hs = mh;
+    rm->maxd2 = pump_maxd2(rm->prm.offset_mult);
+    TRY(upload_n(&rm->row_start, t.row_start, (size_t)num + 1, s));
+    TRY(upload_n(&rm->edge_src, (const int *)src.data(), E, s));
+    TRY(upload_n(&rm->edge_dst, t.edge_dst, E, s));
+    TRY(upload_n(&rm->copts_edge, t.copts_edge, E, s));
+    TRY(upload_n(&rm->topts_edge, t.topts_edge, E, s));
+    TRY(upload_n(&rm->h, t.h, (size_t)num, s));
+    TRY(upload_n(&rm->is_free_edges, t.is_free_edges, E, s));
+    TRY(upload_n(&rm->is_free_samples, t.is_free_samples, (size_t)num, s));
+    TRY(upload_n(&rm->wp_offsets, t.wp_offsets, E + 1, s));
+    TRY(upload_n(&rm->as, t.as, (size_t)n_wp * mh * 3, s));
+    TRY(upload_n(&rm->bs, t.bs, (size_t)n_wp * mh, s));
+    TRY(upload_n(&rm->count, (const int *)count.data(), (size_t)n_wp, s));
+    TRY(upload_n(&rm->edge_hs_mask, (const uint64_t *)mask.data(), E, s));
+    TRY(dalloc(&rm->in_degree, (size_t)num));
+    TRY(pump_max_nn_size(rm->row_start, rm->edge_dst, rm->n_edges, num, rm->in_degree, &rm->max_nn,
+                         s));
+    return rm_finish(rm, t_begin);
+}
+
+} // namespace
+
+extern "C" int pump_roadmap_from_tables(const pump_roadmap_tables *t, const float *offsets_time_h,
+                                        const float *xcomb_h, int n_mc, int tmax_steps,
+                                        const pump_params *prm, pump_roadmap **out)
+{
+    if (!t || !out || !prm || t->num < 2 || !t->samples || t->n_obs < 0 ||
+        (t->n_obs > 0 && !t->obstacles) || t->n_edges < 0 || !t->row_start || !t->wp_offsets ||
+        !t->h || !t->is_free_samples || t->max_hs < 1 ||
+        (t->n_edges > 0 && (!t->edge_dst || !t->copts_edge || !t->topts_edge ||
+                            !t->is_free_edges || !t->as || !t->bs)) ||
+        !offsets_time_h || !xcomb_h || n_mc < 1 || tmax_steps < 1)
+        return PUMP_E_INVALID;
+    if (t->n_edges > 0x7fffffff)
+        return PUMP_E_RANGE;
+    *out = nullptr;
+    pump_roadmap *rm = new pump_roadmap();
+    rm->num = t->num;
+    rm->n_obs = t->n_obs;
+    rm->n_mc = n_mc;
+    rm->tmax_steps = tmax_steps;
+    rm->prm = *prm;
+    std::memcpy(rm->goal, t->samples + (size_t)(t->num - 1) * 6, sizeof(rm->goal));
+    rm->samples_h.assign(t->samples, t->samples + (size_t)t->num * 6);
+    hipError_t ce = hipStreamCreate(&rm->stream);
+    if (ce != hipSuccess) {
+        note_hip_error(ce, __FILE__, __LINE__);
+        delete rm;
+        return PUMP_E_HIP;
+    }
+    int r = from_tables(rm, *t, offsets_time_h, xcomb_h);
+    if (r != PUMP_OK) {
+        pump_roadmap_free(rm);
         return r;
     }
     *out = rm;
@@ -920,6 +1038,8 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
             wf_next[i] = -1;
             n_valid++;
         }
+        if (status == -6)
+            break; // as the device search: stop before this iteration's paths count
         if (n_valid > 0) {
             TRY(d_rows.ensure(sizeof(int) * n_valid));
             TRYH(hipMemcpyAsync(d_rows.p, src_rows.data(), sizeof(int) * n_valid,
@@ -962,6 +1082,26 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
     res->num_paths = num_paths;
     res->ms_propagate = ms_prop;
     res->soln_path_idx = -1;
+    {
+        // the stored path table, where pump_search_copy_paths reads it
+        const size_t np = (size_t)num_paths;
+        TRY(W->prev.ensure(sizeof(int) * np));
+        TRY(W->node.ensure(sizeof(int) * np));
+        TRY(W->cost.ensure(sizeof(float) * np));
+        TRY(W->cpv.ensure(sizeof(float) * np));
+        TRYH(hipMemcpyAsync(W->prev.p, path_prev.data(), sizeof(int) * np, hipMemcpyHostToDevice,
+                            s));
+        TRYH(hipMemcpyAsync(W->node.p, path_node.data(), sizeof(int) * np, hipMemcpyHostToDevice,
+                            s));
+        TRYH(hipMemcpyAsync(W->cost.p, path_cost.data(), sizeof(float) * np,
+                            hipMemcpyHostToDevice, s));
+        TRYH(hipMemcpyAsync(W->time.p, path_time.data(), sizeof(float) * np,
+                            hipMemcpyHostToDevice, s));
+        TRYH(hipMemcpyAsync(W->cpv.p, path_cp.data(), sizeof(float) * np, hipMemcpyHostToDevice,
+                            s));
+        TRYH(hipStreamSynchronize(s));
+        W->last_num_paths = num_paths;
+    }
 
     if (status == 0) {
         PathTable T{path_prev.data(), path_node.data(), path_cost.data(), path_time.data(),
@@ -975,6 +1115,25 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
 }
 
 } // namespace
+
+extern "C" int pump_search_copy_paths(const pump_roadmap *rm, int64_t n, int *prev, int *node,
+                                      float *cost, float *time, float *cp)
+{
+    if (!rm || n < 0)
+        return PUMP_E_INVALID;
+    const pump::rmb::SearchWs *W = rm->search_ws;
+    if (n == 0)
+        return PUMP_OK;
+    if (!W || n > W->last_num_paths)
+        return PUMP_E_RANGE;
+    const size_t k = (size_t)n;
+    TRY(d2h(prev, W->prev.p, sizeof(int) * k));
+    TRY(d2h(node, W->node.p, sizeof(int) * k));
+    TRY(d2h(cost, W->cost.p, sizeof(float) * k));
+    TRY(d2h(time, W->time.p, sizeof(float) * k));
+    TRY(d2h(cp, W->cpv.p, sizeof(float) * k));
+    return PUMP_OK;
+}
 
 extern "C" int pump_search_run(pump_roadmap *rm, const pump_params *prm, pump_search_result *res)
 {

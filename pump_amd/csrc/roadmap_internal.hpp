@@ -141,6 +141,8 @@ struct SearchWs {
         node_cur, grp, surv, src_rows, bkt, bflag, prop;
     // goal-path selection: candidate paths, their lengths and MC hit counts
     DevBuf sel_paths, sel_T, sel_hits;
+    // rows of the path table (prev, node, cost, time, cpv) the last search stored
+    int64_t last_num_paths = 0;
 };
 
 // the roadmap's search buffers, created on first use (nullptr when out of host memory)

@@ -562,6 +562,7 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
     res->num_paths = num_paths;
     res->ms_propagate = ms_prop;
     res->soln_path_idx = -1;
+    W->last_num_paths = num_paths;
 
     if (status == 0) {
         const size_t np = (size_t)num_paths;

@@ -1,20 +1,25 @@
 """Parity at the benchmark's full sizes.
 
-* NUM=4000 (10 boxes, 20 waypoints per edge, 1000 particles): the whole
-  pipeline against the CPU oracle, every table and the search result, in both
-  validity forms.
+* NUM=4000 (10 boxes, 20 waypoints per edge, 1000 particles) and NUM=16000
+  (30 boxes, W <= 64, 2000 particles): the whole pipeline against the CPU
+  oracle -- every pair of the all-pairs tables, rn, every edge, waypoint,
+  flag, half-space -- and the search: its result and its whole stored path
+  table against the oracle's search record.
 * Three of the 256 uneven batch instances (NUM 1000-8000, 5-50 boxes,
   random start/goal corners): the same, against the oracle.
-* NUM=16000 (30 boxes, W <= 64, 2000 particles), the NUM=32000 wall world
-  (200 thin boxes) and NUM=64000 (100 boxes, W <= 128, 8192 particles):
-  sampled rows of the pair tables against the oracle, the
-  neighbour radius as an exact order statistic of the full table, the edge
-  sets of the sampled rows, waypoints, flags and half-spaces of sampled edges
-  against the oracle, and the returned path re-derived (chain of free edges,
-  cost and time sums, full-MC CP and HSMC CP recomputed by the oracle).
-* Against the FP64 oracle (every returned path, NUM=4000 included): the same
-  node sequence sampled from FP64 arrival times in FP64, its full-MC CP with
-  the same noise draws within one MC standard error of the returned CP.
+* The NUM=32000 wall world (200 thin boxes, 4096 particles) and NUM=64000
+  (100 boxes, W <= 128, 8192 particles): sampled rows of the pair tables,
+  the neighbour radius as an exact order statistic of the full table, the
+  edge sets of the sampled rows; then the whole edge pass over the device's
+  edge set (every waypoint, flag and half-space) and the search on the
+  device's tables (result and stored path table) against the oracle; the
+  split-BF16 form of the search against the FP32 one (identical table, band
+  count reported).  NUM=32000 also runs the batched full MC on 64 candidate
+  paths against the oracle; NUM=64000 reports the cost error of the
+  FP64-polished arrival times against the FP64 oracle.
+* Against the FP64 oracle (every returned path): the same node sequence
+  sampled from FP64 arrival times in FP64, its full-MC CP with the same noise
+  draws within one MC standard error of the returned CP.
 """
 import os
 
@@ -52,14 +57,48 @@ def build(i, **over):
     return cfg, s, o, ot, xc, rm
 
 
-@pytest.mark.parametrize("bits", [0, 1])
-def test_full_num4000_parity(bits):
-    cfg, s, o, ot, xc, rm = build(0, validity_bits=bits)
+def gpu_tables(rm, e, f, hs):
+    """The device's roadmap arrays as the oracle search reads them."""
+    return dict(row_start=e["row_start"], edge_dst=e["edge_dst"], copts_edge=e["copts_edge"],
+                topts_edge=e["topts_edge"], h=e["h"], is_free_edges=f["is_free_edges"],
+                is_free_samples=f["is_free_samples"], wp_offsets=f["wp_offsets"],
+                as_=hs["as_"], bs=hs["bs"], max_hs=rm.info.max_hs, rn=np.float32(rm.info.rn),
+                n_edges=rm.info.n_edges)
+
+
+SEARCH_KEYS = ("status", "itrs", "num_paths", "num_considered", "num_goal_paths",
+               "soln_path_idx", "soln_nodes", "particle_steps", "clamped_time_idx", "mccp_calls")
+TABLE_KEYS = ("path_prev", "path_node", "path_cost", "path_time", "path_cp")
+
+
+def search_parity(rm, prm, tables, s, o, ot, xc):
+    """The device search against the oracle's search over the same tables:
+    every result field and every row of the stored path table, bit for bit."""
+    got = P.search_result_dict(rm.search(prm))
+    ref = OP.search(tables, s, s[-1], o, ot, xc, prm, nthreads=NT, path_cap=prm.path_cap,
+                    trace=True, wf_cap=0)
+    print("gpu", {k: got[k] for k in SEARCH_KEYS + ("soln_cost", "soln_cp", "band_particles")},
+          "\noracle", {k: ref[k] for k in SEARCH_KEYS + ("soln_cost", "soln_cp")}, flush=True)
+    for k in SEARCH_KEYS:
+        assert got[k] == ref[k], k
+    for k in ("soln_cost", "soln_cp", "soln_time", "soln_hsmc_cp"):
+        assert np.float32(got[k]) == np.float32(ref[k]), k
+    tab = rm.paths(got["num_paths"])
+    for k in TABLE_KEYS:
+        assert np.array_equal(tab[k], ref[k]), k
+    return got, tab
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("i,bits", [(0, 0), (0, 1), (1, 0)])
+def test_full_pipeline_parity(i, bits):
+    cfg, s, o, ot, xc, rm = build(i, validity_bits=bits)
     prm = cfg["params"]
     orc = OP.roadmap(s, o, s[-1], prm, nthreads=NT)
     c, t = rm.pairs()
     assert_close_f32(t, orc["topts"])
     assert_close_f32(c, orc["copts"])
+    del c, t
     assert rm.info.rn_idx == orc["rn_idx"] and np.float32(rm.info.rn) == orc["rn"]
     e = rm.edges()
     for k in ("row_start", "edge_src", "edge_dst"):
@@ -74,14 +113,9 @@ def test_full_num4000_parity(bits):
     assert np.array_equal(hs["count_stored"], orc["count_stored"])
     assert_close_f32(hs["bs"], orc["bs"])
     assert_close_f32(hs["as_"], orc["as_"])
-    got = P.search_result_dict(rm.search())
-    ref = OP.search(orc, s, s[-1], o, ot, xc, prm, nthreads=NT, path_cap=prm.path_cap)
-    print("gpu", got, "\noracle", ref, flush=True)
-    for k in ("status", "itrs", "num_paths", "num_considered", "num_goal_paths", "soln_path_idx",
-              "soln_nodes", "particle_steps", "clamped_time_idx"):
-        assert got[k] == ref[k], k
-    for k in ("soln_cost", "soln_cp", "soln_time", "soln_hsmc_cp"):
-        assert np.float32(got[k]) == np.float32(ref[k]), k
+    print(cfg["name"], "edges", rm.info.n_edges, "free", int(f["is_free_edges"].sum()), flush=True)
+    # the search on the oracle's own roadmap (equal to the device's above)
+    got, _ = search_parity(rm, prm, orc, s, o, ot, xc)
     if got["soln_len"] > 0:
         rederive_path(cfg, s, o, ot, xc, rm, got, e, f, hs)
     rm.close()
@@ -175,10 +209,91 @@ def rederive_path(cfg, s, o, ot, xc, rm, got, e, f, hs):
     assert np.float32(got["soln_hsmc_cp"]) == hcp
 
 
-@pytest.mark.parametrize("i", [1, 3, 4])
+def full_edge_pass(cfg, s, o, rm, e, f, hs):
+    """Every waypoint, flag and half-space of the device's edge set against
+    the oracle (findDiscretizedPath re-bisects each edge, as the reference's
+    host loop does; then freeEdges and cacheHalfspaces)."""
+    prm = cfg["params"]
+    num = s.shape[0]
+    wo = f["wp_offsets"]
+    E = rm.info.n_edges
+    W = np.diff(wo)
+    assert np.array_equal(W, OP.waypoint_counts(e["topts_edge"], prm.w_fixed, prm.wp_dt,
+                                                 prm.w_cap))
+    n_wp = int(wo[-1])
+    disc = np.zeros((n_wp, 6), np.float32)
+    O.po_discretize_edges(oracle.fp(s), oracle.lp(e["row_start"]), oracle.ip(e["edge_dst"]), num,
+                          oracle.lp(wo), oracle.fp(disc), NT)
+    assert_close_f32(rm.waypoints(), disc)
+    flags = np.zeros(E, np.uint8)
+    O.po_free_edges(oracle.fp(o), o.shape[0], oracle.fp(disc), oracle.lp(wo), E,
+                    oracle.bp(flags), NT)
+    assert np.array_equal(f["is_free_edges"], flags)
+    mh = rm.info.max_hs
+    as_ = np.zeros((n_wp * mh, 3), np.float32)
+    bs = np.zeros(n_wp * mh, np.float32)
+    cnt = np.zeros(n_wp, np.int32)
+    O.po_setup_as_bs(n_wp * mh, oracle.fp(bs), oracle.fp(as_))
+    O.po_cache_halfspaces(E, oracle.fp(disc), oracle.lp(wo), oracle.bp(flags), o.shape[0], mh,
+                          oracle.fp(o), oracle.fp(as_), oracle.fp(bs), oracle.ip(cnt),
+                          O.po_maxd2(np.float32(prm.offset_mult)), NT)
+    assert np.array_equal(hs["count_stored"], cnt)
+    assert_close_f32(hs["bs"], bs)
+    assert_close_f32(hs["as_"], as_)
+    print(cfg["name"], "edges", E, "free", int(flags.sum()), "waypoints", n_wp,
+          "stored half-spaces", int(cnt.sum()), flush=True)
+
+
+def mccp_candidates(cfg, s, o, xc, rm, tab, n=64):
+    """The batched full MC (pump_collision_probability_many) on n candidate
+    paths -- the deepest stored paths of the search, traced back to the init
+    and sampled by findOptimalPath -- against the oracle's per-path hits."""
+    prm = cfg["params"]
+    prev, node = tab["path_prev"], tab["path_node"]
+    depth = np.zeros(prev.size, np.int64)
+    for k in range(1, prev.size):
+        depth[k] = depth[prev[k]] + 1
+    pick = np.lexsort((np.arange(prev.size), -depth))[:n]
+    cap = prm.path_cap
+    paths = np.zeros((pick.size, cap, 6), np.float32)
+    T = np.zeros(pick.size, np.int32)
+    n_mc, tsteps = xc.shape[0], xc.shape[1]
+    ref = np.zeros(pick.size, np.int64)
+    for q, k in enumerate(pick.tolist()):
+        chain = []
+        while k != -2:
+            chain.append(int(node[k]))
+            k = int(prev[k])
+        xs = np.ascontiguousarray(s[np.asarray(chain[::-1])], np.float32)
+        ln = np.zeros(1, np.int32)
+        O.po_find_optimal_path(prm.dt, oracle.fp(paths[q]), oracle.fp(xs), xs.shape[0],
+                               oracle.ip(ln), cap)
+        assert ln[0] > 0
+        T[q] = ln[0]
+        hits = np.zeros(n_mc, np.int32)
+        ref[q] = O.po_mccp_hits(oracle.fp(o), o.shape[0], oracle.fp(xc), tsteps, n_mc,
+                                np.float32(prm.offset_mult), oracle.fp(paths[q]), int(ln[0]),
+                                oracle.ip(hits))
+    g_o, g_x = P.DeviceArray(o), P.DeviceArray(xc)
+    g_p, g_t = P.DeviceArray(paths), P.DeviceArray(T)
+    g_h = P.DeviceArray(shape=(pick.size,), dtype=np.int32)
+    P.check("pump_collision_probability_many", P.lib.pump_collision_probability_many(
+        g_o.p, o.shape[0], g_x.p, tsteps, n_mc, np.float32(prm.offset_mult), g_p.p, cap, g_t.p,
+        pick.size, g_h.p, 1, None))
+    got = g_h.numpy()
+    print(cfg["name"], f"full MC of {pick.size} candidate paths (depth {depth[pick].min()}-"
+          f"{depth[pick].max()}, {T.min()}-{T.max()} rows): hits {got.min()}-{got.max()} of "
+          f"{n_mc}", flush=True)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("i", [3, 4])
 def test_full_size_sampled(i):
-    # NUM=64000 keeps validity as bit masks: bytes would need 265 GB
-    cfg, s, o, ot, xc, rm = build(i, **({"validity_bits": 1} if i == 4 else {}))
+    # NUM=64000 keeps validity as bit masks (bytes would need 265 GB) and
+    # also builds the FP64-polished arrival times
+    over = {"validity_bits": 1, "polish": 1} if i == 4 else {}
+    cfg, s, o, ot, xc, rm = build(i, **over)
     prm = cfg["params"]
     num = s.shape[0]
     rng = np.random.default_rng(100 + i)
@@ -197,62 +312,57 @@ def test_full_size_sampled(i):
     assert rm.info.rn_idx == rn_idx
     rn = np.float32(rm.info.rn)
     assert np.count_nonzero(c < rn) <= rn_idx < np.count_nonzero(c <= rn)
-    # edge sets of the sampled rows: costs below rn, ascending destination
+    # edge sets of the sampled rows, from the oracle's rows: costs below rn,
+    # ascending destination
     e = rm.edges()
     assert e["row_start"][-1] == rm.info.n_edges == np.count_nonzero(c < rn)
     jj = np.arange(num - 1)
-    for r in rows.tolist():
-        seg = c[r * (num - 1):(r + 1) * (num - 1)]
-        sel = seg < rn
+    for k, r in enumerate(rows.tolist()):
+        sel = rc[k] < rn
         dst = (jj + (jj >= r))[sel]
         lo, hi = e["row_start"][r], e["row_start"][r + 1]
         assert np.array_equal(e["edge_dst"][lo:hi], dst)
-        assert np.array_equal(e["copts_edge"][lo:hi], seg[sel])
-        assert np.array_equal(e["topts_edge"][lo:hi], t[r * (num - 1):(r + 1) * (num - 1)][sel])
+        assert_close_f32(e["copts_edge"][lo:hi], rc[k][sel])
+        assert_close_f32(e["topts_edge"][lo:hi], rt[k][sel])
     del c, t
-    # waypoints, flags and half-spaces of sampled edges
+    if i == 4:
+        # FP64-polished arrival times of the sampled rows against the FP64
+        # oracle: cost errors of c64(polished tau), of the FP32 table, and of
+        # c64(bisected tau), each relative to max(|c64(tau*)|, 0.01)
+        pol = rm.polished()
+        tp = np.concatenate([pol[r * (num - 1):(r + 1) * (num - 1)] for r in rows.tolist()])
+        del pol
+        st = np.zeros(6, np.float64)
+        O.po64_rows_cost_error(oracle.fp(s), num, oracle.ip(rows), rows.size,
+                               np.float32(prm.tmax), oracle.fp(rt), oracle.fp(rc),
+                               oracle.dp(np.ascontiguousarray(tp)), oracle.dp(st), NT)
+        print(f"FP64 oracle over {int(st[5])} pairs: max relative cost error, polished tau "
+              f"{st[0]:.2e}, FP32 table {st[1]:.2e}, bisected tau {st[2]:.2e}; max relative "
+              f"tau error polished {st[3]:.2e}, bisected {st[4]:.2e}", flush=True)
+        assert st[0] <= 1e-6 and st[3] <= 1e-6
+        assert st[0] < st[2]
+    # every waypoint, flag and half-space of the device's edge set
     f = rm.flags()
-    wo = f["wp_offsets"]
-    eid = np.sort(rng.choice(rm.info.n_edges, 4000, replace=False))
-    src, dst = e["edge_src"][eid], e["edge_dst"][eid]
-    W = (wo[eid + 1] - wo[eid]).astype(np.int64)
-    for k in range(eid.size):
-        assert O.po_waypoints_for(e["topts_edge"][eid[k]], prm.w_fixed, np.float32(prm.wp_dt),
-                                  prm.w_cap) == W[k]
-    samples, row_start, edge_dst, off = sub_edges(s, src, dst, W)
-    disc = np.zeros((int(off[-1]), 6), np.float32)
-    O.po_discretize_edges(oracle.fp(samples), oracle.lp(row_start), oracle.ip(edge_dst),
-                          samples.shape[0], oracle.lp(off), oracle.fp(disc), NT)
-    gw = rm.waypoints()
-    take = np.concatenate([np.arange(wo[x], wo[x + 1]) for x in eid])
-    assert_close_f32(gw[take], disc)
-    flags = np.zeros(eid.size, np.uint8)
-    O.po_free_edges(oracle.fp(o), o.shape[0], oracle.fp(disc), oracle.lp(off), eid.size,
-                    oracle.bp(flags), NT)
-    assert np.array_equal(f["is_free_edges"][eid], flags)
-    assert flags.min() == 0 and flags.max() == 1
-    mh = rm.info.max_hs
-    n_wp = int(off[-1])
-    as_ = np.zeros((n_wp * mh, 3), np.float32)
-    bs = np.zeros(n_wp * mh, np.float32)
-    cnt = np.zeros(n_wp, np.int32)
-    O.po_setup_as_bs(n_wp * mh, oracle.fp(bs), oracle.fp(as_))
-    O.po_cache_halfspaces(eid.size, oracle.fp(disc), oracle.lp(off), oracle.bp(flags),
-                          o.shape[0], mh, oracle.fp(o), oracle.fp(as_), oracle.fp(bs),
-                          oracle.ip(cnt), O.po_maxd2(np.float32(prm.offset_mult)), NT)
     hs = rm.halfspaces()
-    assert np.array_equal(hs["count_stored"][take], cnt)
-    slots = (take[:, None] * mh + np.arange(mh)[None, :]).ravel()
-    assert_close_f32(hs["bs"][slots], bs)
-    assert_close_f32(hs["as_"][slots], as_)
-    assert cnt.max() > 0
-    # the search and its returned path
-    got = P.search_result_dict(rm.search())
-    print(cfg["name"], {k: got[k] for k in ("status", "itrs", "num_paths", "soln_cost",
-                                            "soln_cp", "soln_hsmc_cp")}, flush=True)
+    full_edge_pass(cfg, s, o, rm, e, f, hs)
+    # the search on the device's tables, against the oracle's search record
+    tables = gpu_tables(rm, e, f, hs)
+    got, tab = search_parity(rm, prm, tables, s, o, ot, xc)
     assert got["status"] in (0, 1)
     if got["soln_len"] > 0:
         rederive_path(cfg, s, o, ot, xc, rm, got, e, f, hs)
+    # split-BF16 half-space products: the same stored path table
+    p1 = configs.config(i, keep_pairs=1, **dict(over, validity_bits=1, precision=1))["params"]
+    got1 = P.search_result_dict(rm.search(p1))
+    tab1 = rm.paths(got1["num_paths"])
+    print(cfg["name"], "split-BF16 search: band re-tests", got1["band_particles"],
+          "of particle-steps", got1["particle_steps"], flush=True)
+    for k in SEARCH_KEYS:
+        assert got1[k] == got[k], k
+    for k in TABLE_KEYS:
+        assert np.array_equal(tab1[k], tab[k]), k
+    if i == 3:
+        mccp_candidates(cfg, s, o, xc, rm, tab)
     rm.close()
 
 

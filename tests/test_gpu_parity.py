@@ -490,6 +490,18 @@ def test_collision_probability_entry_points():
     assert np.float32(cp.value) == np.float32(np.float32(ref[1]) / np.float32(n_mc))
 
 
+def fixed_rows(samples, orc, W):
+    """W waypoints per edge (numDisc = W - 1), from the oracle's
+    findDiscretizedPath (2pBVP.cu:53-69) of every edge."""
+    E = orc["n_edges"]
+    off = np.arange(0, (E + 1) * W, W, dtype=np.int64)
+    rows = np.zeros((E * W, 6), np.float32)
+    O.po_discretize_edges(oracle.fp(samples), oracle.lp(orc["row_start"]),
+                          oracle.ip(orc["edge_dst"]), samples.shape[0], oracle.lp(off),
+                          oracle.fp(rows), 4)
+    return rows
+
+
 def test_free_edges_and_discretize_entry_points():
     cfg, s, o, ot, xc, rm, orc = build_pair(CASES[1])
     E, n_wp = orc["n_edges"], orc["n_wp"]
@@ -502,9 +514,25 @@ def test_free_edges_and_discretize_entry_points():
     assert_close_f32(host(disc), orc["disc"])
     gdisc = dev(orc["disc"])
     flags = dev(np.full(E, 9, np.uint8))
-    P.check("free_edges", P.lib.pump_free_edges(ptr(go), o.shape[0], ptr(gs), None, 0, ptr(goff),
-                                                ptr(gdisc), ptr(flags), E, None))
+    P.check("free_edges_var", P.lib.pump_free_edges_var(ptr(go), o.shape[0], ptr(goff),
+                                                        ptr(gdisc), ptr(flags), E, None))
     assert np.array_equal(host(flags), orc["is_free_edges"])
+    # freeEdges with the reference's own arguments (collisionCheck.cu:6-8):
+    # numDisc in its slot, fixed numDisc + 1 rows per edge (mainPUMP.cu:448-451)
+    for num_disc in (8, 1):
+        W = num_disc + 1
+        rows = fixed_rows(s, orc, W)
+        off = np.arange(0, (E + 1) * W, W, dtype=np.int64)
+        ref = np.zeros(E, np.uint8)
+        O.po_free_edges(oracle.fp(o), o.shape[0], oracle.fp(rows), oracle.lp(off), E,
+                        oracle.bp(ref), 4)
+        grows = dev(rows)
+        flags = dev(np.full(E, 9, np.uint8))
+        P.check("free_edges", P.lib.pump_free_edges(ptr(go), o.shape[0], ptr(gs), None, num_disc,
+                                                    ptr(grows), ptr(flags), E, None))
+        got = host(flags)
+        assert np.array_equal(got, ref), num_disc
+        assert 0 < got.sum() < E
     sf = dev(np.zeros(s.shape[0], np.uint8))
     P.check("sample_free", P.lib.pump_sample_free(ptr(go), o.shape[0], ptr(gs), s.shape[0],
                                                   ptr(sf), None))

@@ -139,6 +139,93 @@ def test_path_point_endpoints_and_velocity():
         assert np.allclose(vel_fd, out0[3:], atol=5e-3)
 
 
+def hermite64(t, tau, a, b):
+    """The double integrator's minimum-energy trajectory between states a and
+    b over [0, tau] in FP64: the cubic Hermite segment through (p0, v0) and
+    (p1, v1), position and velocity."""
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    p0, v0, p1, v1 = a[:3], a[3:], b[:3], b[3:]
+    c2 = 3 * (p1 - p0) / tau ** 2 - (2 * v0 + v1) / tau
+    c3 = 2 * (p0 - p1) / tau ** 3 + (v0 + v1) / tau ** 2
+    pos = p0 + v0 * t + c2 * t ** 2 + c3 * t ** 3
+    vel = v0 + 2 * c2 * t + 3 * c3 * t ** 2
+    return np.concatenate([pos, vel])
+
+
+def test_path_point_is_cubic_hermite():
+    """pathPoint (2pBVP.cu:117-131) evaluates the cubic Hermite segment: every
+    component against the FP64 closed form (a dropped term, a wrong sign or a
+    swapped coefficient of any of the six expressions fails)."""
+    rng = np.random.default_rng(41)
+    out = np.zeros(6, np.float32)
+    worst = 0.0
+    for _ in range(2000):
+        a, b = rand_state(rng), rand_state(rng)
+        tau = np.float32(rng.uniform(0.2, 4.5))
+        t = np.float32(rng.uniform(0, 1) * tau)
+        O.po_path_point(t, tau, fp(a), fp(b), fp(out))
+        ref = hermite64(float(t), float(tau), a, b)
+        worst = max(worst, float(np.max(np.abs(out - ref) / (1 + np.abs(ref)))))
+    assert worst < 2e-6, worst
+
+
+@pytest.mark.parametrize("num_disc", [1, 2, 8, 19])
+def test_find_discretized_path_spacing(num_disc):
+    """findDiscretizedPath (2pBVP.cu:53-69): row 0 and row numDisc are exact
+    copies of the end states, row i is the trajectory at i * (tau/numDisc)
+    with tau the bisected arrival time, which it returns."""
+    rng = np.random.default_rng(42 + num_disc)
+    wp = np.zeros((num_disc + 1, 6), np.float32)
+    for _ in range(300):
+        a, b = rand_state(rng), rand_state(rng)
+        tau = O.po_find_discretized_path(fp(wp), fp(a), fp(b), num_disc)
+        assert np.float32(tau) == np.float32(O.po_topt_bisection(fp(a), fp(b), np.float32(5)))
+        assert np.array_equal(wp[0].view(np.uint32), a.view(np.uint32))
+        assert np.array_equal(wp[num_disc].view(np.uint32), b.view(np.uint32))
+        step = float(np.float32(np.float32(tau) / np.float32(num_disc)))
+        for i in range(1, num_disc):
+            ref = hermite64(float(np.float32(step * i)), float(tau), a, b)
+            assert np.allclose(wp[i], ref, rtol=0, atol=5e-6 * (1 + np.abs(ref)).max()), i
+
+
+def test_find_optimal_path_samples_the_global_time_grid():
+    """findOptimalPath (2pBVP.cu:5-36): hop i gets T = (int)((tau - toff)/dt +
+    1) rows at local times toff + k*dt and carries toff = (t_last - tau) + dt,
+    so that row r of the whole path is the trajectory at global time r*dt and
+    there are floor(sum tau / dt) + 1 rows.  Checked against the FP64
+    trajectory of the chained hops; one row more or less in a hop, or a carry
+    without its + dt, shifts every later row by dt and fails."""
+    rng = np.random.default_rng(43)
+    cap = 4096
+    split = np.zeros((cap, 6), np.float32)
+    ln = np.zeros(1, np.int32)
+    checked = 0
+    for trial in range(60):
+        n = int(rng.integers(2, 7))
+        xs = np.stack([rand_state(rng) for _ in range(n)]).astype(np.float32)
+        dt = np.float32([0.05, 0.1, 0.0625][trial % 3])
+        taus = [float(O.po_topt_bisection(fp(xs[k]), fp(xs[k + 1]), np.float32(5)))
+                for k in range(n - 1)]
+        if min(taus) <= 2 * float(dt):
+            continue  # a hop shorter than its carry gets no rows (a quirk of the loop)
+        total = O.po_find_optimal_path(dt, fp(split), fp(xs), n, ip(ln), cap)
+        assert np.float32(total) == np.float32(sum(np.float32(t) for t in taus)) or \
+            abs(total - sum(taus)) < 1e-5
+        cum = np.concatenate([[0.0], np.cumsum(taus)])
+        frac = cum[-1] / float(dt) - np.floor(cum[-1] / float(dt))
+        if 0.01 < frac < 0.99:
+            assert ln[0] == int(np.floor(cum[-1] / float(dt))) + 1, (ln[0], cum[-1])
+        for r in range(ln[0]):
+            g = r * float(dt)
+            k = min(int(np.searchsorted(cum, g, side="right")) - 1, n - 2)
+            ref = hermite64(g - cum[k], taus[k], xs[k], xs[k + 1])
+            assert np.allclose(split[r], ref, rtol=0, atol=2e-5 * (1 + np.abs(ref)).max()), \
+                (trial, r, k)
+        checked += 1
+    assert checked >= 40
+
+
 # ----------------------------------------------------- collision checks ----
 def box(lo, hi):
     o = np.zeros(12, np.float32)
