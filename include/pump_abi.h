@@ -265,6 +265,25 @@ size_t pump_split_offsets_bytes(int tmax_steps, int n_mc);
 int pump_split_offsets(const float *d_offsets_time, int tmax_steps, int n_mc, void *d_split,
                        pump_stream_t stream);
 
+/* pump_propagate_cp_bits without a workspace argument and without any
+ * device allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The batched form's workspace is a
+ * fixed 64 MiB device array held by each GPU the library runs on until the
+ * process ends; a wavefront whose pump_propagate_workspace(num_new_paths,
+ * tmax_steps) exceeds it (566,564 paths at 2001 time indices) returns
+ * PUMP_E_CAPACITY before anything runs.  All calls on one GPU share that
+ * GPU's array: calls that overlap on one GPU race (calls on one stream are
+ * ordered).  Same results as pump_propagate_cp_bits. */
+int pump_propagate_cp_bits_noalloc(
+    int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets, const int *d_wf_edge,
+    const int *d_wf_prev, const float *d_as, const float *d_bs, const uint64_t *d_edge_hs_mask,
+    const float *d_times, const float *d_topts, float dt, int max_hs,
+    const float *d_offsets_time, int tmax_steps, float offset_mult, int num_mc_particles,
+    const uint64_t *d_valid_src, uint64_t *d_valid_dst, int64_t dst_base, int n_p,
+    float *d_path_cp, int64_t cp_base, int precision, const void *d_offsets_split,
+    unsigned long long *d_band_count, unsigned long long *d_clamped, pump_stream_t stream);
+
 /* Initial bit masks for pump_propagate_cp_bits: all n_p particles valid in
  * each of n_paths rows (the bit form of setupValidParticles). */
 int pump_setup_valid_bits(uint64_t *d_valid, int n_p, int64_t n_paths, pump_stream_t stream);

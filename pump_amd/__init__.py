@@ -177,6 +177,9 @@ _SIGS = {
     "pump_propagate_cp_bits": (c_int, [c_int, c_int, S, S, S, S, S, S, S, S, c_float, c_int, S,
                                        c_int, c_float, c_int, S, S, c_int64, c_int, S, c_int64,
                                        c_int, S, S, S, S, c_size_t, S]),
+    "pump_propagate_cp_bits_noalloc": (c_int, [c_int, c_int, S, S, S, S, S, S, S, S, c_float,
+                                               c_int, S, c_int, c_float, c_int, S, S, c_int64,
+                                               c_int, S, c_int64, c_int, S, S, S, S]),
     "pump_split_offsets_bytes": (c_size_t, [c_int, c_int]),
     "pump_split_offsets": (c_int, [S, c_int, c_int, S, S]),
     "pump_setup_valid_bits": (c_int, [S, c_int, c_int64, S]),

@@ -36,6 +36,7 @@ SOURCES = [
     "problem_io.hip",
     "gmt.hip",
     "runtime.hip",
+    "noalloc.hip",
 ]
 
 ARCH = os.environ.get("PUMP_OFFLOAD_ARCH", "gfx950")

@@ -51,6 +51,16 @@ def test_invalid_arguments_rejected_without_gpu():
     assert P.lib.pump_gmt_workspace(0) == 0
 
 
+def test_noalloc_rejects_oversize_wavefront_without_gpu():
+    # a wavefront whose batched workspace exceeds the fixed 64 MiB device array
+    # is rejected before any device call
+    assert P.lib.pump_propagate_workspace(566564, 2001) <= 64 << 20
+    assert P.lib.pump_propagate_workspace(566565, 2001) > 64 << 20
+    assert P.lib.pump_propagate_cp_bits_noalloc(
+        566565, 9, None, None, None, None, None, None, None, None, 0.05, 5, None, 2001, 1.0, 64,
+        None, None, 0, 64, None, 0, 0, None, None, None, None) == -3
+
+
 def test_philox_known_answers():
     # Random123 known-answer vectors for philox4x32-10
     assert P.philox4x32([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
