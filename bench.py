@@ -305,10 +305,34 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
             bdst2.p, 0, n_p, cp_split.p, 0, 1, d.offsets_split, band.p, None, g_ws.p, ws_bytes,
             stream.h))
 
+    # copy-on-write forms: children that meet no half-space keep their
+    # parent's row and CP (parents all valid: CP 0); only the listed children's
+    # rows are written, the rows a caller keeps are gathered afterwards
+    g_pcp = P.DeviceArray(shape=(n_par,), dtype=np.float32, fill=0)
+    row_src = P.DeviceArray(shape=(n_new,), dtype=np.int32, fill=0)
+    cdst = P.DeviceArray(shape=(n_new, nw), dtype=np.uint64, fill=0)
+    cp_cow = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
+    cp_cow_split = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
+    cdst_b = P.DeviceArray(shape=(n_new, n_p), dtype=np.uint8, fill=0)
+    cp_cow_b = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
+
+    def cow(bits, precision, srcp, dstp, cpp):
+        P.check("pump_propagate_cp_cow", P.lib.pump_propagate_cp_cow(
+            bits, n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, d.edge_hs_mask,
+            g_times.p, d.topts_edge, prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult,
+            n_mc, srcp, g_pcp.p, dstp, 0, row_src.p, n_p, cpp, 0, precision,
+            d.offsets_split if precision else None, None, None, g_ws.p, ws_bytes, stream.h))
+
     forms = (("dropin_fp32_bytes", run_dropin, lambda: cp_in.numpy()[n_par:]),
              ("batched_fp32_bytes", run_bytes, lambda: cp_bytes.numpy()),
              ("batched_fp32_bits", run_bits, lambda: g_cp.numpy()),
-             ("batched_split_bf16_bits", run_split, lambda: cp_split.numpy()))
+             ("batched_split_bf16_bits", run_split, lambda: cp_split.numpy()),
+             ("cow_fp32_bytes", lambda: cow(0, 0, src.p, cdst_b.p, cp_cow_b.p),
+              lambda: cp_cow_b.numpy()),
+             ("cow_fp32_bits", lambda: cow(1, 0, bsrc.p, cdst.p, cp_cow.p),
+              lambda: cp_cow.numpy()),
+             ("cow_split_bf16_bits", lambda: cow(1, 1, bsrc.p, cdst.p, cp_cow_split.p),
+              lambda: cp_cow_split.numpy()))
     for name, fn, result in forms:
         fn()
         stream.sync()
@@ -323,6 +347,24 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
         log(f"hsmc {name}: {ms:.3f} ms, {steps / (ms / 1e3):.3e} particle-steps/s")
     ref = cps["dropin_fp32_bytes"]
     out["forms_agree"] = bool(all(np.array_equal(ref, v) for v in cps.values()))
+    # the copy-on-write rows: every child gathered (the upper bound of what a
+    # search keeps) must equal the bit form's rows; the gather is timed too
+    full = P.DeviceArray(shape=(n_new, nw), dtype=np.uint64, fill=0)
+
+    def gather_all():
+        P.check("pump_gather_rows_cow", P.lib.pump_gather_rows_cow(
+            cdst.p, bsrc.p, row_src.p, None, n_new, full.p, 0, nw * 8, 1, stream.h))
+
+    cow(1, 0, bsrc.p, cdst.p, cp_cow.p)
+    gather_all()
+    stream.sync()
+    out["cow_rows_agree"] = bool(np.array_equal(full.numpy(), bdst.numpy()))
+    timer.start()
+    for _ in range(reps):
+        gather_all()
+    out["cow_gather_all_bits_ms"] = timer.stop() / reps
+    rsrc = row_src.numpy()
+    out["cow_rows_written"] = int((rsrc >= 0).sum())
     out["mean_cp"] = float(ref.mean())
     band.upload(np.zeros(1, np.uint64))
     run_split()
@@ -332,7 +374,12 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
                                    / out["batched_split_bf16_bits"]["ms"])
     out["fp32_bits_vs_dropin"] = (out["dropin_fp32_bytes"]["ms"]
                                   / out["batched_fp32_bits"]["ms"])
-    best = min(("batched_fp32_bits", "batched_split_bf16_bits"), key=lambda k: out[k]["ms"])
+    out["cow_fp32_bits_vs_dropin"] = (out["dropin_fp32_bytes"]["ms"]
+                                      / out["cow_fp32_bits"]["ms"])
+    out["cow_split_bf16_vs_dropin"] = (out["dropin_fp32_bytes"]["ms"]
+                                       / out["cow_split_bf16_bits"]["ms"])
+    best = min(("batched_fp32_bits", "batched_split_bf16_bits", "cow_fp32_bits",
+                "cow_split_bf16_bits"), key=lambda k: out[k]["ms"])
     out["best_form"] = best
     out["value"] = out[best]["particle_steps_per_s"]
     out["unit"] = "particle-steps/s"

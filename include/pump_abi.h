@@ -265,6 +265,35 @@ size_t pump_split_offsets_bytes(int tmax_steps, int n_mc);
 int pump_split_offsets(const float *d_offsets_time, int tmax_steps, int n_mc, void *d_split,
                        pump_stream_t stream);
 
+/* The batched step with copy-on-write child rows (byte or bit validity):
+ * the rows of new paths that meet no stored half-space equal their parents'
+ * rows and are not written.  d_row_src[w] = w when child w's row was written
+ * (row dst_base + w of d_valid_dst), -1 - prev when it is parent prev's row
+ * (of d_valid_src); such a child's CP is d_parent_cp[prev] (the same count,
+ * the same float).  Other arguments as pump_propagate_cp(_bits); precision 1
+ * needs bit validity.  The rows a caller keeps are materialised with
+ * pump_gather_rows_cow -- the reference copies its survivors' rows at that
+ * point anyway (PUMP.cu:236-237). */
+int pump_propagate_cp_cow(int validity_bits, int num_new_paths, int waypoints_count,
+                          const int64_t *d_wp_offsets, const int *d_wf_edge, const int *d_wf_prev,
+                          const float *d_as, const float *d_bs, const uint64_t *d_edge_hs_mask,
+                          const float *d_times, const float *d_topts, float dt, int max_hs,
+                          const float *d_offsets_time, int tmax_steps, float offset_mult,
+                          int num_mc_particles, const void *d_valid_src, const float *d_parent_cp,
+                          void *d_valid_dst, int64_t dst_base, int *d_row_src, int n_p,
+                          float *d_path_cp, int64_t cp_base, int precision,
+                          const void *d_offsets_split, unsigned long long *d_band_count,
+                          unsigned long long *d_clamped, void *d_workspace,
+                          size_t workspace_bytes, pump_stream_t stream);
+
+/* Child sel[j] (j < n_sel; sel NULL: j itself) of a copy-on-write step to
+ * row dst_base + j of d_dst, from d_children (rows relative to the step's
+ * dst_base) or d_parents as d_row_src says; byte rows (validity_bits 0) are
+ * normalised to 0/1.  Parent rows must not lie in the destination rows. */
+int pump_gather_rows_cow(const void *d_children, const void *d_parents, const int *d_row_src,
+                         const int *d_sel, int n_sel, void *d_dst, int64_t dst_base,
+                         int64_t row_bytes, int validity_bits, pump_stream_t stream);
+
 /* pump_propagate_cp_bits without a workspace argument and without any
  * device allocation.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving

@@ -101,6 +101,13 @@ typedef struct pump_noise_spec {
 void pump_noise_defaults(pump_noise_spec *spec);
 int pump_make_noise(const pump_noise_spec *spec, float *offsets_time_host, float *xcomb_host,
                     float *sigma_pos_host);
+/* The same tables generated on the device (stream-ordered): d_chol [36]
+ * receives the FP64 Cholesky factor L of Q (required, it also carries L to
+ * the particle kernel); d_offsets_time / d_xcomb / d_sigma_pos as above,
+ * NULL to skip.  L and sigma_pos are bit-identical to the host's; the tables
+ * differ only where the device's FP64 log/sin/cos round differently. */
+int pump_make_noise_device(const pump_noise_spec *spec, float *d_offsets_time, float *d_xcomb,
+                           float *d_sigma_pos, double *d_chol, pump_stream_t stream);
 
 /* ---- roadmap precompute + search --------------------------------------- */
 typedef struct pump_params {

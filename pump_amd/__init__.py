@@ -180,6 +180,10 @@ _SIGS = {
     "pump_propagate_cp_bits_noalloc": (c_int, [c_int, c_int, S, S, S, S, S, S, S, S, c_float,
                                                c_int, S, c_int, c_float, c_int, S, S, c_int64,
                                                c_int, S, c_int64, c_int, S, S, S, S]),
+    "pump_propagate_cp_cow": (c_int, [c_int, c_int, c_int, S, S, S, S, S, S, S, S, c_float,
+                                      c_int, S, c_int, c_float, c_int, S, S, S, c_int64, S,
+                                      c_int, S, c_int64, c_int, S, S, S, S, c_size_t, S]),
+    "pump_gather_rows_cow": (c_int, [S, S, S, S, c_int, S, c_int64, c_int64, c_int, S]),
     "pump_split_offsets_bytes": (c_size_t, [c_int, c_int]),
     "pump_split_offsets": (c_int, [S, c_int, c_int, S, S]),
     "pump_setup_valid_bits": (c_int, [S, c_int, c_int64, S]),
@@ -196,6 +200,7 @@ _SIGS = {
     "pump_gen_problem": (c_int, [ctypes.POINTER(ProblemSpec), f32p, f32p]),
     "pump_noise_defaults": (None, [ctypes.POINTER(NoiseSpec)]),
     "pump_make_noise": (c_int, [ctypes.POINTER(NoiseSpec), f32p, f32p, f32p]),
+    "pump_make_noise_device": (c_int, [ctypes.POINTER(NoiseSpec), S, S, S, S, S]),
     "pump_params_defaults": (None, [ctypes.POINTER(Params)]),
     "pump_roadmap_build": (c_int, [f32p, c_int, f32p, c_int, f32p, f32p, f32p, c_int, c_int,
                                    ctypes.POINTER(Params), ctypes.POINTER(c_void_p)]),
@@ -400,6 +405,19 @@ def make_noise(ns, sigma=False):
     sg = np.zeros((ns.tmax_steps, 3, 3), np.float32) if sigma else None
     check("pump_make_noise", lib.pump_make_noise(ctypes.byref(ns), fp(ot), fp(xc), fp(sg)))
     return (ot, xc, sg) if sigma else (ot, xc)
+
+
+def make_noise_device(ns):
+    """The noise tables generated on the device: (offsets_time [T, n_mc, 3],
+    xcomb [n_mc, T, 3], sigma_pos [T, 3, 3], L [6, 6] FP64 Cholesky of Q)."""
+    ot = DeviceArray(shape=(ns.tmax_steps, ns.n_mc, 3), dtype=np.float32)
+    xc = DeviceArray(shape=(ns.n_mc, ns.tmax_steps, 3), dtype=np.float32)
+    sg = DeviceArray(shape=(ns.tmax_steps, 3, 3), dtype=np.float32)
+    ch = DeviceArray(shape=(6, 6), dtype=np.float64)
+    check("pump_make_noise_device", lib.pump_make_noise_device(
+        ctypes.byref(ns), ot.p, xc.p, sg.p, ch.p, None))
+    sync()
+    return ot.numpy(), xc.numpy(), sg.numpy(), ch.numpy()
 
 
 def params(**kw):

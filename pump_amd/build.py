@@ -37,6 +37,7 @@ SOURCES = [
     "gmt.hip",
     "runtime.hip",
     "noalloc.hip",
+    "noise.hip",
 ]
 
 ARCH = os.environ.get("PUMP_OFFLOAD_ARCH", "gfx950")

@@ -546,6 +546,136 @@ __global__ __launch_bounds__(256) void k_propagate_bits(
 // waypoints' half-spaces, whose data are wave-uniform loads.  Decisions use
 // the reference's float order.  No LDS, no workgroup barriers.
 // One wave per new path (single launch; small wavefronts).
+// One path's propagate + CP by one wave (the body of the wave-per-path
+// forms): marked waypoints from the edge's summary or from bs, particles 64 at
+// a time, the child row written in full, the CP from its final count.
+template <bool kBits>
+__device__ __forceinline__ void propagate_path(
+    int64_t w, int e, int prev, const PathInfo &pi, const float *__restrict__ as,
+    const float *__restrict__ bs, const uint64_t *__restrict__ edge_mask, float dt, int max_hs,
+    const float *__restrict__ offsets_time, int tmax_steps, float om, int n_mc,
+    const void *__restrict__ valid_src, void *__restrict__ valid_dst, int64_t dst_base, int n_p,
+    float *__restrict__ path_cp, int64_t cp_base, unsigned long long *__restrict__ clamped)
+{
+    // eight particle groups per round: a path's particles in few rounds of
+    // memory latency
+    constexpr int NQ = 2 * kWalkGroups;
+    const int lane = threadIdx.x & 63;
+    const uint64_t emask = edge_mask ? edge_mask[e] : 0ull;
+    int count = 0; // valid particles (bits) or invalid particles (bytes), per lane
+    // waypoints in groups of 256; later groups continue from the child row
+    for (int i0 = 0; i0 < pi.W; i0 += 256) {
+        uint64_t masks[4];
+        const uint64_t any = hs_masks(masks, pi, i0, edge_mask != nullptr, emask, max_hs, bs,
+                                      dt, tmax_steps, clamped);
+        const bool last = i0 + 256 >= pi.W;
+        if (kBits) {
+            const int nwords = (n_p + 63) >> 6;
+            const uint64_t *src =
+                i0 == 0 ? (const uint64_t *)valid_src + (int64_t)prev * nwords
+                        : (const uint64_t *)valid_dst + (dst_base + w) * (int64_t)nwords;
+            uint64_t *dst = (uint64_t *)valid_dst + (dst_base + w) * (int64_t)nwords;
+            if (!any) {
+                for (int wd = lane; wd < nwords; wd += 64) {
+                    const uint64_t x = src[wd];
+                    if (i0 == 0)
+                        dst[wd] = x;
+                    if (last)
+                        count += __popcll(x);
+                }
+            } else {
+                for (int wd0 = 0; wd0 < nwords; wd0 += NQ) {
+                    bool v[NQ];
+                    int p[NQ];
+                    uint64_t alive = 0;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        const int wd = wd0 + q;
+                        p[q] = wd * 64 + lane;
+                        const uint64_t word = wd < nwords ? src[wd] : 0ull;
+                        v[q] = (p[q] < n_p) && ((word >> lane) & 1ull);
+                        alive |= __ballot(v[q]);
+                    }
+                    if (alive)
+                        particle_walk<NQ>(v, p, n_p, masks, i0, pi, max_hs, as, bs,
+                                          offsets_time, n_mc, tmax_steps, dt, om);
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        const uint64_t out = __ballot(v[q]);
+                        if (lane == q && wd0 + q < nwords) {
+                            dst[wd0 + q] = out;
+                            if (last)
+                                count += __popcll(out);
+                        }
+                    }
+                }
+            }
+        } else {
+            const uint8_t *src = i0 == 0
+                                     ? (const uint8_t *)valid_src + (int64_t)prev * n_p
+                                     : (const uint8_t *)valid_dst + (dst_base + w) * (int64_t)n_p;
+            uint8_t *dst = (uint8_t *)valid_dst + (dst_base + w) * (int64_t)n_p;
+            const bool words = ((n_p & 3) == 0) &&
+                               ((((uintptr_t)src) | ((uintptr_t)dst)) & 3) == 0;
+            const bool wide = ((n_p & 15) == 0) &&
+                              ((((uintptr_t)src) | ((uintptr_t)dst)) & 15) == 0;
+            if (!any && wide) {
+                const uint4 *s16 = (const uint4 *)src;
+                uint4 *d16 = (uint4 *)dst;
+                for (int q = lane; q < (n_p >> 4); q += 64) {
+                    uint4 x = s16[q];
+                    x.x = nonzero_bytes01(x.x);
+                    x.y = nonzero_bytes01(x.y);
+                    x.z = nonzero_bytes01(x.z);
+                    x.w = nonzero_bytes01(x.w);
+                    d16[q] = x;
+                    if (last)
+                        count += 16 - __popc(x.x) - __popc(x.y) - __popc(x.z) - __popc(x.w);
+                }
+            } else if (!any && words) {
+                const uint32_t *s4 = (const uint32_t *)src;
+                uint32_t *d4 = (uint32_t *)dst;
+                const int nq = n_p >> 2;
+                for (int q = lane; q < nq; q += 64) {
+                    const uint32_t x = nonzero_bytes01(s4[q]);
+                    d4[q] = x;
+                    if (last)
+                        count += 4 - __popc(x);
+                }
+            } else {
+                for (int p0 = 0; p0 < n_p; p0 += 64 * NQ) {
+                    bool v[NQ];
+                    int p[NQ];
+                    uint64_t alive = 0;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        p[q] = p0 + q * 64 + lane;
+                        v[q] = (p[q] < n_p) && src[p[q]] != 0;
+                        alive |= __ballot(v[q]);
+                    }
+                    if (any && alive)
+                        particle_walk<NQ>(v, p, n_p, masks, i0, pi, max_hs, as, bs,
+                                          offsets_time, n_mc, tmax_steps, dt, om);
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        if (p[q] < n_p) {
+                            dst[p[q]] = v[q] ? 1 : 0;
+                            if (last)
+                                count += v[q] ? 0 : 1;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    count = wave_sum(count);
+    if (path_cp && lane == 0) {
+        const int invalid = kBits ? n_p - count : count;
+        path_cp[cp_base + w] = ((float)invalid) / (float)n_p;
+    }
+}
+
+// One wave per new path (single launch; small wavefronts).
 template <bool kBits>
 __global__ __launch_bounds__(256) void k_propagate_wave(
     int n_new, const int64_t *__restrict__ wp_offsets, int w_fixed,
@@ -557,125 +687,14 @@ __global__ __launch_bounds__(256) void k_propagate_wave(
     void *__restrict__ valid_dst, int64_t dst_base, int n_p, float *__restrict__ path_cp,
     int64_t cp_base, unsigned long long *__restrict__ clamped)
 {
-    constexpr int NQ = kWalkGroups;
-    const int lane = threadIdx.x & 63;
     const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
     for (int64_t w = wave_index(); w < n_new; w += n_waves) {
         const int e = wf_edge[w];
         const int prev = wf_prev[w];
         const PathInfo pi = path_info(e, prev, wp_offsets, w_fixed, topts, times);
-        const uint64_t emask = edge_mask ? edge_mask[e] : 0ull;
-        int count = 0; // valid particles (bits) or invalid particles (bytes), per lane
-        // waypoints in groups of 256; later groups continue from the child row
-        for (int i0 = 0; i0 < pi.W; i0 += 256) {
-            uint64_t masks[4];
-            const uint64_t any = hs_masks(masks, pi, i0, edge_mask != nullptr, emask, max_hs, bs,
-                                          dt, tmax_steps, clamped);
-            const bool last = i0 + 256 >= pi.W;
-            if (kBits) {
-                const int nwords = (n_p + 63) >> 6;
-                const uint64_t *src =
-                    i0 == 0 ? (const uint64_t *)valid_src + (int64_t)prev * nwords
-                            : (const uint64_t *)valid_dst + (dst_base + w) * (int64_t)nwords;
-                uint64_t *dst = (uint64_t *)valid_dst + (dst_base + w) * (int64_t)nwords;
-                if (!any) {
-                    for (int wd = lane; wd < nwords; wd += 64) {
-                        const uint64_t x = src[wd];
-                        if (i0 == 0)
-                            dst[wd] = x;
-                        if (last)
-                            count += __popcll(x);
-                    }
-                } else {
-                    for (int wd0 = 0; wd0 < nwords; wd0 += NQ) {
-                        bool v[NQ];
-                        int p[NQ];
-                        uint64_t alive = 0;
-#pragma unroll
-                        for (int q = 0; q < NQ; ++q) {
-                            const int wd = wd0 + q;
-                            p[q] = wd * 64 + lane;
-                            const uint64_t word = wd < nwords ? src[wd] : 0ull;
-                            v[q] = (p[q] < n_p) && ((word >> lane) & 1ull);
-                            alive |= __ballot(v[q]);
-                        }
-                        if (alive)
-                            particle_walk<NQ>(v, p, n_p, masks, i0, pi, max_hs, as, bs,
-                                              offsets_time, n_mc, tmax_steps, dt, om);
-#pragma unroll
-                        for (int q = 0; q < NQ; ++q) {
-                            const uint64_t out = __ballot(v[q]);
-                            if (lane == q && wd0 + q < nwords) {
-                                dst[wd0 + q] = out;
-                                if (last)
-                                    count += __popcll(out);
-                            }
-                        }
-                    }
-                }
-            } else {
-                const uint8_t *src = i0 == 0
-                                         ? (const uint8_t *)valid_src + (int64_t)prev * n_p
-                                         : (const uint8_t *)valid_dst + (dst_base + w) * (int64_t)n_p;
-                uint8_t *dst = (uint8_t *)valid_dst + (dst_base + w) * (int64_t)n_p;
-                const bool words = ((n_p & 3) == 0) &&
-                                   ((((uintptr_t)src) | ((uintptr_t)dst)) & 3) == 0;
-                const bool wide = ((n_p & 15) == 0) &&
-                                  ((((uintptr_t)src) | ((uintptr_t)dst)) & 15) == 0;
-                if (!any && wide) {
-                    const uint4 *s16 = (const uint4 *)src;
-                    uint4 *d16 = (uint4 *)dst;
-                    for (int q = lane; q < (n_p >> 4); q += 64) {
-                        uint4 x = s16[q];
-                        x.x = nonzero_bytes01(x.x);
-                        x.y = nonzero_bytes01(x.y);
-                        x.z = nonzero_bytes01(x.z);
-                        x.w = nonzero_bytes01(x.w);
-                        d16[q] = x;
-                        if (last)
-                            count += 16 - __popc(x.x) - __popc(x.y) - __popc(x.z) - __popc(x.w);
-                    }
-                } else if (!any && words) {
-                    const uint32_t *s4 = (const uint32_t *)src;
-                    uint32_t *d4 = (uint32_t *)dst;
-                    const int nq = n_p >> 2;
-                    for (int q = lane; q < nq; q += 64) {
-                        const uint32_t x = nonzero_bytes01(s4[q]);
-                        d4[q] = x;
-                        if (last)
-                            count += 4 - __popc(x);
-                    }
-                } else {
-                    for (int p0 = 0; p0 < n_p; p0 += 64 * NQ) {
-                        bool v[NQ];
-                        int p[NQ];
-                        uint64_t alive = 0;
-#pragma unroll
-                        for (int q = 0; q < NQ; ++q) {
-                            p[q] = p0 + q * 64 + lane;
-                            v[q] = (p[q] < n_p) && src[p[q]] != 0;
-                            alive |= __ballot(v[q]);
-                        }
-                        if (any && alive)
-                            particle_walk<NQ>(v, p, n_p, masks, i0, pi, max_hs, as, bs,
-                                              offsets_time, n_mc, tmax_steps, dt, om);
-#pragma unroll
-                        for (int q = 0; q < NQ; ++q) {
-                            if (p[q] < n_p) {
-                                dst[p[q]] = v[q] ? 1 : 0;
-                                if (last)
-                                    count += v[q] ? 0 : 1;
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        count = wave_sum(count);
-        if (path_cp && lane == 0) {
-            const int invalid = kBits ? n_p - count : count;
-            path_cp[cp_base + w] = ((float)invalid) / (float)n_p;
-        }
+        propagate_path<kBits>(w, e, prev, pi, as, bs, edge_mask, dt, max_hs, offsets_time,
+                              tmax_steps, om, n_mc, valid_src, valid_dst, dst_base, n_p, path_cp,
+                              cp_base, clamped);
     }
 }
 
@@ -701,7 +720,8 @@ __global__ __launch_bounds__(kClassifyBlock) void k_classify(
     const float *__restrict__ times, const float *__restrict__ topts, float dt, int tmax_steps,
     uint8_t *__restrict__ has_hs, HsPath *__restrict__ recs, int *__restrict__ ctl,
     int *__restrict__ hist, unsigned long long *__restrict__ acc,
-    unsigned long long *__restrict__ clamped)
+    unsigned long long *__restrict__ clamped, int *__restrict__ row_src,
+    const float *__restrict__ parent_cp, float *__restrict__ path_cp, int64_t cp_base)
 {
     __shared__ int wcount[kClassifyBlock / 64];
     __shared__ int block_base;
@@ -737,6 +757,18 @@ __global__ __launch_bounds__(kClassifyBlock) void k_classify(
                 r.pad = 0;
             }
             has_hs[w] = has ? 1 : 0;
+            // copy-on-write: a path that meets no half-space keeps its
+            // parent's row and CP (the same count, the same float)
+            if (row_src) {
+                if (has) {
+                    row_src[w] = (int)w;
+                } else {
+                    const int prev = wf_prev[w];
+                    row_src[w] = -1 - prev;
+                    if (path_cp)
+                        path_cp[cp_base + w] = parent_cp[prev];
+                }
+            }
         }
         if (n_clamped && clamped)
             atomicAdd(clamped, (unsigned long long)n_clamped);
@@ -881,6 +913,215 @@ __global__ __launch_bounds__(256) void k_copy_rows(int n_new, const int *__restr
     }
 }
 
+// 64 new paths per wave iteration: lane j loads its path's edge data and
+// finds whether it meets a stored half-space (the edge's summary, or its
+// slots' b); the rows of the paths that meet none are copied eight at a time
+// by the whole wave (their loads in flight together), and only the others
+// run the per-path body.  The same rows and CPs as one wave per path,
+// without a round of dependent loads per path.
+template <bool kBits>
+__global__ __launch_bounds__(256) void k_propagate_b64(
+    int n_new, const int64_t *__restrict__ wp_offsets, int w_fixed,
+    const int *__restrict__ wf_edge, const int *__restrict__ wf_prev,
+    const float *__restrict__ as, const float *__restrict__ bs,
+    const uint64_t *__restrict__ edge_mask, const float *__restrict__ times,
+    const float *__restrict__ topts, float dt, int max_hs, const float *__restrict__ offsets_time,
+    int tmax_steps, float om, int n_mc, const void *__restrict__ valid_src,
+    void *__restrict__ valid_dst, int64_t dst_base, int n_p, float *__restrict__ path_cp,
+    int64_t cp_base, unsigned long long *__restrict__ clamped, int unit)
+{
+    constexpr int R = 8;
+    const int lane = threadIdx.x & 63;
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int64_t row_bytes = kBits ? (int64_t)((n_p + 63) >> 6) * 8 : (int64_t)n_p;
+    const int units = (int)(row_bytes / unit);
+    // wave b takes paths b, b + nb, b + 2 nb, ...: the paths that meet a
+    // half-space come in runs (edges near boxes), and a stride spreads each
+    // run over many waves instead of queueing it on one
+    const int64_t nb = (n_new + 63) / 64;
+    for (int64_t bt = wave_index(); bt < nb; bt += n_waves) {
+        const int64_t w = bt + (int64_t)lane * nb;
+        const bool act = w < n_new;
+        int e = 0, prev = 0;
+        PathInfo pi{0, 2, 0.0f, 0.0f};
+        bool has = false;
+        if (act) {
+            e = wf_edge[w];
+            prev = wf_prev[w];
+            pi = path_info(e, prev, wp_offsets, w_fixed, topts, times);
+            if (edge_mask) {
+                has = mask_bits(edge_mask[e], pi.W) != 0ull;
+            } else {
+                // the edge's slots are contiguous: 16 independent loads per
+                // round, so a long edge costs a few rounds of latency
+                const int64_t n_slots = (int64_t)pi.W * max_hs;
+                const float *b0 = bs + pi.g0 * (int64_t)max_hs;
+                for (int64_t c = 0; c < n_slots && !has; c += 16) {
+                    float b[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        b[u] = c + u < n_slots ? b0[c + u] : 1111.0f;
+                    bool any = false;
+#pragma unroll
+                    for (int u = 0; u < 16; ++u)
+                        any |= !(b[u] >= 1000.0f);
+                    has = any;
+                }
+            }
+        }
+        // rows of the paths that meet no half-space: copies of their parents
+        uint64_t m = __ballot(act && !has);
+        while (m) {
+            int l[R];
+            for (int r = 0; r < R; ++r) {
+                l[r] = m ? __builtin_ctzll(m) : -1;
+                m &= m ? m - 1 : 0ull;
+            }
+            const uint8_t *sp[R];
+            uint8_t *dp[R];
+            int count[R];
+            for (int r = 0; r < R; ++r) {
+                const int pr = l[r] >= 0 ? __builtin_amdgcn_readlane(prev, l[r]) : 0;
+                sp[r] = (const uint8_t *)valid_src + (int64_t)pr * row_bytes;
+                dp[r] = (uint8_t *)valid_dst +
+                        (dst_base + bt + (int64_t)(l[r] >= 0 ? l[r] : 0) * nb) * row_bytes;
+                count[r] = 0;
+            }
+            if (unit == 16) {
+                for (int j = lane; j < units; j += 64) {
+                    uint4 x[R];
+                    for (int r = 0; r < R; ++r)
+                        if (l[r] >= 0)
+                            x[r] = ((const uint4 *)sp[r])[j];
+                    for (int r = 0; r < R; ++r)
+                        if (l[r] >= 0) {
+                            count[r] += copy_count<kBits>(x[r]);
+                            ((uint4 *)dp[r])[j] = x[r];
+                        }
+                }
+            } else if (kBits) { // 8-byte words
+                for (int j = lane; j < units; j += 64) {
+                    uint64_t x[R];
+                    for (int r = 0; r < R; ++r)
+                        if (l[r] >= 0)
+                            x[r] = ((const uint64_t *)sp[r])[j];
+                    for (int r = 0; r < R; ++r)
+                        if (l[r] >= 0) {
+                            count[r] += __popcll(x[r]);
+                            ((uint64_t *)dp[r])[j] = x[r];
+                        }
+                }
+            } else { // single bytes
+                for (int j = lane; j < units; j += 64) {
+                    uint8_t x[R];
+                    for (int r = 0; r < R; ++r)
+                        if (l[r] >= 0)
+                            x[r] = sp[r][j];
+                    for (int r = 0; r < R; ++r)
+                        if (l[r] >= 0) {
+                            count[r] += copy_count<false>(x[r]);
+                            dp[r][j] = x[r];
+                        }
+                }
+            }
+            if (path_cp)
+                for (int r = 0; r < R; ++r) {
+                    const int c = wave_sum(count[r]);
+                    if (l[r] >= 0 && lane == 0) {
+                        const int invalid = kBits ? n_p - c : c;
+                        path_cp[cp_base + bt + (int64_t)l[r] * nb] = ((float)invalid) / (float)n_p;
+                    }
+                }
+        }
+        // the paths that meet a half-space, one at a time
+        m = __ballot(act && has);
+        while (m) {
+            const int l = __builtin_ctzll(m);
+            m &= m - 1;
+            PathInfo q;
+            q.g0 = ((int64_t)__builtin_amdgcn_readlane((int)(pi.g0 >> 32), l) << 32) |
+                   (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pi.g0, l);
+            q.W = __builtin_amdgcn_readlane(pi.W, l);
+            q.dtl = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pi.dtl), l));
+            q.tbase = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pi.tbase), l));
+            propagate_path<kBits>(bt + (int64_t)l * nb, __builtin_amdgcn_readlane(e, l),
+                                  __builtin_amdgcn_readlane(prev, l), q, as, bs, edge_mask, dt,
+                                  max_hs, offsets_time, tmax_steps, om, n_mc, valid_src, valid_dst,
+                                  dst_base, n_p, path_cp, cp_base, clamped);
+        }
+    }
+}
+
+// Copy-on-write form: the listed paths' child rows start as their parents'
+// rows (the half-space pass ANDs into them).  One wave per listed path.
+template <int kUnit>
+__global__ __launch_bounds__(256) void k_copy_listed(const HsPath *__restrict__ recs,
+                                                     const int *__restrict__ ctl,
+                                                     const uint8_t *__restrict__ src,
+                                                     uint8_t *__restrict__ dst, int64_t dst_base,
+                                                     int64_t row_bytes)
+{
+    using T = typename CopyWord<kUnit>::T;
+    if (ctl[kCtlOverflow]) // the group form writes whole rows itself
+        return;
+    const int lane = threadIdx.x & 63;
+    const int n = ctl[kCtlListed];
+    const int units = (int)(row_bytes / kUnit);
+    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t slot = wave_index(); slot < n; slot += n_waves) {
+        const HsPath r = recs[slot];
+        const T *sp = reinterpret_cast<const T *>(src + (int64_t)r.prev * row_bytes);
+        T *dp = reinterpret_cast<T *>(dst + (dst_base + r.w) * row_bytes);
+        for (int j = lane; j < units; j += 64)
+            dp[j] = sp[j];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_row_src_iota(int n, int *__restrict__ row_src)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        row_src[i] = (int)i;
+}
+
+// Materialises selected children of a copy-on-write step: child sel[j] goes
+// to row dst_base + j, from its own row (row_src >= 0) or its parent's.
+// Byte rows are normalised to 0/1 as the copy form writes them.  Several rows
+// per workgroup, one segment of lanes per row.
+__global__ __launch_bounds__(256) void k_gather_cow(const uint8_t *__restrict__ children,
+                                                    const uint8_t *__restrict__ parents,
+                                                    const int *__restrict__ row_src,
+                                                    const int *__restrict__ sel, int n_sel,
+                                                    uint8_t *__restrict__ dst, int64_t dst_base,
+                                                    int64_t row_bytes, int bits, int seg)
+{
+    const int per_block = blockDim.x / seg;
+    const int sub = threadIdx.x / seg, j0 = threadIdx.x % seg;
+    const bool wide = ((row_bytes & 15) == 0) &&
+                      ((((uintptr_t)children) | ((uintptr_t)parents) | ((uintptr_t)dst)) & 15) == 0;
+    for (int64_t k = (int64_t)blockIdx.x * per_block + sub; k < n_sel;
+         k += (int64_t)gridDim.x * per_block) {
+        const int i = sel ? sel[k] : (int)k;
+        const int r = row_src[i];
+        const uint8_t *s = r >= 0 ? children + (int64_t)r * row_bytes
+                                  : parents + (int64_t)(-1 - r) * row_bytes;
+        uint8_t *d = dst + (dst_base + k) * row_bytes;
+        if (wide) {
+            const uint4 *s4 = reinterpret_cast<const uint4 *>(s);
+            uint4 *d4 = reinterpret_cast<uint4 *>(d);
+            for (int64_t q = j0; q < row_bytes / 16; q += seg) {
+                uint4 x = s4[q];
+                if (!bits)
+                    (void)copy_count<false>(x);
+                d4[q] = x;
+            }
+        } else {
+            for (int64_t q = j0; q < row_bytes; q += seg)
+                d[q] = bits ? s[q] : (s[q] ? 1 : 0);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_setup_bits(uint64_t *__restrict__ v, int64_t n_paths,
                                                     int n_p)
 {
@@ -1011,6 +1252,49 @@ static int check_prop_args(int num_new_paths, int waypoints_count, const int64_t
     return PUMP_OK;
 }
 
+// workgroups of a grid-stride launch with one wave per item, at most 32 per CU
+static unsigned wave_blocks_for(int64_t waves)
+{
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t cap = (int64_t)cus * 32;
+    const int64_t blocks = (waves + 3) / 4;
+    return (unsigned)(blocks < 1 ? 1 : (blocks < cap ? blocks : cap));
+}
+
+// The 64-paths-per-wave propagate: its copy unit and grid
+template <bool kBits>
+static int launch_b64(int n_new, const int64_t *wp_offsets, int W_fixed, const int *wf_edge,
+                      const int *wf_prev, const float *as, const float *bs,
+                      const uint64_t *edge_mask, const float *times, const float *topts, float dt,
+                      int max_hs, const float *offsets_time, int tmax_steps, float om, int n_mc,
+                      const void *src, void *dst, int64_t dst_base, int n_p, float *path_cp,
+                      int64_t cp_base, unsigned long long *clamped, hipStream_t s)
+{
+    const int64_t row_bytes = kBits ? (int64_t)((n_p + 63) / 64) * 8 : (int64_t)n_p;
+    const uintptr_t al = (uintptr_t)src | (uintptr_t)dst;
+    const int unit = (row_bytes % 16 == 0 && (al & 15) == 0) ? 16 : (kBits ? 8 : 1);
+    hipLaunchKernelGGL(k_propagate_b64<kBits>, dim3(wave_blocks_for((n_new + 63) / 64)),
+                       dim3(256), 0, s, n_new, wp_offsets, W_fixed, wf_edge, wf_prev, as, bs,
+                       edge_mask, times, topts, dt, max_hs, offsets_time, tmax_steps, om, n_mc,
+                       src, dst, dst_base, n_p, path_cp, cp_base, clamped, unit);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+
+// defined below, with the batched form
+template <bool kBits>
+static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, const int *wf_edge,
+                            const int *wf_prev, const float *as, const float *bs,
+                            const uint64_t *edge_mask, const float *times, const float *topts,
+                            float dt, int max_hs, const float *offsets_time, int tmax_steps,
+                            float om, int n_mc, const void *src, void *dst, int64_t dst_base,
+                            int n_p, float *path_cp, int64_t cp_base, unsigned long long *clamped,
+                            const uint16_t *split, unsigned long long *band, void *ws,
+                            size_t ws_bytes, hipStream_t s, const float *parent_cp = nullptr,
+                            int *row_src = nullptr);
+
 extern "C" int pump_propagate_valid_particles(
     int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets, const int *d_wf_edge,
     const int *d_wf_prev, int64_t path_base, const float *d_as, const float *d_bs,
@@ -1027,6 +1311,30 @@ extern "C" int pump_propagate_valid_particles(
         return PUMP_OK;
     PUMP_REQUIRE(d_wf_edge && d_wf_prev && d_as && d_bs && d_times && d_topts && d_offsets_time &&
                  d_valid);
+    hipStream_t s = as_stream(stream);
+    if (num_new_paths >= 4096 && !use_block_kernel()) {
+        // A large wavefront runs the batched form in place (classify, row
+        // copies, time-binned half-space tests) with a workspace from the
+        // stream-ordered pool, released on the same stream; the reference
+        // allocates scratch at this call site on every iteration too
+        // (PUMP.cu:89).  Without one, the per-path form below runs.
+        const size_t ws_bytes = pump_propagate_workspace(num_new_paths, tmax_steps);
+        void *ws = nullptr;
+        if (hipMallocAsync(&ws, ws_bytes, s) == hipSuccess && ws) {
+            const int r = launch_propagate<false>(
+                num_new_paths, waypoints_count, d_wp_offsets, d_wf_edge, d_wf_prev, d_as, d_bs,
+                (const uint64_t *)nullptr, d_times, d_topts, dt, max_hs, d_offsets_time,
+                tmax_steps, offset_mult, num_mc_particles, (const void *)d_valid, (void *)d_valid,
+                path_base, n_p, (float *)nullptr, (int64_t)0, d_clamped,
+                (const uint16_t *)nullptr, (unsigned long long *)nullptr, ws, ws_bytes, s);
+            const hipError_t fe = hipFreeAsync(ws, s);
+            if (r)
+                return r;
+            PUMP_HIP(fe);
+            return PUMP_OK;
+        }
+        (void)hipGetLastError(); // the failed request is not an error of this call
+    }
     if (use_block_kernel())
         hipLaunchKernelGGL(k_propagate, dim3(num_new_paths), dim3(256), 0, as_stream(stream),
                            num_new_paths, d_wp_offsets, waypoints_count, d_wf_edge, d_wf_prev,
@@ -1034,12 +1342,12 @@ extern "C" int pump_propagate_valid_particles(
                            offset_mult, num_mc_particles, d_valid, d_valid, path_base, n_p,
                            (float *)nullptr, (int64_t)0, d_clamped);
     else
-        hipLaunchKernelGGL(k_propagate_wave<false>, dim3(wave_grid(num_new_paths)), dim3(256), 0,
-                           as_stream(stream), num_new_paths, d_wp_offsets, waypoints_count,
-                           d_wf_edge, d_wf_prev, d_as, d_bs, (const uint64_t *)nullptr, d_times,
-                           d_topts, dt, max_hs, d_offsets_time, tmax_steps, offset_mult,
-                           num_mc_particles, (const void *)d_valid, (void *)d_valid, path_base,
-                           n_p, (float *)nullptr, (int64_t)0, d_clamped);
+        return launch_b64<false>(num_new_paths, d_wp_offsets, waypoints_count, d_wf_edge,
+                                 d_wf_prev, d_as, d_bs, (const uint64_t *)nullptr, d_times,
+                                 d_topts, dt, max_hs, d_offsets_time, tmax_steps, offset_mult,
+                                 num_mc_particles, (const void *)d_valid, (void *)d_valid,
+                                 path_base, n_p, (float *)nullptr, (int64_t)0, d_clamped,
+                                 as_stream(stream));
     PUMP_LAUNCHED();
     return PUMP_OK;
 }
@@ -1113,7 +1421,8 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
                             float om, int n_mc, const void *src, void *dst, int64_t dst_base,
                             int n_p, float *path_cp, int64_t cp_base, unsigned long long *clamped,
                             const uint16_t *split, unsigned long long *band, void *ws,
-                            size_t ws_bytes, hipStream_t s)
+                            size_t ws_bytes, hipStream_t s, const float *parent_cp,
+                            int *row_src)
 {
     const int64_t row_bytes = kBits ? (int64_t)((n_p + 63) / 64) * 8 : (int64_t)n_p;
     const BatchLayout L(n_new, tmax_steps);
@@ -1162,11 +1471,27 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
         hipLaunchKernelGGL(k_classify, dim3(grid_for(n_new, kClassifyBlock, 2048)),
                            dim3(kClassifyBlock), 0, s, n_new, wp_offsets, W_fixed, wf_edge,
                            wf_prev, bs, edge_mask, max_hs, times, topts, dt, tmax_steps, has,
-                           recs, ctl, hist, acc, clamped);
+                           recs, ctl, hist, acc, clamped, row_src, parent_cp, path_cp, cp_base);
         PUMP_LAUNCHED();
         int r = bins_scan(B, s);
         if (r)
             return r;
+        if (row_src) {
+            // copy-on-write: only the listed paths' rows are written
+            const uintptr_t al = ((uintptr_t)src | (uintptr_t)dst);
+            const unsigned g = wave_blocks_for(n_new);
+            if (row_bytes % 16 == 0 && (al & 15) == 0)
+                hipLaunchKernelGGL(k_copy_listed<16>, dim3(g), dim3(256), 0, s, recs, ctl,
+                                   (const uint8_t *)src, (uint8_t *)dst, dst_base, row_bytes);
+            else if (row_bytes % 8 == 0 && (al & 7) == 0)
+                hipLaunchKernelGGL(k_copy_listed<8>, dim3(g), dim3(256), 0, s, recs, ctl,
+                                   (const uint8_t *)src, (uint8_t *)dst, dst_base, row_bytes);
+            else
+                hipLaunchKernelGGL(k_copy_listed<1>, dim3(g), dim3(256), 0, s, recs, ctl,
+                                   (const uint8_t *)src, (uint8_t *)dst, dst_base, row_bytes);
+            PUMP_LAUNCHED();
+            return bins_finish(B, kBits, s);
+        }
         // copy unit and segment width
         const uintptr_t al = ((uintptr_t)src | (uintptr_t)dst);
         int unit = 1;
@@ -1219,13 +1544,19 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
                                W_fixed, wf_edge, wf_prev, as, bs, times, topts, dt, max_hs,
                                offsets_time, tmax_steps, om, n_mc, (const uint8_t *)src,
                                (uint8_t *)dst, dst_base, n_p, path_cp, cp_base, clamped);
+        PUMP_LAUNCHED();
     } else {
-        hipLaunchKernelGGL(k_propagate_wave<kBits>, dim3(wave_grid(n_new)), dim3(256), 0, s, n_new,
-                           wp_offsets, W_fixed, wf_edge, wf_prev, as, bs, edge_mask, times, topts,
-                           dt, max_hs, offsets_time, tmax_steps, om, n_mc, src, dst, dst_base, n_p,
-                           path_cp, cp_base, clamped);
+        int r = launch_b64<kBits>(n_new, wp_offsets, W_fixed, wf_edge, wf_prev, as, bs, edge_mask,
+                                  times, topts, dt, max_hs, offsets_time, tmax_steps, om, n_mc,
+                                  src, dst, dst_base, n_p, path_cp, cp_base, clamped, s);
+        if (r)
+            return r;
     }
-    PUMP_LAUNCHED();
+    if (row_src) { // every child row was written
+        hipLaunchKernelGGL(k_row_src_iota, dim3(grid_for(n_new, 256, 4096)), dim3(256), 0, s, n_new,
+                           row_src);
+        PUMP_LAUNCHED();
+    }
     return PUMP_OK;
 }
 
@@ -1306,6 +1637,73 @@ extern "C" int pump_propagate_cp_bits(int num_new_paths, int waypoints_count,
                                   num_mc_particles, d_valid_src, d_valid_dst, dst_base, n_p,
                                   d_path_cp, cp_base, d_clamped, split, d_band_count, d_workspace,
                                   workspace_bytes, as_stream(stream));
+}
+
+extern "C" int pump_propagate_cp_cow(int validity_bits, int num_new_paths, int waypoints_count,
+                                     const int64_t *d_wp_offsets, const int *d_wf_edge,
+                                     const int *d_wf_prev, const float *d_as, const float *d_bs,
+                                     const uint64_t *d_edge_hs_mask, const float *d_times,
+                                     const float *d_topts, float dt, int max_hs,
+                                     const float *d_offsets_time, int tmax_steps,
+                                     float offset_mult, int num_mc_particles,
+                                     const void *d_valid_src, const float *d_parent_cp,
+                                     void *d_valid_dst, int64_t dst_base, int *d_row_src, int n_p,
+                                     float *d_path_cp, int64_t cp_base, int precision,
+                                     const void *d_offsets_split,
+                                     unsigned long long *d_band_count,
+                                     unsigned long long *d_clamped, void *d_workspace,
+                                     size_t workspace_bytes, pump_stream_t stream)
+{
+    int r = check_prop_args(num_new_paths, waypoints_count, d_wp_offsets, max_hs, tmax_steps, n_p,
+                            num_mc_particles);
+    if (r)
+        return r;
+    if (num_new_paths == 0)
+        return PUMP_OK;
+    PUMP_REQUIRE(validity_bits == 0 || validity_bits == 1);
+    PUMP_REQUIRE(d_wf_edge && d_wf_prev && d_as && d_bs && d_times && d_topts && d_offsets_time &&
+                 d_valid_src && d_valid_dst && d_valid_src != d_valid_dst && d_parent_cp &&
+                 d_row_src && d_path_cp);
+    PUMP_REQUIRE(precision == 0 || (precision == 1 && validity_bits && d_offsets_split));
+    if (validity_bits) {
+        const uint16_t *split = (precision == 1 && offset_mult > 0.0f && offset_mult < INFINITY)
+                                    ? (const uint16_t *)d_offsets_split
+                                    : nullptr;
+        return launch_propagate<true>(
+            num_new_paths, waypoints_count, d_wp_offsets, d_wf_edge, d_wf_prev, d_as, d_bs,
+            d_edge_hs_mask, d_times, d_topts, dt, max_hs, d_offsets_time, tmax_steps, offset_mult,
+            num_mc_particles, d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp, cp_base,
+            d_clamped, split, d_band_count, d_workspace, workspace_bytes, as_stream(stream),
+            d_parent_cp, d_row_src);
+    }
+    return launch_propagate<false>(
+        num_new_paths, waypoints_count, d_wp_offsets, d_wf_edge, d_wf_prev, d_as, d_bs,
+        d_edge_hs_mask, d_times, d_topts, dt, max_hs, d_offsets_time, tmax_steps, offset_mult,
+        num_mc_particles, d_valid_src, d_valid_dst, dst_base, n_p, d_path_cp, cp_base, d_clamped,
+        (const uint16_t *)nullptr, (unsigned long long *)nullptr, d_workspace, workspace_bytes,
+        as_stream(stream), d_parent_cp, d_row_src);
+}
+
+extern "C" int pump_gather_rows_cow(const void *d_children, const void *d_parents,
+                                    const int *d_row_src, const int *d_sel, int n_sel,
+                                    void *d_dst, int64_t dst_base, int64_t row_bytes,
+                                    int validity_bits, pump_stream_t stream)
+{
+    PUMP_REQUIRE(n_sel >= 0 && row_bytes > 0 && dst_base >= 0);
+    if (n_sel == 0)
+        return PUMP_OK;
+    PUMP_REQUIRE(d_children && d_parents && d_row_src && d_dst && d_children != d_dst);
+    int seg = 1;
+    const int64_t units = (row_bytes % 16 == 0) ? row_bytes / 16 : row_bytes;
+    while (seg < units && seg < 256)
+        seg <<= 1;
+    const int per_block = 256 / seg;
+    hipLaunchKernelGGL(k_gather_cow, dim3(grid_for(n_sel, per_block, 65536)), dim3(256), 0,
+                       as_stream(stream), (const uint8_t *)d_children, (const uint8_t *)d_parents,
+                       d_row_src, d_sel, n_sel, (uint8_t *)d_dst, dst_base, row_bytes,
+                       validity_bits ? 1 : 0, seg);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
 }
 
 // initial bit masks: all n_p particles valid

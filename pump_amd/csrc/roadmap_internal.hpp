@@ -138,7 +138,7 @@ struct SearchWs {
     DevBuf prev, node, cost, time, cpv, next, head, open, valid, flags, steps, clamp;
     // per-iteration wavefront state
     DevBuf cnt, wf_prev, wf_edge, wf_next, stage, cp, new_time, new_cost, keep, node_cnt,
-        node_cur, grp, surv, src_rows, bkt, bflag, prop;
+        node_cur, grp, surv, src_rows, bkt, bflag, prop, row_src;
     // goal-path selection: candidate paths, their lengths and MC hit counts
     DevBuf sel_paths, sel_T, sel_hits;
     // rows of the path table (prev, node, cost, time, cpv) the last search stored

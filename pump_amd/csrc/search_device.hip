@@ -372,7 +372,7 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
            &stage = W->stage, &d_cp = W->cp, &new_time = W->new_time, &new_cost = W->new_cost,
            &keep = W->keep, &node_cnt = W->node_cnt, &node_cur = W->node_cur, &grp = W->grp,
            &surv = W->surv, &src_rows = W->src_rows, &bkt = W->bkt, &bflag = W->bflag,
-           &d_ws = W->prop;
+           &d_ws = W->prop, &row_src = W->row_src;
     TRY(node_cnt.ensure(sizeof(int64_t) * ((size_t)num + 1)));
     TRY(node_cur.ensure(sizeof(int) * (size_t)num));
 
@@ -426,23 +426,17 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
         TRY(stage.ensure((size_t)n_new * row_bytes));
         TRY(d_ws.ensure(pump_propagate_workspace(n_new, rm->tmax_steps)));
         TRY(d_cp.ensure(sizeof(float) * n_new));
+        TRY(row_src.ensure(sizeof(int) * n_new));
         unsigned long long *cl = as<unsigned long long>(clamp);
         tm.start();
-        if (bits) {
-            TRY(pump_propagate_cp_bits(
-                n_new, 0, rm->wp_offsets, as<int>(wf_edge), as<int>(wf_prev), rm->as, rm->bs,
-                rm->edge_hs_mask, as<float>(time), rm->topts_edge, P.dt, rm->max_hs,
-                rm->offsets_time, rm->tmax_steps, P.offset_mult, rm->n_mc, as<uint64_t>(valid),
-                as<uint64_t>(stage), 0, n_p, as<float>(d_cp), 0, P.precision, rm->offsets_split,
-                cl + 1, cl, d_ws.p, d_ws.bytes, s));
-        } else {
-            TRY(pump_propagate_cp(n_new, 0, rm->wp_offsets, as<int>(wf_edge), as<int>(wf_prev),
-                                  rm->as, rm->bs, rm->edge_hs_mask, as<float>(time),
-                                  rm->topts_edge, P.dt, rm->max_hs, rm->offsets_time,
-                                  rm->tmax_steps, P.offset_mult, rm->n_mc, as<uint8_t>(valid),
-                                  as<uint8_t>(stage), 0, n_p, as<float>(d_cp), 0, cl, d_ws.p,
-                                  d_ws.bytes, s));
-        }
+        // copy-on-write: the children that meet no half-space keep their
+        // parent's row and CP; the survivors' rows are materialised below
+        TRY(pump_propagate_cp_cow(
+            bits ? 1 : 0, n_new, 0, rm->wp_offsets, as<int>(wf_edge), as<int>(wf_prev), rm->as,
+            rm->bs, rm->edge_hs_mask, as<float>(time), rm->topts_edge, P.dt, rm->max_hs,
+            rm->offsets_time, rm->tmax_steps, P.offset_mult, rm->n_mc, valid.p, as<float>(cpv),
+            stage.p, 0, as<int>(row_src), n_p, as<float>(d_cp), 0, bits ? P.precision : 0,
+            rm->offsets_split, cl + 1, cl, d_ws.p, d_ws.bytes, s));
         ms_prop += tm.stop();
 
         // sums, cutoff, dominance (PUMP.cu:164-218)
@@ -525,8 +519,9 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
             TRY(placed);
             for (int b = 0; b < nb; ++b)
                 sizeG[b] += add[b];
-            TRY(pump_gather_rows(as<uint8_t>(stage), as<int>(src_rows), (int)n_valid,
-                                 as<uint8_t>(valid), num_paths, row_bytes, s));
+            TRY(pump_gather_rows_cow(stage.p, valid.p, as<int>(row_src), as<int>(src_rows),
+                                     (int)n_valid, valid.p, num_paths, row_bytes, bits ? 1 : 0,
+                                     s));
         }
         num_paths += n_valid;
 

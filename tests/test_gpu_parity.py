@@ -243,9 +243,10 @@ def oracle_propagate(d):
     return v, cp, cl
 
 
-@pytest.mark.parametrize("variable", [False, True])
-def test_propagate_entry_points(variable):
-    d = random_hsmc_inputs(7 + variable, variable=variable)
+@pytest.mark.parametrize("variable,n_new", [(False, 50), (True, 50), (True, 6000)])
+def test_propagate_entry_points(variable, n_new):
+    # 6000 new paths: the drop-in runs the batched form in place
+    d = random_hsmc_inputs(7 + variable, variable=variable, n_new=n_new)
     ref_v, ref_cp, ref_cl = oracle_propagate(d)
     assert ref_cl > 0
     g = {k: dev(d[k]) for k in ("off", "as_", "bs", "offs", "topts", "times", "wf_edge",
@@ -386,6 +387,70 @@ def test_propagate_batched_form(n_p, w_range, T):
             None, None, 0, None))
         assert np.array_equal(host(dst1), ref_v[npar:])
         assert np.array_equal(host(cp1), ref_cp[npar:])
+
+
+@pytest.mark.parametrize("n_new", [6000, 500])
+def test_propagate_copy_on_write(n_new):
+    """Copy-on-write step (pump_propagate_cp_cow): children that meet no
+    stored half-space point at their parent's row and take its CP; the rows
+    gathered for every child (pump_gather_rows_cow) equal the oracle's, in byte
+    and bit validity, FP32 and split-BF16; a small wavefront (per-path form)
+    writes every row."""
+    n_p = 700
+    d = random_hsmc_inputs(91, n_edges=3000, W=9, max_hs=5, n_p=n_p, n_mc=1024, T=300,
+                           n_new=n_new, n_parent=64)
+    keep = np.random.default_rng(3).uniform(size=d["off"].shape[0] - 1) < 0.15
+    for e in np.nonzero(~keep)[0]:
+        d["bs"][d["off"][e] * d["max_hs"]: d["off"][e + 1] * d["max_hs"]] = 1111
+    ref_v, ref_cp, ref_cl = oracle_propagate(d)
+    npar = d["n_parent"]
+    pcp = (np.count_nonzero(d["valid"][:npar] == 0, axis=1).astype(np.float32) /
+           np.float32(n_p)).astype(np.float32)
+    g = {k: dev(d[k]) for k in ("off", "as_", "bs", "offs", "topts", "times", "wf_edge",
+                                 "wf_prev")}
+    g_mask = dev(edge_mask_host(d["off"], d["bs"], d["max_hs"]))
+    g_split = split_table(g["offs"], d["T"], d["n_mc"])
+    g_pcp = dev(pcp)
+    ws_bytes = P.lib.pump_propagate_workspace(n_new, d["T"])
+    ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
+    has_hs = (edge_mask_host(d["off"], d["bs"], d["max_hs"]) != 0)[d["wf_edge"]]
+    for bits, precision in ((0, 0), (1, 0), (1, 1)):
+        width = (n_p + 63) // 64 * 8 if bits else n_p
+        src = dev(pack_bits(d["valid"][:npar], n_p) if bits else d["valid"][:npar])
+        stage = dev(np.zeros((n_new, width), np.uint8))
+        row_src = dev(np.full(n_new, 12345, np.int32))
+        cp = dev(np.zeros(n_new, np.float32))
+        cnt = dev(np.zeros(2, np.uint64))
+        P.check("propagate_cp_cow", P.lib.pump_propagate_cp_cow(
+            bits, n_new, 0, ptr(g["off"]), ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
+            ptr(g["bs"]), ptr(g_mask), ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"],
+            ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(src), ptr(g_pcp), ptr(stage), 0,
+            ptr(row_src), n_p, ptr(cp), 0, precision, g_split.p if precision else None,
+            cnt.ptr + 8, ptr(cnt), ws.p, ws_bytes, None))
+        rs = host(row_src)
+        if n_new >= 4096:
+            assert np.array_equal(rs[~has_hs], -1 - d["wf_prev"][~has_hs])
+            assert np.array_equal(rs[has_hs], np.arange(n_new)[has_hs])
+        else:
+            assert np.array_equal(rs, np.arange(n_new))
+        assert np.array_equal(host(cp), ref_cp[npar:]), (bits, precision)
+        assert int(host(cnt)[0]) == ref_cl
+        out = dev(np.zeros((n_new + 3, width), np.uint8))
+        P.check("gather_rows_cow", P.lib.pump_gather_rows_cow(
+            ptr(stage), ptr(src), ptr(row_src), None, n_new, ptr(out), 3, width, bits, None))
+        rows = host(out)[3:]
+        got = unpack_bits(rows.view(np.uint64), n_p) if bits else rows
+        assert np.array_equal(got, ref_v[npar:]), (bits, precision)
+        # a selection: every third child, in reverse
+        sel = np.arange(n_new)[::-3].astype(np.int32).copy()
+        out2 = dev(np.zeros((sel.size, width), np.uint8))
+        g_sel = dev(sel)
+        P.check("gather_rows_cow", P.lib.pump_gather_rows_cow(
+            ptr(stage), ptr(src), ptr(row_src), ptr(g_sel), sel.size, ptr(out2), 0, width, bits,
+            None))
+        rows2 = host(out2)
+        got2 = unpack_bits(rows2.view(np.uint64), n_p) if bits else rows2
+        assert np.array_equal(got2, ref_v[npar:][sel])
 
 
 def test_split_bf16_band_decisions():
