@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--skip-hsmc", action="store_true")
     ap.add_argument("--batch", type=int, default=256,
                     help="instances of the uneven batch workload (0: skip)")
+    ap.add_argument("--hs-sweep", action="store_true",
+                    help="only the half-space density sweep of the MC pass (FP32 against "
+                         "split-BF16), one JSON line per density")
     return ap.parse_args()
 
 
@@ -83,6 +86,9 @@ def main():
     from pump_amd import configs
 
     assert P.lib.pump_dev_count() >= 1, "no GPU visible"
+    if args.hs_sweep:
+        hs_sweep(P, configs, args.hsmc_reps)
+        return
     log("device:", P.device_name())
     over = {"num": args.num} if args.num else {}
     cfg = configs.config(args.config, **over)
@@ -146,6 +152,35 @@ def main():
     log(f"plain edge pass (discretize + freeEdges): {ms_plain:.3f} ms, "
         f"fused is {plain['fused_vs_plain']:.2f}x, flags equal {plain['flags_equal']}")
 
+    # chosen form: the fused pass with FP32 waypoints (within a stated bound
+    # of the reference's) and exact flags, into separate output arrays
+    disc32 = P.DeviceArray(shape=(n_wp, 6), dtype=np.float32, fill=0)
+    flags32 = P.DeviceArray(shape=(E,), dtype=np.uint8, fill=9)
+    retests = P.DeviceArray(shape=(1,), dtype=np.uint64, fill=0)
+
+    def fp32_pass():
+        P.check("pump_edges_discretize_check_fp32", P.lib.pump_edges_discretize_check_fp32(
+            d.samples, d.edge_src, d.edge_dst, d.topts_edge, E, d.wp_offsets, 0, d.obstacles,
+            n_obs, disc32.p, flags32.p, retests.p, stream.h))
+
+    fp32_pass()
+    stream.sync()
+    n_retests = int(retests.numpy()[0])
+    timer.start()
+    for _ in range(args.steps):
+        fp32_pass()
+    ms_fp32 = timer.stop() / args.steps
+    edges_fp32 = {"ms": ms_fp32, "edge_checks_per_s": E / (ms_fp32 / 1e3),
+                  "achieved_gbs": alg_bytes / (ms_fp32 / 1e3) / 1e9,
+                  "frac_hbm_peak": alg_bytes / (ms_fp32 / 1e3) / HBM_PEAK,
+                  "vs_exact_fused": ms_step / ms_fp32, "vs_plain": ms_plain / ms_fp32,
+                  "exact_retests_per_pass": n_retests,
+                  "flags_equal": bool(np.array_equal(flags32.numpy(), fused_flags))}
+    log(f"FP32-waypoint edge pass: {ms_fp32:.3f} ms, {E / (ms_fp32 / 1e3):.3e} edge-checks/s, "
+        f"{edges_fp32['frac_hbm_peak'] * 100:.1f}% of HBM peak, {n_retests} exact re-tests, "
+        f"flags equal {edges_fp32['flags_equal']}")
+    disc32.free()
+
     # a plain device copy of known size: calibrates the HBM byte counters of a
     # profiled run (FETCH_SIZE / WRITE_SIZE of the copy vs its 2 x 256 MiB)
     calib = P.DeviceArray(shape=(2, 256 << 20), dtype=np.uint8, fill=7)
@@ -154,9 +189,9 @@ def main():
     calib.free()
 
     # ---------------- kernel (c): HSMC pass over every free edge -----------
-    hsmc = None
+    hsmc, hsmc_cp = None, None
     if not args.skip_hsmc:
-        hsmc = hsmc_pass(P, rm, d, prm, info, stream, args.hsmc_reps)
+        hsmc, hsmc_cp = hsmc_pass(P, rm, d, prm, info, stream, args.hsmc_reps)
 
     # ---------------- the PUMP search, bookkeeping on the device and host --
     search = search_pass(P, rm, prm)
@@ -174,6 +209,10 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(rm, samples, obstacles, prm, args.cpu_seconds)
+        cpu["one_thread"] = cpu_edges_one_thread(rm, samples, obstacles, args.cpu_seconds / 3)
+        cpu["pair_solves"] = cpu_pairs(rm, samples, prm, d, args.cpu_seconds / 3)
+        if hsmc_cp is not None:
+            cpu["particle_steps"] = cpu_particles(rm, prm, ot, hsmc_cp, args.cpu_seconds / 3)
 
     traffic = None
     tpath = os.path.join(HERE, "profiles", "edge_pass_num16000_traffic.json")
@@ -205,6 +244,7 @@ def main():
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "edges_plain": plain,
+        "edges_fp32": edges_fp32,
         "hsmc": hsmc,
         "pairs": pairs,
         "search": search,
@@ -217,22 +257,29 @@ def main():
     rm.close()
 
 
-def hsmc_pass(P, rm, d, prm, info, stream, reps):
-    """Every free edge expanded once from one of 4096 synthetic parent paths
-    (times uniform in [0, 2] s, all particles valid)."""
-    n_p, n_mc, T = prm.n_p, info.n_mc, info.tmax_steps
+def synthetic_wavefront(rm):
+    """The HSMC workload: every free edge (up to 1M) expanded once, child i
+    from parent i % 4096; parent times uniform in [0, 2] s."""
     fl = rm.flags()
     free = np.nonzero(fl["is_free_edges"])[0].astype(np.int32)
-    n_new = int(free.shape[0])
     cap = int(os.environ.get("PUMP_BENCH_MAX_NEW", "1000000"))
-    if n_new > cap:
-        free = free[:cap]
-        n_new = cap
+    free = free[:cap]
     n_par = 4096
     rng = np.random.default_rng(0)
     times = rng.uniform(0, 2.0, n_par).astype(np.float32)
-    prev = (np.arange(n_new) % n_par).astype(np.int32)
-    wo = fl["wp_offsets"]
+    prev = (np.arange(free.shape[0]) % n_par).astype(np.int32)
+    return dict(free=free, prev=prev, times=times, n_par=n_par, wp_offsets=fl["wp_offsets"])
+
+
+def hsmc_pass(P, rm, d, prm, info, stream, reps):
+    """Every free edge expanded once from one of 4096 synthetic parent paths
+    (times uniform in [0, 2] s, all particles valid).  Returns the figures
+    and the drop-in form's CPs (the CPU baseline's check)."""
+    n_p, n_mc, T = prm.n_p, info.n_mc, info.tmax_steps
+    wf = synthetic_wavefront(rm)
+    free, prev, times, n_par = wf["free"], wf["prev"], wf["times"], wf["n_par"]
+    n_new = int(free.shape[0])
+    wo = wf["wp_offsets"]
     W = (wo[free + 1] - wo[free]).astype(np.int64)
     steps = int(n_p) * int(W.sum())
     counts = rm.halfspaces()["count_stored"]
@@ -323,6 +370,26 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
             n_mc, srcp, g_pcp.p, dstp, 0, row_src.p, n_p, cpp, 0, precision,
             d.offsets_split if precision else None, None, None, g_ws.p, ws_bytes, stream.h))
 
+    # particles per wave of the binned half-space test, tuned per
+    # configuration on this workload (the tuned shape then serves every
+    # batched form of the configuration below)
+    import ctypes
+    ppw = {}
+    for name, bits, prec, srcp, dstp, cpp in (
+            ("fp32_bits", 1, 0, bsrc.p, cdst.p, cp_cow.p),
+            ("split_bf16_bits", 1, 1, bsrc.p, cdst.p, cp_cow_split.p),
+            ("fp32_bytes", 0, 0, src.p, cdst_b.p, cp_cow_b.p)):
+        chosen = ctypes.c_int(0)
+        P.check("pump_hsmc_autotune_cow", P.lib.pump_hsmc_autotune_cow(
+            bits, n_new, 0, d.wp_offsets, g_edge.p, g_prev.p, d.as_, d.bs, d.edge_hs_mask,
+            g_times.p, d.topts_edge, prm.dt, info.max_hs, d.offsets_time, T, prm.offset_mult,
+            n_mc, srcp, g_pcp.p, dstp, 0, row_src.p, n_p, cpp, 0, prec,
+            d.offsets_split if prec else None, None, None, g_ws.p, ws_bytes, stream.h,
+            ctypes.byref(chosen)))
+        ppw[name] = chosen.value
+    out["particles_per_wave"] = ppw
+    log(f"hsmc autotune, particles per wave: {ppw}")
+
     forms = (("dropin_fp32_bytes", run_dropin, lambda: cp_in.numpy()[n_par:]),
              ("batched_fp32_bytes", run_bytes, lambda: cp_bytes.numpy()),
              ("batched_fp32_bits", run_bits, lambda: g_cp.numpy()),
@@ -383,7 +450,110 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
     out["best_form"] = best
     out["value"] = out[best]["particle_steps_per_s"]
     out["unit"] = "particle-steps/s"
-    return out
+    ms_best = out[best]["ms"]
+    out["stored_halfspace_tests_per_s"] = n_p * hs_total / (ms_best / 1e3)
+    # the MC pass's HBM bytes in the chosen form: per listed child its parent
+    # row read and its own row written, per child the CP and row source
+    # written and its edge, parent and time read, and each offsets row the
+    # tests need once (12 bytes per particle and time index; at most the
+    # table's T rows, whatever the number of marked waypoints)
+    row_bytes = nw * 8 if "bits" in best else n_p
+    listed = out["cow_rows_written"] if best.startswith("cow") else n_new
+    offset_rows = min(T, n_marked)
+    mc_bytes = 2 * listed * row_bytes + 8 * n_new + 12 * n_new + 12 * n_p * offset_rows
+    out["mc"] = {"form": best, "alg_bytes_per_pass": mc_bytes,
+                 "achieved_gbs": mc_bytes / (ms_best / 1e3) / 1e9,
+                 "frac_hbm_peak": mc_bytes / (ms_best / 1e3) / HBM_PEAK,
+                 "frac_fp32_peak": out[best]["frac_fp32_peak"],
+                 "offset_rows_bound": offset_rows,
+                 "counters": mc_counters()}
+    return out, ref
+
+
+def hs_sweep(P, configs, reps):
+    """The batched MC pass (bit validity) in FP32 and in split-BF16 over
+    synthetic half-space tables of rising density: 200,000 edges of 8
+    waypoints, each expanded once from one of 4096 all-valid parents, 2000
+    particles, the noise table of the NUM=16000 workload; a fraction f of
+    the waypoints holds k live half-spaces (a ~ N(0, 0.05^2) per component,
+    b = a.a / 2 rescaled as the cache stores it).  Both forms must give the
+    same CPs."""
+    cfg = configs.config(1)
+    prm = cfg["params"]
+    ot, _ = P.make_noise(cfg["noise"])
+    T, n_mc = ot.shape[0], ot.shape[1]
+    n_edges, W, max_hs, n_p, n_par = 200000, 8, 8, 2000, 4096
+    off = (np.arange(n_edges + 1, dtype=np.int64) * W)
+    rng = np.random.default_rng(5)
+    topts = rng.uniform(0.3, 1.2, n_edges).astype(np.float32)
+    times = rng.uniform(0, 2.0, n_par).astype(np.float32)
+    edge = np.arange(n_edges, dtype=np.int32)
+    prev = (np.arange(n_edges) % n_par).astype(np.int32)
+    g = {"off": P.DeviceArray(off), "topts": P.DeviceArray(topts), "times": P.DeviceArray(times),
+         "edge": P.DeviceArray(edge), "prev": P.DeviceArray(prev), "ot": P.DeviceArray(ot)}
+    split_bytes = int(P.lib.pump_split_offsets_bytes(T, n_mc))
+    split = P.DeviceArray(shape=(split_bytes,), dtype=np.uint8, fill=0)
+    P.check("pump_split_offsets", P.lib.pump_split_offsets(g["ot"].p, T, n_mc, split.p, None))
+    nw = (n_p + 63) // 64
+    src = P.DeviceArray(shape=(n_par, nw), dtype=np.uint64, fill=0)
+    P.check("pump_setup_valid_bits", P.lib.pump_setup_valid_bits(src.p, n_p, n_par, None))
+    ws_bytes = int(P.lib.pump_propagate_workspace(n_edges, T))
+    ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
+    stream = P.Stream()
+    timer = P.Timer(stream)
+    for f, k in ((0.01, 1), (0.03, 1), (0.1, 1), (0.1, 4), (0.3, 4), (1.0, 4), (1.0, 8)):
+        n_wp = n_edges * W
+        a = rng.normal(0, 0.05, (n_wp, max_hs, 3)).astype(np.float32)
+        b = (0.5 * (a.astype(np.float64) ** 2).sum(-1)).astype(np.float32)
+        b[:, k:] = 1111.0
+        b[rng.uniform(size=n_wp) >= f] = 1111.0
+        ga, gb = P.DeviceArray(a.reshape(-1, 3)), P.DeviceArray(b.reshape(-1))
+        marked = int((b < 1000).any(axis=1).sum())
+        tests = int((b < 1000).sum()) * n_p
+        mask = np.zeros(n_edges, np.uint64)
+        has = (b < 1000).any(axis=1).reshape(n_edges, W)
+        for kk in range(W):
+            mask |= np.where(has[:, kk], np.uint64(1) << np.uint64(kk), np.uint64(0))
+        gm = P.DeviceArray(mask)
+        row = {"f": f, "k": k, "marked_waypoints": marked, "halfspace_tests": tests}
+        cps = {}
+        for name, prec in (("fp32", 0), ("split_bf16", 1)):
+            dst = P.DeviceArray(shape=(n_edges, nw), dtype=np.uint64, fill=0)
+            cp = P.DeviceArray(shape=(n_edges,), dtype=np.float32, fill=0)
+            band = P.DeviceArray(shape=(1,), dtype=np.uint64, fill=0)
+
+            def run():
+                P.check("pump_propagate_cp_bits", P.lib.pump_propagate_cp_bits(
+                    n_edges, 0, g["off"].p, g["edge"].p, g["prev"].p, ga.p, gb.p, gm.p,
+                    g["times"].p, g["topts"].p, prm.dt, max_hs, g["ot"].p, T, prm.offset_mult,
+                    n_mc, src.p, dst.p, 0, n_p, cp.p, 0, prec, split.p if prec else None,
+                    band.p if prec else None, None, ws.p, ws_bytes, stream.h))
+
+            # the first timed configuration also warms the clocks up
+            for _ in range(1 if cps or f != 0.01 else 20):
+                run()
+            stream.sync()
+            cps[name] = cp.numpy()
+            if prec:
+                row["band_retests"] = int(band.numpy()[0])
+            timer.start()
+            for _ in range(reps):
+                run()
+            ms = timer.stop() / reps
+            row[name + "_ms"] = ms
+            row[name + "_tests_per_s"] = tests / (ms / 1e3)
+            dst.free()
+            cp.free()
+            band.free()
+        row["split_vs_fp32"] = row["fp32_ms"] / row["split_bf16_ms"]
+        row["cps_equal"] = bool(np.array_equal(cps["fp32"], cps["split_bf16"]))
+        log(f"density f={f} k={k}: fp32 {row['fp32_ms']:.3f} ms, split-BF16 "
+            f"{row['split_bf16_ms']:.3f} ms ({row['split_vs_fp32']:.2f}x), "
+            f"CPs equal {row['cps_equal']}")
+        print(json.dumps(row), flush=True)
+        ga.free()
+        gb.free()
+        gm.free()
 
 
 def search_pass(P, rm, prm):
@@ -505,46 +675,153 @@ def batch_pass(P, n_inst, reps):
     return out
 
 
-def cpu_baseline(rm, samples, obstacles, prm, seconds):
-    """The CPU oracle's edge pass (findDiscretizedPath + freeEdges) over the
-    first R candidate edges, OpenMP over all host cores; R is sized so that
-    the run takes about `seconds`."""
+def _cpu_edges(rm, samples, obstacles, R, cores):
+    """The CPU oracle's edge pass over the first R candidate edges; returns
+    the seconds and whether its flags equal the GPU's."""
     import oracle
     from oracle import bp, fp, ip, lp
     O = oracle.load()
     e = rm.edges()
     fl = rm.flags()
-    cores = len(os.sched_getaffinity(0))
     obs = np.ascontiguousarray(obstacles, np.float32)
     num = samples.shape[0]
+    rs = e["row_start"]
+    last_row = int(np.searchsorted(rs, R, side="left"))
+    rs_cut = np.minimum(rs[: last_row + 1], R).astype(np.int64)
+    rs_cut = np.concatenate([rs_cut, np.full(num + 1 - rs_cut.shape[0], R, np.int64)])
+    wo = np.ascontiguousarray(fl["wp_offsets"][: R + 1])
+    disc = np.zeros((int(wo[-1]) + 1, 6), np.float32)
+    flags = np.zeros(R + 1, np.uint8)
+    dst = np.ascontiguousarray(e["edge_dst"][:R])
+    t0 = time.perf_counter()
+    O.po_discretize_edges(fp(samples), lp(rs_cut), ip(dst), num, lp(wo), fp(disc), cores)
+    O.po_free_edges(fp(obs), obs.shape[0], fp(disc), lp(wo), R, bp(flags), cores)
+    dt = time.perf_counter() - t0
+    return dt, bool(np.array_equal(flags[:R], fl["is_free_edges"][:R]))
 
-    def run(R):
-        # rows covering the first R edges
-        rs = e["row_start"]
-        last_row = int(np.searchsorted(rs, R, side="left"))
-        rs_cut = np.minimum(rs[: last_row + 1], R).astype(np.int64)
-        rs_cut = np.concatenate([rs_cut, np.full(num + 1 - rs_cut.shape[0], R, np.int64)])
-        wo = np.ascontiguousarray(fl["wp_offsets"][: R + 1])
-        disc = np.zeros((int(wo[-1]) + 1, 6), np.float32)
-        flags = np.zeros(R + 1, np.uint8)
-        dst = np.ascontiguousarray(e["edge_dst"][:R])
-        t0 = time.perf_counter()
-        O.po_discretize_edges(fp(samples), lp(rs_cut), ip(dst), num, lp(wo), fp(disc), cores)
-        O.po_free_edges(fp(obs), obs.shape[0], fp(disc), lp(wo), R, bp(flags), cores)
-        dt = time.perf_counter() - t0
-        agree = bool(np.array_equal(flags[:R], fl["is_free_edges"][:R]))
-        return dt, agree
 
+def _sized(run, n0, n_max, seconds):
+    """run(n) -> (seconds, ...): a first run at n0, then one sized to take
+    about `seconds`."""
+    n = min(n_max, n0)
+    res = run(n)
+    if res[0] < seconds * 0.5 and n < n_max:
+        n = int(min(n_max, n * max(1.0, seconds / max(res[0], 1e-3))))
+        res = run(n)
+    return n, res
+
+
+def cpu_baseline(rm, samples, obstacles, prm, seconds):
+    """The CPU oracle's edge pass (findDiscretizedPath + freeEdges) over the
+    first R candidate edges, OpenMP over all host cores; R is sized so that
+    the run takes about `seconds`."""
+    cores = len(os.sched_getaffinity(0))
     E = int(rm.info.n_edges)
-    R = min(E, 20000)
-    dt, agree = run(R)
-    if dt < seconds * 0.5 and R < E:
-        R = int(min(E, R * max(1.0, seconds / max(dt, 1e-3))))
-        dt, agree = run(R)
+    R, (dt, agree) = _sized(lambda r: _cpu_edges(rm, samples, obstacles, r, cores), 20000, E,
+                            seconds)
     log(f"cpu baseline: {R} edges in {dt:.2f} s on {cores} threads, flags agree: {agree}")
     return {"value": R / dt, "unit": "edge-checks/s", "cores": cores, "kind": "port",
             "sample": f"first {R} of {E} candidate edges (CPU oracle findDiscretizedPath + "
                       f"freeEdges, OpenMP)", "seconds": dt, "flags_match_gpu": agree}
+
+
+def cpu_edges_one_thread(rm, samples, obstacles, seconds):
+    """The same CPU edge pass on one thread."""
+    E = int(rm.info.n_edges)
+    R, (dt, agree) = _sized(lambda r: _cpu_edges(rm, samples, obstacles, r, 1), 2000, E,
+                            seconds)
+    log(f"cpu baseline, one thread: {R} edges in {dt:.2f} s, flags agree: {agree}")
+    return {"value": R / dt, "unit": "edge-checks/s", "cores": 1, "kind": "port",
+            "sample": f"first {R} of {E} candidate edges, one thread", "seconds": dt,
+            "flags_match_gpu": agree}
+
+
+def cpu_pairs(rm, samples, prm, d, seconds):
+    """The CPU oracle's all-pairs 2pBVP (fillCoptsTopts, 2pBVP.cu:22-50) over
+    the first r source rows (r x NUM pair solves), all host cores; its rows
+    are compared with the device's when the roadmap kept the pair matrix."""
+    import oracle
+    import pump_amd as P
+    O = oracle.load()
+    cores = len(os.sched_getaffinity(0))
+    num = samples.shape[0]
+    s = np.ascontiguousarray(samples, np.float32)
+
+    def run(r):
+        rows = np.arange(r, dtype=np.int32)
+        c = np.zeros((r, num), np.float32)
+        t = np.zeros((r, num), np.float32)
+        t0 = time.perf_counter()
+        O.po_fill_rows(oracle.fp(s), num, np.float32(prm.tmax), oracle.ip(rows), r,
+                       oracle.fp(c), oracle.fp(t), cores)
+        return time.perf_counter() - t0, c, t
+
+    r, (dt, c, t) = _sized(run, 8, num, seconds)
+    agree = None
+    if d.topts:
+        g = P.DeviceArray(shape=(2, r, num), dtype=np.float32, fill=0)
+        P.check("pump_dev_copy", P.lib.pump_dev_copy(g.ptr, d.copts, r * num * 4))
+        P.check("pump_dev_copy", P.lib.pump_dev_copy(g.ptr + r * num * 4, d.topts, r * num * 4))
+        P.sync()
+        gh = g.numpy()
+        g.free()
+        agree = bool(np.array_equal(gh[0].view(np.uint32), c.view(np.uint32)) and
+                     np.array_equal(gh[1].view(np.uint32), t.view(np.uint32)))
+    log(f"cpu pair solves: {r * num} in {dt:.2f} s on {cores} threads, rows agree: {agree}")
+    return {"value": r * num / dt, "unit": "pair-solves/s", "cores": cores, "kind": "port",
+            "sample": f"first {r} of {num} source rows ({r * num} pair solves)", "seconds": dt,
+            "rows_match_gpu": agree}
+
+
+def cpu_particles(rm, prm, offsets_time, gpu_cp, seconds):
+    """The CPU oracle's HSMC step (propagateValidParticles + calculateCP_HSMC,
+    hsmc.cu:101-199) over the first R children of the synthetic wavefront,
+    all host cores; its CPs are compared with the device drop-in form's."""
+    import oracle
+    from oracle import bp, fp, ip, lp
+    O = oracle.load()
+    cores = len(os.sched_getaffinity(0))
+    info = rm.info
+    wf = synthetic_wavefront(rm)
+    hs = rm.halfspaces()
+    topts = rm.edges()["topts_edge"]
+    wo = wf["wp_offsets"]
+    n_p, n_par = int(prm.n_p), wf["n_par"]
+    ot = np.ascontiguousarray(offsets_time, np.float32)
+
+    def run(R):
+        edge = np.ascontiguousarray(wf["free"][:R])
+        prev = np.ascontiguousarray(wf["prev"][:R])
+        valid = np.zeros((n_par + R, n_p), np.uint8)
+        valid[:n_par] = 1
+        cp = np.zeros(n_par + R, np.float32)
+        t0 = time.perf_counter()
+        O.po_propagate_valid_particles(R, lp(wo), ip(edge), ip(prev), n_par, fp(hs["as_"]),
+                                       fp(hs["bs"]), fp(wf["times"]), fp(topts), prm.dt,
+                                       info.max_hs, fp(ot), info.tmax_steps, prm.offset_mult,
+                                       info.n_mc, bp(valid), n_p, cores)
+        O.po_calculate_cp_hsmc(R, n_par, fp(cp), bp(valid), n_p)
+        dt = time.perf_counter() - t0
+        steps = n_p * int((wo[edge + 1] - wo[edge]).sum())
+        return dt, steps, bool(np.array_equal(cp[n_par:], gpu_cp[:R]))
+
+    n_new = int(wf["free"].shape[0])
+    R, (dt, steps, agree) = _sized(run, 2000, n_new, seconds)
+    log(f"cpu particle steps: {steps} ({R} children) in {dt:.2f} s on {cores} threads, "
+        f"CPs agree: {agree}")
+    return {"value": steps / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"first {R} of {n_new} children of the synthetic wavefront", "seconds": dt,
+            "cp_match_gpu": agree}
+
+
+def mc_counters():
+    """The MC pass's VALU and MFMA activity from the committed counter run
+    (profiles/hsmc_num16000_counters.json), or None."""
+    path = os.path.join(HERE, "profiles", "hsmc_num16000_counters.json")
+    try:
+        return json.load(open(path))
+    except (OSError, ValueError):
+        return None
 
 
 if __name__ == "__main__":

@@ -7,7 +7,7 @@
  * meaning and array layouts:
  *   samples[i*6 + d]                       (roadMap.cuh:16)
  *   obstacles[o*12 + {lo 0..5, hi 6..11}]  (mainPUMP.cu:214-221)
- *   discMotions[(wp)*6 + d], wp = e*W + k  (collisionCheck.cu:67-70)
+ *   discMotions[(wp)*6 + d], wp = e*W + k  (collisionCheck.cu:20-23)
  *   as[(wp*maxHS + h)*3 + d], bs[wp*maxHS + h], countStoredHSs[wp]
  *                                          (hsmc.cu:49,86-89; PUMP.cu:642-648)
  *   offsetsTime[(t*nMC + p)*3 + d]         (hardCoded.cuh:18-19)
@@ -160,6 +160,24 @@ int pump_edges_discretize_check(const float *d_samples, const int *d_edge_src,
                                 const float *d_obstacles, int n_obs, float *d_disc_motions,
                                 uint8_t *d_is_free_edges, pump_stream_t stream);
 
+/* The fused edge pass with FP32 waypoints (chosen form): the same call as
+ * pump_edges_discretize_check, with every interior waypoint evaluated in
+ * FP32 from per-edge FP32 coefficients of the cubic Hermite segment
+ * (Horner's rule at the reference's t) instead of the reference's FP64
+ * expression tree.  Each position component is within eps = 12 u S of the
+ * reference's waypoint (u = 2^-24, S = |p0| + |v0| tau + |c2| tau^2 +
+ * |c3| tau^3 of its axis); end states are exact copies.  The flags are the
+ * reference's bit for bit: a segment whose outcome against a box could
+ * change within those bounds (counting the reference's own FP32 roundings)
+ * is re-tested with the exact FP64 waypoints, counted in *d_retests
+ * (optional). */
+int pump_edges_discretize_check_fp32(const float *d_samples, const int *d_edge_src,
+                                     const int *d_edge_dst, const float *d_topts_edge,
+                                     int64_t n_edges, const int64_t *d_wp_offsets,
+                                     int waypoints_count, const float *d_obstacles, int n_obs,
+                                     float *d_disc_motions, uint8_t *d_is_free_edges,
+                                     unsigned long long *d_retests, pump_stream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Half-spaces and HSMC ("hsmc")
  * ------------------------------------------------------------------------ */
@@ -285,6 +303,39 @@ int pump_propagate_cp_cow(int validity_bits, int num_new_paths, int waypoints_co
                           const void *d_offsets_split, unsigned long long *d_band_count,
                           unsigned long long *d_clamped, void *d_workspace,
                           size_t workspace_bytes, pump_stream_t stream);
+
+/* Autotuner of the batched MC pass (no counterpart in the reference, whose
+ * step has no launch shape to choose): runs pump_propagate_cp_cow with the
+ * same arguments once per candidate count of particles per wave of the
+ * half-space test (64, 128, 256, 512), times three repetitions of each on
+ * the stream, keeps the fastest for the process under the configuration
+ * (validity form, precision, n_p, max_hs) -- every later batched pass of
+ * that configuration uses it -- and finishes with one call in the chosen
+ * shape, whose outputs (and counters) are the call's results.  The
+ * candidates' outputs are identical; *particles_per_wave receives the
+ * choice.  Synchronizes the stream. */
+int pump_hsmc_autotune_cow(int validity_bits, int num_new_paths, int waypoints_count,
+                           const int64_t *d_wp_offsets, const int *d_wf_edge,
+                           const int *d_wf_prev, const float *d_as, const float *d_bs,
+                           const uint64_t *d_edge_hs_mask, const float *d_times,
+                           const float *d_topts, float dt, int max_hs,
+                           const float *d_offsets_time, int tmax_steps, float offset_mult,
+                           int num_mc_particles, const void *d_valid_src,
+                           const float *d_parent_cp, void *d_valid_dst, int64_t dst_base,
+                           int *d_row_src, int n_p, float *d_path_cp, int64_t cp_base,
+                           int precision, const void *d_offsets_split,
+                           unsigned long long *d_band_count, unsigned long long *d_clamped,
+                           void *d_workspace, size_t workspace_bytes, pump_stream_t stream,
+                           int *particles_per_wave);
+
+/* Particles per wave the batched MC pass uses for a configuration: the
+ * tuned choice, or 256 where none was made. */
+int pump_hsmc_particles_per_wave(int validity_bits, int precision, int n_p, int max_hs);
+
+/* Sets the choice for a configuration by hand (64, 128, 256 or 512; 0
+ * forgets it).  No counterpart in the reference. */
+int pump_hsmc_set_particles_per_wave(int validity_bits, int precision, int n_p, int max_hs,
+                                     int particles_per_wave);
 
 /* Child sel[j] (j < n_sel; sel NULL: j itself) of a copy-on-write step to
  * row dst_base + j of d_dst, from d_children (rows relative to the step's

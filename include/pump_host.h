@@ -217,7 +217,12 @@ int pump_roadmap_device(const pump_roadmap *rm, pump_roadmap_dev *dev);
  * 256 problems with NUM in [1000, 8000].  The pair pass of every instance is one persistent
  * launch, the fused edge pass plus the half-space cache of every instance are
  * two more, each captured in a hipGraph and replayable.  Launch shapes are
- * chosen by timing the candidates on the device when autotune != 0. */
+ * chosen by timing the candidates on the device when autotune != 0: resident
+ * workgroups per CU of the pair pass, edges per workgroup and the obstacle
+ * tile (boxes staged in LDS) of the edge pass.  The choice is kept for the
+ * process, keyed by the configuration's shape (instance count, pair, edge
+ * and waypoint totals, largest box count), and a later build of the same
+ * shape reuses it without timing. */
 typedef struct pump_batch pump_batch;
 
 typedef struct pump_batch_instance {
@@ -238,6 +243,9 @@ typedef struct pump_batch_info {
     float ms_edges;           /* one launch of the edge-pass + half-space graph */
     float ms_autotune;        /* device time spent timing candidates */
     float ms_total;           /* wall time of the build */
+    int edge_obs_tile;        /* boxes staged in LDS (with candidate masks) by the edge pass */
+    int autotune_cached;      /* bit 0: pair-pass shape, bit 1: edge-pass shape reused from an
+                                 earlier build of the same configuration in this process */
 } pump_batch_info;
 
 int pump_batch_build(int n_inst, const pump_batch_instance *inst, const float *offsets_time_host,

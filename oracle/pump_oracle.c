@@ -473,7 +473,7 @@ void po_sample_free(const float *obstacles, int n_obs, const float *samples, int
     }
 }
 
-/* faceContainsProjection -- collisionCheck.cu:180-190 (closed intervals) */
+/* faceContainsProjection -- collisionCheck.cu:133-143 (closed intervals) */
 static int face_contains(const float *v, const float *w, float lambda, int j, const float *lo,
                          const float *hi)
 {
@@ -485,7 +485,7 @@ static int face_contains(const float *v, const float *w, float lambda, int j, co
     return 1;
 }
 
-/* motionValidQ -- collisionCheck.cu:161-178: per axis, the face on v's side
+/* motionValidQ -- collisionCheck.cu:114-131: per axis, the face on v's side
  * (lo if v<lo else hi), lambda unclamped and unguarded. */
 static int motion_valid_q(const float *v, const float *w, const float *lo, const float *hi)
 {
@@ -498,7 +498,7 @@ static int motion_valid_q(const float *v, const float *w, const float *lo, const
     return 1;
 }
 
-/* isMotionValid + broadphaseValidQ -- collisionCheck.cu:112-159.
+/* isMotionValid + broadphaseValidQ -- collisionCheck.cu:65-112.
  * Pinned by brute force on the segment/box geometry (an entering segment
  * must be reported) and separating-axis cases (tests/test_oracle_math.py). */
 int po_is_motion_valid(const float *v, const float *w, const float *bb_min, const float *bb_max,

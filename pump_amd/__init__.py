@@ -122,7 +122,7 @@ class BatchInfo(ctypes.Structure):
     _fields_ = [("n_inst", c_int), ("pairs_blocks_per_cu", c_int), ("edges_per_block", c_int),
                 ("n_pairs", c_int64), ("n_edges", c_int64), ("n_waypoints", c_int64),
                 ("ms_pairs", c_float), ("ms_edges", c_float), ("ms_autotune", c_float),
-                ("ms_total", c_float)]
+                ("ms_total", c_float), ("edge_obs_tile", c_int), ("autotune_cached", c_int)]
 
 
 class ProblemFile(ctypes.Structure):
@@ -160,6 +160,8 @@ _SIGS = {
     "pump_free_edges": (c_int, [S, c_int, S, S, c_int, S, S, c_int64, S]),
     "pump_free_edges_var": (c_int, [S, c_int, S, S, S, c_int64, S]),
     "pump_edges_discretize_check": (c_int, [S, S, S, S, c_int64, S, c_int, S, c_int, S, S, S]),
+    "pump_edges_discretize_check_fp32": (c_int, [S, S, S, S, c_int64, S, c_int, S, c_int, S, S,
+                                                 S, S]),
     "pump_setup_as_bs": (c_int, [c_int64, S, S, S]),
     "pump_maxd2": (c_float, [c_float]),
     "pump_cache_halfspaces": (c_int, [c_int64, S, S, c_int, c_int, S, c_int, S, S, S, S, c_float,
@@ -183,6 +185,12 @@ _SIGS = {
     "pump_propagate_cp_cow": (c_int, [c_int, c_int, c_int, S, S, S, S, S, S, S, S, c_float,
                                       c_int, S, c_int, c_float, c_int, S, S, S, c_int64, S,
                                       c_int, S, c_int64, c_int, S, S, S, S, c_size_t, S]),
+    "pump_hsmc_autotune_cow": (c_int, [c_int, c_int, c_int, S, S, S, S, S, S, S, S, c_float,
+                                       c_int, S, c_int, c_float, c_int, S, S, S, c_int64, S,
+                                       c_int, S, c_int64, c_int, S, S, S, S, c_size_t, S,
+                                       ctypes.POINTER(c_int)]),
+    "pump_hsmc_particles_per_wave": (c_int, [c_int, c_int, c_int, c_int]),
+    "pump_hsmc_set_particles_per_wave": (c_int, [c_int, c_int, c_int, c_int, c_int]),
     "pump_gather_rows_cow": (c_int, [S, S, S, S, c_int, S, c_int64, c_int64, c_int, S]),
     "pump_split_offsets_bytes": (c_size_t, [c_int, c_int]),
     "pump_split_offsets": (c_int, [S, c_int, c_int, S, S]),

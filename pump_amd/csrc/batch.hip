@@ -11,13 +11,17 @@
 // prefix table in device memory, so small instances share the chip instead of
 // each leaving most CUs idle.  Each launch sequence is captured in a hipGraph
 // and can be replayed.  The launch shape of the pair pass (resident
-// workgroups per CU) and of the edge pass (edges per workgroup) is chosen by
-// timing the candidates on the device.  The host-synchronous stages between
+// workgroups per CU) and of the edge pass (edges per workgroup, boxes staged
+// in LDS) is chosen by timing the candidates on the device, and kept for the
+// process per configuration shape.  The host-synchronous stages between
 // the passes (rn, edge numbering, neighbour tables, waypoint offsets) run per
 // instance on the batch's one stream, as in the single-instance build.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/pump_host.h"
@@ -48,7 +52,31 @@ namespace {
 
 constexpr int kEpb[3] = {64, 128, 256};    // edges per workgroup of the edge pass
 constexpr int kBpc[3] = {4, 8, 16};        // resident workgroups per CU of the pair pass
+constexpr int kObsTile[4] = {32, 64, 128, 256}; // boxes staged in LDS by the edge pass
 constexpr int64_t kTunePairs = 1ll << 26;  // pairs timed per pair-pass candidate
+
+// Tuned launch shapes, kept for the process.  Key: the stage, the instance
+// count, its totals (pairs; or edges and waypoints) and the largest box
+// count.  Value: the chosen candidate indices.
+using TuneKey = std::array<int64_t, 5>;
+std::mutex g_tune_mu;
+std::map<TuneKey, std::array<int, 2>> g_tuned;
+
+bool tuned_get(const TuneKey &k, std::array<int, 2> *v)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    const auto it = g_tuned.find(k);
+    if (it == g_tuned.end())
+        return false;
+    *v = it->second;
+    return true;
+}
+
+void tuned_put(const TuneKey &k, const std::array<int, 2> &v)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[k] = v;
+}
 
 template <typename T> int upload(T **d, const std::vector<T> &h, hipStream_t s)
 {
@@ -125,22 +153,30 @@ int pair_pass(pump_batch *b, int autotune)
     TRY(upload(&b->pair_prefix, prefix, s));
     TRY(dalloc(&b->counter, 1));
     int bpc = 8;
-    if (autotune) {
+    const TuneKey key{0, n, total, 0, 0};
+    std::array<int, 2> cached{};
+    if (autotune && tuned_get(key, &cached)) {
+        bpc = kBpc[cached[0]];
+        b->info.autotune_cached |= 1;
+    } else if (autotune) {
         // the first kTunePairs pairs (rewritten later by the captured pass)
         const int64_t sample = std::min(total, kTunePairs);
         float best = 0;
+        int pick = 1;
         EventTimer tm(s);
         TRY(fill_pairs_many(b->pair_jobs, b->pair_prefix, n, sample, bpc, b->counter, s));
-        for (int c : kBpc) {
+        for (int c = 0; c < 3; ++c) {
             tm.start();
-            TRY(fill_pairs_many(b->pair_jobs, b->pair_prefix, n, sample, c, b->counter, s));
+            TRY(fill_pairs_many(b->pair_jobs, b->pair_prefix, n, sample, kBpc[c], b->counter, s));
             const float ms = tm.stop();
             b->info.ms_autotune += ms;
             if (best == 0 || ms < best) {
                 best = ms;
-                bpc = c;
+                pick = c;
             }
         }
+        bpc = kBpc[pick];
+        tuned_put(key, {pick, 0});
     }
     b->info.pairs_blocks_per_cu = bpc;
     TRY(capture(b, 0, [&] {
@@ -182,28 +218,54 @@ int edge_pass(pump_batch *b, int autotune)
     }
     auto blocks = [&](int c) { return eprefix[(size_t)c * (n + 1) + n]; };
     auto prefix_of = [&](int c) { return b->edge_prefix + (size_t)c * (n + 1); };
-    int pick = 1;
-    if (autotune) {
+    // obstacle tiles: those below the largest box count, and the first that
+    // holds every box (the default)
+    int n_tiles = 0, tile_all = 0;
+    for (int t : kObsTile) {
+        ++n_tiles;
+        if (t >= max_obs)
+            break;
+    }
+    tile_all = n_tiles - 1;
+    int pick = 1, tile = tile_all;
+    int64_t n_wp = 0, n_edges = 0;
+    for (const pump_roadmap *rm : b->rms) {
+        n_wp += rm->n_wp;
+        n_edges += rm->n_edges;
+    }
+    const TuneKey key{1, n, n_edges, n_wp, max_obs};
+    std::array<int, 2> cached{};
+    if (autotune && tuned_get(key, &cached)) {
+        pick = cached[0];
+        tile = cached[1];
+        b->info.autotune_cached |= 2;
+    } else if (autotune) {
         // every candidate writes the same waypoints and flags
         float best = 0;
         EventTimer tm(s);
-        TRY(edges_many(b->edge_jobs, prefix_of(1), n, blocks(1), kEpb[1], max_obs, s));
-        for (int c = 0; c < 3; ++c) {
-            tm.start();
-            TRY(edges_many(b->edge_jobs, prefix_of(c), n, blocks(c), kEpb[c], max_obs, s));
-            const float ms = tm.stop();
-            b->info.ms_autotune += ms;
-            if (best == 0 || ms < best) {
-                best = ms;
-                pick = c;
+        TRY(edges_many(b->edge_jobs, prefix_of(1), n, blocks(1), kEpb[1], max_obs,
+                       kObsTile[tile_all], s));
+        for (int c = 0; c < 3; ++c)
+            for (int ti = 0; ti < n_tiles; ++ti) {
+                tm.start();
+                TRY(edges_many(b->edge_jobs, prefix_of(c), n, blocks(c), kEpb[c], max_obs,
+                               kObsTile[ti], s));
+                const float ms = tm.stop();
+                b->info.ms_autotune += ms;
+                if (best == 0 || ms < best) {
+                    best = ms;
+                    pick = c;
+                    tile = ti;
+                }
             }
-        }
+        tuned_put(key, {pick, tile});
     }
     b->info.edges_per_block = kEpb[pick];
+    b->info.edge_obs_tile = std::min(kObsTile[tile], std::max(max_obs, 1));
     const int64_t hs_blocks = hprefix[n];
     TRY(capture(b, 1, [&] {
         int r = edges_many(b->edge_jobs, prefix_of(pick), n, blocks(pick), kEpb[pick], max_obs,
-                           s);
+                           kObsTile[tile], s);
         if (r == PUMP_OK)
             r = cache_hs_many(b->hs_jobs, b->hs_prefix, n, hs_blocks, max_obs, max_hs, s);
         return r;

@@ -16,7 +16,10 @@
 //     pairs staged in LDS (selected with PUMP_HSMC_KERNEL=block);
 //   * the batched form for large wavefronts: classify, copy, and the
 //     time-binned half-space pass of hsmc_bins.hip.
+#include <array>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 
 #include "hsmc_wave.hpp"
 #include "pump_device.hpp"
@@ -1410,6 +1413,24 @@ __global__ __launch_bounds__(256) void k_zero_words(int *__restrict__ p, int64_t
         p[i] = 0;
 }
 
+// Particle groups of 64 per wave of the batched form's half-space test,
+// chosen by pump_hsmc_autotune_cow per configuration (validity form,
+// precision, particle count, half-space slots) and kept for the process;
+// 4 (256 particles) where nothing was tuned.
+using PpwKey = std::array<int, 4>;
+static std::mutex g_ppw_mu;
+static std::map<PpwKey, int> g_ppw;
+static thread_local int t_ppw_try = 0; // candidate being timed by the autotuner
+
+static int ppw_groups(bool bits, bool split, int n_p, int max_hs)
+{
+    if (t_ppw_try)
+        return t_ppw_try;
+    std::lock_guard<std::mutex> lk(g_ppw_mu);
+    const auto it = g_ppw.find(PpwKey{bits ? 1 : 0, split ? 1 : 0, n_p, max_hs});
+    return it == g_ppw.end() ? 4 : it->second;
+}
+
 // Launches the propagate + CP step for a wavefront: the batched form when a
 // workspace is given and the wavefront is large, else one wave-per-path (or,
 // on request, workgroup-per-path) launch.
@@ -1462,6 +1483,7 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
         B.cp_base = cp_base;
         B.split = split;
         B.band = band;
+        B.nq = ppw_groups(kBits, split != nullptr, n_p, max_hs);
         // control words, histogram and cursors, cleared by a kernel (safe
         // under graph capture)
         const int64_t n_zero = (int64_t)(L.tstart / 4);
@@ -1715,5 +1737,97 @@ extern "C" int pump_setup_valid_bits(uint64_t *d_valid, int n_p, int64_t n_paths
     hipLaunchKernelGGL(k_setup_bits, dim3(grid_for(total, 256)), dim3(256), 0, as_stream(stream),
                        d_valid, n_paths, n_p);
     PUMP_LAUNCHED();
+    return PUMP_OK;
+}
+
+extern "C" int pump_hsmc_autotune_cow(
+    int validity_bits, int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets,
+    const int *d_wf_edge, const int *d_wf_prev, const float *d_as, const float *d_bs,
+    const uint64_t *d_edge_hs_mask, const float *d_times, const float *d_topts, float dt,
+    int max_hs, const float *d_offsets_time, int tmax_steps, float offset_mult,
+    int num_mc_particles, const void *d_valid_src, const float *d_parent_cp, void *d_valid_dst,
+    int64_t dst_base, int *d_row_src, int n_p, float *d_path_cp, int64_t cp_base, int precision,
+    const void *d_offsets_split, unsigned long long *d_band_count, unsigned long long *d_clamped,
+    void *d_workspace, size_t workspace_bytes, pump_stream_t stream, int *particles_per_wave)
+{
+    hipStream_t s = as_stream(stream);
+    auto run = [&](unsigned long long *band, unsigned long long *clamped) {
+        return pump_propagate_cp_cow(validity_bits, num_new_paths, waypoints_count, d_wp_offsets,
+                                     d_wf_edge, d_wf_prev, d_as, d_bs, d_edge_hs_mask, d_times,
+                                     d_topts, dt, max_hs, d_offsets_time, tmax_steps, offset_mult,
+                                     num_mc_particles, d_valid_src, d_parent_cp, d_valid_dst,
+                                     dst_base, d_row_src, n_p, d_path_cp, cp_base, precision,
+                                     d_offsets_split, band, clamped, d_workspace, workspace_bytes,
+                                     stream);
+    };
+    static const int kCand[4] = {1, 2, 4, 8};
+    int best = 4;
+    float best_ms = 0;
+    int r = PUMP_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t ce = hipEventCreate(&e0);
+    if (ce == hipSuccess) {
+        ce = hipEventCreate(&e1);
+        if (ce != hipSuccess)
+            hipEventDestroy(e0);
+    }
+    if (ce != hipSuccess) {
+        note_hip_error(ce, __FILE__, __LINE__);
+        return PUMP_E_HIP;
+    }
+    for (int c : kCand) {
+        t_ppw_try = c;
+        r = run(nullptr, nullptr); // warm-up
+        if (r != PUMP_OK)
+            break;
+        hipEventRecord(e0, s);
+        for (int rep = 0; rep < 3 && r == PUMP_OK; ++rep)
+            r = run(nullptr, nullptr);
+        hipEventRecord(e1, s);
+        float ms = 0;
+        if (r == PUMP_OK && (hipEventSynchronize(e1) != hipSuccess ||
+                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            r = PUMP_E_HIP;
+        if (r != PUMP_OK)
+            break;
+        if (best_ms == 0 || ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    t_ppw_try = 0;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (r != PUMP_OK)
+        return r;
+    const bool split = validity_bits && precision == 1 && offset_mult > 0.0f &&
+                       offset_mult < INFINITY;
+    {
+        std::lock_guard<std::mutex> lk(g_ppw_mu);
+        g_ppw[PpwKey{validity_bits ? 1 : 0, split ? 1 : 0, n_p, max_hs}] = best;
+    }
+    if (particles_per_wave)
+        *particles_per_wave = 64 * best;
+    // the outputs, and the counters, from the chosen shape
+    return run(d_band_count, d_clamped);
+}
+
+extern "C" int pump_hsmc_particles_per_wave(int validity_bits, int precision, int n_p, int max_hs)
+{
+    return 64 * ppw_groups(validity_bits != 0, validity_bits && precision == 1, n_p, max_hs);
+}
+
+extern "C" int pump_hsmc_set_particles_per_wave(int validity_bits, int precision, int n_p,
+                                                int max_hs, int particles_per_wave)
+{
+    if (particles_per_wave != 0 && particles_per_wave != 64 && particles_per_wave != 128 &&
+        particles_per_wave != 256 && particles_per_wave != 512)
+        return PUMP_E_INVALID;
+    const PpwKey key{validity_bits ? 1 : 0, (validity_bits && precision == 1) ? 1 : 0, n_p, max_hs};
+    std::lock_guard<std::mutex> lk(g_ppw_mu);
+    if (particles_per_wave == 0)
+        g_ppw.erase(key);
+    else
+        g_ppw[key] = particles_per_wave / 64;
     return PUMP_OK;
 }

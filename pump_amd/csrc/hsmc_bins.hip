@@ -28,7 +28,7 @@ namespace {
 using namespace wave;
 
 constexpr int kScanBlock = 1024;
-constexpr int kTestGroups = 4; // particle groups of 64 per test wave
+constexpr int kTestGroups = 4; // particle groups of 64 per test wave, unless tuned
 constexpr int kSlotTile = 8;   // half-space slots staged per pass
 
 // exclusive scan of (x, y) over the block; returns the totals in *tx, *ty
@@ -166,11 +166,10 @@ __global__ __launch_bounds__(256) void k_bin_emit(const HsPath *__restrict__ rec
     }
 }
 
-// Binned test: one wave per (chunk, 64*kTestGroups particles).
-template <bool kBits>
+// Binned test: one wave per (chunk, 64*NQ particles).
+template <bool kBits, int NQ>
 __device__ __forceinline__ void bin_test(const BinArgs &a, int64_t n_chunks)
 {
-    constexpr int NQ = kTestGroups;
     const int lane = threadIdx.x & 63;
     const int n_p = a.n_p, max_hs = a.max_hs;
     const float om = a.om;
@@ -276,28 +275,28 @@ struct MfmaRow {
     int w;
 };
 
-template <bool kBits>
+template <bool kBits, int NQ>
 __device__ __forceinline__ void clear_range(const BinArgs &a, int w, int p_begin, int lane)
 {
     const int n_p = a.n_p, n_groups = (n_p + 63) >> 6;
     if (kBits) {
         // whole 64-bit words: bits past n_p are zero already
         const int g0 = p_begin >> 6;
-        for (int g = g0; g < g0 + kTestGroups && g < n_groups; ++g)
+        for (int g = g0; g < g0 + NQ && g < n_groups; ++g)
             atomicAnd((unsigned long long *)a.valid_dst + (a.dst_base + w) * n_groups + g, 0ull);
     } else {
         uint8_t *r = (uint8_t *)a.valid_dst + (a.dst_base + w) * n_p;
-        for (int p = p_begin; p < p_begin + 64 * kTestGroups && p < n_p; ++p)
+        for (int p = p_begin; p < p_begin + 64 * NQ && p < n_p; ++p)
             r[p] = 0;
     }
     (void)lane;
 }
 
-template <bool kBits>
+template <bool kBits, int NQ>
 __device__ __forceinline__ void bin_test_mfma(const BinArgs &a, int64_t n_chunks, MfmaRow *rows)
 {
     using namespace mfma;
-    constexpr int NQ = kTestGroups; // 64-particle groups per item (16-particle tiles: 4*NQ)
+    // NQ: 64-particle groups per item (16-particle tiles: 4*NQ)
     constexpr int NT = 4 * NQ;
     const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
     const int n_p = a.n_p, max_hs = a.max_hs;
@@ -344,7 +343,7 @@ __device__ __forceinline__ void bin_test_mfma(const BinArgs &a, int64_t n_chunks
             }
         }
         if (dead) {
-            clear_range<kBits>(a, my_w, pbase, lane);
+            clear_range<kBits, NQ>(a, my_w, pbase, lane);
             live = 0; // every particle of the range has failed already
         }
         const int cnt = __popc(live);
@@ -584,16 +583,36 @@ __device__ __forceinline__ void group_pass(const BinArgs &a, int64_t n_listed)
 // The listed paths' half-space tests, binned (FP32, or split-BF16 MFMA when
 // a particle-vector table is given); k_hs_groups runs instead when the
 // binned pairs overflowed.  (Separate kernels keep each one's registers.)
-template <bool kBits, bool kMfma>
+template <bool kBits, bool kMfma, int NQ>
 __global__ __launch_bounds__(256) void k_hs_pass(const BinArgs a)
 {
     if (a.ctl[kCtlOverflow] != 0)
         return;
     if (kMfma) {
         __shared__ MfmaRow rows[256 / 64][kMfmaRows];
-        bin_test_mfma<kBits>(a, a.ctl[kCtlChunks], rows[threadIdx.x >> 6]);
+        bin_test_mfma<kBits, NQ>(a, a.ctl[kCtlChunks], rows[threadIdx.x >> 6]);
     } else {
-        bin_test<kBits>(a, a.ctl[kCtlChunks]);
+        bin_test<kBits, NQ>(a, a.ctl[kCtlChunks]);
+    }
+}
+
+// k_hs_pass with the particle groups per wave given at run time (1, 2, 4, 8)
+template <bool kBits, bool kMfma>
+void launch_hs_pass(const BinArgs &a, int nq, unsigned grid, hipStream_t s)
+{
+    switch (nq) {
+    case 1:
+        hipLaunchKernelGGL((k_hs_pass<kBits, kMfma, 1>), dim3(grid), dim3(256), 0, s, a);
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_hs_pass<kBits, kMfma, 2>), dim3(grid), dim3(256), 0, s, a);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_hs_pass<kBits, kMfma, 8>), dim3(grid), dim3(256), 0, s, a);
+        break;
+    default:
+        hipLaunchKernelGGL((k_hs_pass<kBits, kMfma, kTestGroups>), dim3(grid), dim3(256), 0, s,
+                           a);
     }
 }
 
@@ -665,15 +684,16 @@ int bins_finish(const BinArgs &a, bool bits, hipStream_t s)
                        (const int *)a.hist, (const int *)a.tstart, a.cursor, a.entries);
     PUMP_LAUNCHED();
     const int64_t groups = (a.n_p + 63) / 64;
-    const unsigned g_hs = wave_blocks(a.chunk_cap * ((groups + kTestGroups - 1) / kTestGroups));
+    const int nq = (a.nq == 1 || a.nq == 2 || a.nq == 8) ? a.nq : kTestGroups;
+    const unsigned g_hs = wave_blocks(a.chunk_cap * ((groups + nq - 1) / nq));
     const unsigned g_gr = wave_blocks((int64_t)a.n_new * ((groups + kHsGroups - 1) / kHsGroups));
     const unsigned g_cp = wave_blocks(a.n_new);
     const bool mfma = a.split && a.max_hs <= kSlotTile;
     if (bits) {
         if (mfma)
-            hipLaunchKernelGGL((k_hs_pass<true, true>), dim3(g_hs), dim3(256), 0, s, a);
+            launch_hs_pass<true, true>(a, nq, g_hs, s);
         else
-            hipLaunchKernelGGL((k_hs_pass<true, false>), dim3(g_hs), dim3(256), 0, s, a);
+            launch_hs_pass<true, false>(a, nq, g_hs, s);
         PUMP_LAUNCHED();
         hipLaunchKernelGGL(k_hs_groups<true>, dim3(g_gr), dim3(256), 0, s, a);
         PUMP_LAUNCHED();
@@ -682,9 +702,9 @@ int bins_finish(const BinArgs &a, bool bits, hipStream_t s)
                            a.path_cp, a.cp_base);
     } else {
         if (mfma)
-            hipLaunchKernelGGL((k_hs_pass<false, true>), dim3(g_hs), dim3(256), 0, s, a);
+            launch_hs_pass<false, true>(a, nq, g_hs, s);
         else
-            hipLaunchKernelGGL((k_hs_pass<false, false>), dim3(g_hs), dim3(256), 0, s, a);
+            launch_hs_pass<false, false>(a, nq, g_hs, s);
         PUMP_LAUNCHED();
         hipLaunchKernelGGL(k_hs_groups<false>, dim3(g_gr), dim3(256), 0, s, a);
         PUMP_LAUNCHED();

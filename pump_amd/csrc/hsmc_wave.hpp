@@ -318,6 +318,8 @@ struct BinArgs {
     // table, and the count of FP32 re-tests in the margin band
     const uint16_t *split;
     unsigned long long *band;
+    // particle groups of 64 per wave of the binned test (1, 2, 4 or 8)
+    int nq;
 };
 int bins_scan(const BinArgs &a, hipStream_t s);
 int bins_finish(const BinArgs &a, bool bits, hipStream_t s);

@@ -348,7 +348,8 @@ __host__ __device__ __forceinline__ void edge_bbox(const float *x0, const float 
 }
 
 // ---------------------------------------------------------------------------
-// Segment vs axis-aligned box, collisionCheck.cu:112-190.
+// Segment vs axis-aligned box, collisionCheck.cu:65-143 (isMotionValid,
+// broadphaseValidQ, motionValidQ, faceContainsProjection).
 // Boxes are given by their position bounds lo[3], hi[3].
 // ---------------------------------------------------------------------------
 __host__ __device__ __forceinline__ bool face_contains(const float *v, const float *w, float lambda, int j,

@@ -83,9 +83,11 @@ struct HsJob {
 int fill_pairs_many(const PairJob *d_jobs, const int64_t *d_prefix, int n_inst, int64_t n_pairs,
                     int blocks_per_cu, unsigned long long *d_counter, hipStream_t s);
 // edges per workgroup of the batched edge pass: 64, 128 or 256
-// (max_obs: the largest box count of the instances)
+// (max_obs: the largest box count of the instances; obs_tile: boxes staged
+// in LDS with candidate masks, the rest are tested from global memory)
 int edges_many(const EdgeJob *d_jobs, const int64_t *d_block_prefix, int n_inst,
-               int64_t n_blocks, int edges_per_block, int max_obs, hipStream_t s);
+               int64_t n_blocks, int edges_per_block, int max_obs, int obs_tile,
+               hipStream_t s);
 constexpr int kHsEdgesPerBlock = 128;
 // (max_obs, max_hs: the largest box count and slot count of the instances)
 int cache_hs_many(const HsJob *d_jobs, const int64_t *d_block_prefix, int n_inst,

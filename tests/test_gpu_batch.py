@@ -93,8 +93,18 @@ def test_batch_matches_single_builds(noise):
     assert info.n_inst == len(probs)
     assert info.pairs_blocks_per_cu in (4, 8, 16)
     assert info.edges_per_block in (64, 128, 256)
-    print("batch", info.pairs_blocks_per_cu, info.edges_per_block, info.ms_pairs,
-          info.ms_edges, flush=True)
+    max_obs = max(int(o.shape[0]) for _, _, o, _, _ in probs)
+    assert info.edge_obs_tile == 32 or info.edge_obs_tile == min(64, max_obs)
+    assert info.autotune_cached == 0
+    print("batch", info.pairs_blocks_per_cu, info.edges_per_block, info.edge_obs_tile,
+          info.ms_pairs, info.ms_edges, flush=True)
+    # the same configuration again: the tuned shapes are reused, not re-timed
+    b2 = P.Batch([(s, o, g, prm) for _, s, o, g, prm in probs], ot, xc, autotune=True)
+    i2 = b2.info
+    assert i2.autotune_cached == 3 and i2.ms_autotune == 0
+    assert (i2.pairs_blocks_per_cu, i2.edges_per_block, i2.edge_obs_tile) == (
+        info.pairs_blocks_per_cu, info.edges_per_block, info.edge_obs_tile)
+    b2.close()
     singles = []
     n_pairs = n_edges = n_wp = 0
     for k, (name, s, o, g, prm) in enumerate(probs):
@@ -148,4 +158,50 @@ def test_batch_without_kept_pairs(noise):
     for k in hs_s:
         assert np.array_equal(bits(hs_b[k]), bits(hs_s[k])), k
     single.close()
+    b.close()
+
+
+def test_batch_all_256_instances(noise):
+    """The whole uneven 256-problem workload in one batch; a sample of its
+    instances (the smallest, the largest, the most boxes, the first, the last
+    and six drawn at random) against single-instance builds, and every
+    instance's counts against the batch totals."""
+    ot, xc = noise
+    batch = configs.batch_config()
+    assert len(batch) == 256
+    nums = np.array([c["spec"].num for c in batch])
+    n_obs = np.array([c["spec"].n_obs for c in batch])
+    rng = np.random.default_rng(256)
+    sample = sorted({0, 255, int(np.argmin(nums)), int(np.argmax(nums)), int(np.argmax(n_obs)),
+                     *(int(k) for k in rng.choice(256, 6, replace=False))})
+    probs = []
+    for k, cfg in enumerate(batch):
+        prm = cfg["params"]
+        prm.keep_pairs = 1 if k in sample else 0
+        s, o = P.gen_problem(cfg["spec"])
+        probs.append((cfg["name"], s, o, s[-1], prm))
+    print("sampled instances", [probs[k][0] for k in sample], flush=True)
+    b = P.Batch([(s, o, g, prm) for _, s, o, g, prm in probs], ot, xc, autotune=True)
+    info = b.info
+    assert info.n_inst == 256
+    print("batch of 256:", info.n_pairs, "pairs", info.n_edges, "edges", info.n_waypoints,
+          "waypoints", info.ms_pairs, "ms pairs", info.ms_edges, "ms edges", flush=True)
+    tot = np.zeros(3, np.int64)
+    for k in range(256):
+        i = b.roadmap(k).info
+        assert i.num == nums[k] and i.n_edges > 0 and i.n_waypoints >= 2 * i.n_edges
+        tot += (i.n_pairs, i.n_edges, i.n_waypoints)
+    assert tuple(tot) == (info.n_pairs, info.n_edges, info.n_waypoints)
+    for k in sample:
+        name, s, o, g, prm = probs[k]
+        rm = P.Roadmap(s, o, g, ot, xc, prm)
+        assert_same(tables(b.roadmap(k)), tables(rm), name)
+        rm.close()
+    # the edge graph replays over all 256 and rewrites the same tables
+    assert b.replay(1, reps=2) > 0
+    k = sample[len(sample) // 2]
+    name, s, o, g, prm = probs[k]
+    rm = P.Roadmap(s, o, g, ot, xc, prm)
+    assert_same(tables(b.roadmap(k)), tables(rm), name + " after replay")
+    rm.close()
     b.close()

@@ -30,6 +30,7 @@ import oracle
 import pump_amd as P
 from oracle import pipeline as OP
 from pump_amd import configs
+from test_gpu_edges import check_fp32_waypoints, fp32_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -116,6 +117,21 @@ def test_full_pipeline_parity(i, bits):
     assert_close_f32(hs["bs"], orc["bs"])
     assert_close_f32(hs["as_"], orc["as_"])
     print(cfg["name"], "edges", rm.info.n_edges, "free", int(f["is_free_edges"].sum()), flush=True)
+    # the FP32-waypoint edge pass on the same edges: the same flags bit for
+    # bit, waypoints within its stated bound of the oracle's
+    dv = rm.device()
+    E, n_wp = rm.info.n_edges, rm.info.n_waypoints
+    disc32 = P.DeviceArray(shape=(n_wp, 6), dtype=np.float32, fill=0)
+    flags32 = P.DeviceArray(shape=(E,), dtype=np.uint8, fill=9)
+    cnt = P.DeviceArray(shape=(1,), dtype=np.uint64, fill=0)
+    P.check("pump_edges_discretize_check_fp32", P.lib.pump_edges_discretize_check_fp32(
+        dv.samples, dv.edge_src, dv.edge_dst, dv.topts_edge, E, dv.wp_offsets, 0, dv.obstacles,
+        o.shape[0], disc32.p, flags32.p, cnt.p, None))
+    assert np.array_equal(flags32.numpy(), orc["is_free_edges"])
+    worst = check_fp32_waypoints(disc32.numpy(), orc["disc"], fp32_bound(
+        s, e["edge_src"], e["edge_dst"], e["topts_edge"], f["wp_offsets"]))
+    print(cfg["name"], "FP32 waypoints: flags equal, exact re-tests", int(cnt.numpy()[0]),
+          "largest error / bound", worst, flush=True)
     # the search on the oracle's own roadmap (equal to the device's above)
     got, _ = search_parity(rm, prm, orc, s, o, ot, xc)
     if got["soln_len"] > 0:

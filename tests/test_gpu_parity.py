@@ -451,6 +451,43 @@ def test_propagate_copy_on_write(n_new):
         rows2 = host(out2)
         got2 = unpack_bits(rows2.view(np.uint64), n_p) if bits else rows2
         assert np.array_equal(got2, ref_v[npar:][sel])
+        if n_new < 4096:
+            continue
+        # every particles-per-wave shape of the binned test gives the same
+        # rows and CPs; then the autotuner picks one of them
+        for ppw in (64, 128, 512, 0):
+            P.check("set_ppw", P.lib.pump_hsmc_set_particles_per_wave(
+                bits, precision, n_p, d["max_hs"], ppw))
+            assert P.lib.pump_hsmc_particles_per_wave(bits, precision, n_p, d["max_hs"]) == (
+                ppw or 256)
+            stage2 = dev(np.zeros((n_new, width), np.uint8))
+            cp2 = dev(np.zeros(n_new, np.float32))
+            P.check("propagate_cp_cow", P.lib.pump_propagate_cp_cow(
+                bits, n_new, 0, ptr(g["off"]), ptr(g["wf_edge"]), ptr(g["wf_prev"]),
+                ptr(g["as_"]), ptr(g["bs"]), ptr(g_mask), ptr(g["times"]), ptr(g["topts"]),
+                d["dt"], d["max_hs"], ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(src),
+                ptr(g_pcp), ptr(stage2), 0, ptr(row_src), n_p, ptr(cp2), 0, precision,
+                g_split.p if precision else None, None, None, ws.p, ws_bytes, None))
+            assert np.array_equal(host(cp2), ref_cp[npar:]), (bits, precision, ppw)
+            assert np.array_equal(host(stage2)[has_hs], host(stage)[has_hs]), (bits, ppw)
+        stage3 = dev(np.zeros((n_new, width), np.uint8))
+        cp3 = dev(np.zeros(n_new, np.float32))
+        cnt3 = dev(np.zeros(2, np.uint64))
+        chosen = ctypes.c_int(0)
+        P.check("hsmc_autotune_cow", P.lib.pump_hsmc_autotune_cow(
+            bits, n_new, 0, ptr(g["off"]), ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
+            ptr(g["bs"]), ptr(g_mask), ptr(g["times"]), ptr(g["topts"]), d["dt"], d["max_hs"],
+            ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(src), ptr(g_pcp), ptr(stage3), 0,
+            ptr(row_src), n_p, ptr(cp3), 0, precision, g_split.p if precision else None,
+            cnt3.ptr + 8, ptr(cnt3), ws.p, ws_bytes, None, ctypes.byref(chosen)))
+        assert chosen.value in (64, 128, 256, 512)
+        assert P.lib.pump_hsmc_particles_per_wave(bits, precision, n_p, d["max_hs"]) == chosen.value
+        assert np.array_equal(host(cp3), ref_cp[npar:])
+        assert np.array_equal(host(stage3)[has_hs], host(stage)[has_hs])
+        # the counters come from the final call only
+        assert int(host(cnt3)[0]) == ref_cl and host(cnt3)[1] == host(cnt)[1]
+        P.check("set_ppw", P.lib.pump_hsmc_set_particles_per_wave(
+            bits, precision, n_p, d["max_hs"], 0))
 
 
 def test_split_bf16_band_decisions():
