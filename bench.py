@@ -596,7 +596,11 @@ def search_pass(P, rm, prm):
     return {"ms_device_bookkeeping": d["ms_total"],
             "ms_device_first_search": out["device_first"]["ms_total"],
             "ms_host_bookkeeping": h["ms_total"],
-            "ms_propagate": d["ms_propagate"], "itrs": d["itrs"], "num_paths": d["num_paths"],
+            "ms_propagate": d["ms_propagate"],
+            "particle_steps": d["particle_steps"],
+            "propagate_particle_steps_per_s": (d["particle_steps"] / (d["ms_propagate"] / 1e3)
+                                               if d["ms_propagate"] > 0 else None),
+            "itrs": d["itrs"], "num_paths": d["num_paths"],
             "num_considered": d["num_considered"], "status": d["status"],
             "soln_cost": d["soln_cost"], "soln_cp": d["soln_cp"], "same_result": same,
             "smoothing": sm}

@@ -41,9 +41,11 @@ struct alignas(16) EdgeRec {
 };
 static_assert(sizeof(EdgeRec) == 144, "edge record layout");
 
-// A staged box: lower corner, upper corner (positions).
-struct alignas(8) BoxRec {
-    float lo[3], hi[3];
+// A staged box: lower corner, upper corner (positions), each padded to 16
+// bytes so that a box is two 16-byte LDS reads.
+struct alignas(16) BoxRec {
+    float lo[3], pad_lo;
+    float hi[3], pad_hi;
 };
 
 // Per-edge record of the FP32-waypoint form: per position axis the cubic
@@ -142,12 +144,17 @@ __device__ __forceinline__ SegBox seg_box32(const float *v, const float *w, cons
 // the reference's broad phase rejects the box for every segment within the
 // margin of v -> w.  A NaN coordinate comes only from a non-finite
 // coefficient of its axis, whose margin is then infinite: its bounds are
-// NaN or infinite and reject nothing.
+// NaN or infinite and reject nothing.  Branch-free: up <= lo iff
+// lo - up >= 0 (IEEE subtraction keeps the sign of the exact difference,
+// f32 denormals are kept), and fmaxf passes over a NaN term, so an axis
+// whose bounds are NaN neither rejects nor hides the other axes' verdicts.
 __device__ __forceinline__ bool surely_separated(const SegBox &s, const float *lo,
                                                  const float *hi)
 {
-    return (s.up[0] <= lo[0]) | (hi[0] <= s.dn[0]) | (s.up[1] <= lo[1]) | (hi[1] <= s.dn[1]) |
-           (s.up[2] <= lo[2]) | (hi[2] <= s.dn[2]);
+    const float g0 = fmaxf(lo[0] - s.up[0], s.dn[0] - hi[0]);
+    const float g1 = fmaxf(lo[1] - s.up[1], s.dn[1] - hi[1]);
+    const float g2 = fmaxf(lo[2] - s.up[2], s.dn[2] - hi[2]);
+    return fmaxf(fmaxf(g0, g1), g2) >= 0.0f;
 }
 
 constexpr int kMiss = 0, kHit = 1, kUndecided = 2;
@@ -237,7 +244,7 @@ __host__ __device__ inline EdgeLds edge_lds(int epb, int obs_cap, int words_cap,
         return at;
     };
     L.recs = take(epb * rec_bytes, 16);
-    L.obs = take(obs_cap * (int)sizeof(BoxRec), 8);
+    L.obs = take(obs_cap * (int)sizeof(BoxRec), 16);
     L.emask = take(epb * words_cap * 8, 8);
     L.loff = take((epb + 1) * 4, 4);
     L.ok = take(epb, 4);
@@ -451,6 +458,8 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
             const int b = i / 3, d = i - 3 * (i / 3);
             boxes[b].lo[d] = obstacles[b * 12 + d];
             boxes[b].hi[d] = obstacles[b * 12 + 6 + d];
+            if (d == 0)
+                boxes[b].pad_lo = boxes[b].pad_hi = 0.0f;
         }
     for (int i = t; i <= ne; i += kTile)
         loff[i] = (int)((wp_offsets ? wp_offsets[e0 + i] : (e0 + i) * W_fixed) - g_begin);
