@@ -175,6 +175,8 @@ def main():
                   "frac_hbm_peak": alg_bytes / (ms_fp32 / 1e3) / HBM_PEAK,
                   "vs_exact_fused": ms_step / ms_fp32, "vs_plain": ms_plain / ms_fp32,
                   "exact_retests_per_pass": n_retests,
+                  "traffic": (edge_traffic("fp32_form")
+                              if not args.num and args.config == 1 else None),
                   "flags_equal": bool(np.array_equal(flags32.numpy(), fused_flags))}
     log(f"FP32-waypoint edge pass: {ms_fp32:.3f} ms, {E / (ms_fp32 / 1e3):.3e} edge-checks/s, "
         f"{edges_fp32['frac_hbm_peak'] * 100:.1f}% of HBM peak, {n_retests} exact re-tests, "
@@ -214,13 +216,7 @@ def main():
         if hsmc_cp is not None:
             cpu["particle_steps"] = cpu_particles(rm, prm, ot, hsmc_cp, args.cpu_seconds / 3)
 
-    traffic = None
-    tpath = os.path.join(HERE, "profiles", "edge_pass_num16000_traffic.json")
-    if os.path.exists(tpath) and not args.num and args.config == 1:
-        try:
-            traffic = json.load(open(tpath)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+    traffic = edge_traffic(None) if not args.num and args.config == 1 else None
 
     line = {
         "metric": METRIC,
@@ -664,6 +660,19 @@ def batch_pass(P, n_inst, reps):
     ms_edges_single = timed(single_edges)
     log(f"batch pair pass: {ms_pairs:.2f} ms graph vs {ms_pairs_single:.2f} ms per instance; "
         f"edge pass + half-spaces: {ms_edges:.3f} ms graph vs {ms_edges_single:.3f} ms")
+    # the HSMC step of every instance (each instance's free edges expanded
+    # once from 4096 all-valid parents), captured as the third graph
+    b.mc_build(n_parents=4096, max_paths=1 << 20, seed=1)
+    b.replay(2)
+    ms_mc = b.replay(2, reps)
+    ms_mc_eager = b.mc_eager(reps)
+    mc_steps = 0
+    for k, rm in enumerate(rms):
+        e = b.mc_get(k)["edge"]
+        wo = rm.flags()["wp_offsets"]
+        mc_steps += int(probs[k][3].n_p) * int((wo[e + 1] - wo[e]).sum())
+    log(f"batch HSMC step: {ms_mc:.2f} ms graph vs {ms_mc_eager:.2f} ms eager, "
+        f"{mc_steps / (ms_mc / 1e3):.3e} particle-steps/s")
     out = {"n_inst": int(info.n_inst), "n_pairs": int(info.n_pairs),
            "n_edges": int(info.n_edges), "n_waypoints": int(info.n_waypoints),
            "pairs_blocks_per_cu": int(info.pairs_blocks_per_cu),
@@ -674,7 +683,10 @@ def batch_pass(P, n_inst, reps):
                      "pair_solves_per_s": info.n_pairs / (ms_pairs / 1e3)},
            "edges_and_halfspaces": {"ms_batched": ms_edges, "ms_per_instance": ms_edges_single,
                                     "speedup": ms_edges_single / ms_edges,
-                                    "edge_checks_per_s": info.n_edges / (ms_edges / 1e3)}}
+                                    "edge_checks_per_s": info.n_edges / (ms_edges / 1e3)},
+           "edge_obs_tile": int(info.edge_obs_tile),
+           "hsmc": {"ms_graph": ms_mc, "ms_eager": ms_mc_eager, "particle_steps": mc_steps,
+                    "particle_steps_per_s": mc_steps / (ms_mc / 1e3)}}
     b.close()
     return out
 
@@ -816,6 +828,20 @@ def cpu_particles(rm, prm, offsets_time, gpu_cp, seconds):
     return {"value": steps / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
             "sample": f"first {R} of {n_new} children of the synthetic wavefront", "seconds": dt,
             "cp_match_gpu": agree}
+
+
+def edge_traffic(form):
+    """HBM bytes per launch of the edge pass at NUM=16000 from the committed
+    counter runs (profiles/edge_pass_num16000_traffic.json): the exact fused
+    pass, or the FP32-waypoint form with form="fp32_form"; None if absent."""
+    path = os.path.join(HERE, "profiles", "edge_pass_num16000_traffic.json")
+    try:
+        j = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    if form:
+        j = j.get(form) or {}
+    return j.get("bytes_per_launch")
 
 
 def mc_counters():

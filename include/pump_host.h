@@ -255,8 +255,25 @@ int pump_batch_info_get(const pump_batch *b, pump_batch_info *info);
 /* the roadmap of instance k, owned by the batch (pump_roadmap_free ignores it) */
 int pump_batch_roadmap(pump_batch *b, int k, pump_roadmap **rm);
 /* replays a captured stage reps times: 0 = pair pass (needs keep_pairs on
- * every instance), 1 = edge pass + half-space cache; ms = mean per replay */
+ * every instance), 1 = edge pass + half-space cache, 2 = the MC step of every
+ * instance (pump_batch_mc_build); ms = mean per replay */
 int pump_batch_replay(pump_batch *b, int stage, int reps, float *ms);
+
+/* The HSMC step (PUMP.cu:94-160, copy-on-write form, bit validity, FP32
+ * tests) of every instance, captured as the batch's third graph.  Each
+ * instance's wavefront is its free edges in edge order (at most max_paths),
+ * child i expanded from parent i % n_parents; the parents are all-valid
+ * rows with CP 0 and times uniform in [0, 2) s drawn from seed (splitmix64).
+ * The graph runs the instances' steps one after another on the batch
+ * stream; ms receives its first launch.  Needs the batch's noise tables. */
+int pump_batch_mc_build(pump_batch *b, int n_parents, int max_paths, uint64_t seed, float *ms);
+/* the same steps launched eagerly, reps times; ms = mean per pass */
+int pump_batch_mc_eager(pump_batch *b, int reps, float *ms);
+/* instance k's wavefront and results (the first cap entries; times: the
+ * first cap parent times): children's CPs and row sources (-1 - parent for
+ * a child that met no half-space) */
+int pump_batch_mc_get(pump_batch *b, int k, int cap, int *n_new, int *edge, int *prev,
+                      float *times, float *cp, int *row_src);
 void pump_batch_free(pump_batch *b);
 
 typedef struct pump_search_result {

@@ -236,6 +236,9 @@ _SIGS = {
     "pump_batch_info_get": (c_int, [c_void_p, ctypes.POINTER(BatchInfo)]),
     "pump_batch_roadmap": (c_int, [c_void_p, c_int, ctypes.POINTER(c_void_p)]),
     "pump_batch_replay": (c_int, [c_void_p, c_int, c_int, f32p]),
+    "pump_batch_mc_build": (c_int, [c_void_p, c_int, c_int, c_uint64, f32p]),
+    "pump_batch_mc_eager": (c_int, [c_void_p, c_int, f32p]),
+    "pump_batch_mc_get": (c_int, [c_void_p, c_int, c_int, i32p, i32p, i32p, f32p, f32p, i32p]),
     "pump_batch_free": (None, [c_void_p]),
     "pump_find_optimal_path": (c_int, [c_float, f32p, c_int, f32p, c_int, f32p]),
     "pump_dev_alloc": (c_void_p, [c_size_t]),
@@ -641,8 +644,38 @@ class Batch:
         smp, obs, _, g, prm = self._keep[k]
         return Roadmap._borrowed(h, self, smp, obs, g, prm)
 
+    def mc_build(self, n_parents=4096, max_paths=1 << 20, seed=1):
+        """Captures the MC step of every instance (stage 2); ms of its first launch."""
+        ms = c_float()
+        check("pump_batch_mc_build", lib.pump_batch_mc_build(self._h, n_parents, max_paths, seed,
+                                                             ctypes.byref(ms)))
+        self._mc_parents = n_parents
+        return ms.value
+
+    def mc_eager(self, reps=1):
+        """Mean ms of the same MC steps launched without the graph."""
+        ms = c_float()
+        check("pump_batch_mc_eager", lib.pump_batch_mc_eager(self._h, reps, ctypes.byref(ms)))
+        return ms.value
+
+    def mc_get(self, k):
+        """Instance k's MC wavefront and results: dict of edge, prev, times, cp, row_src."""
+        n = c_int()
+        check("pump_batch_mc_get", lib.pump_batch_mc_get(self._h, k, 0, ctypes.byref(n), None,
+                                                         None, None, None, None))
+        n_new = n.value
+        cap = max(n_new, self._mc_parents)
+        edge, prev = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        times, cp = np.zeros(cap, np.float32), np.zeros(cap, np.float32)
+        rs = np.zeros(cap, np.int32)
+        check("pump_batch_mc_get", lib.pump_batch_mc_get(self._h, k, cap, ctypes.byref(n), ip(edge),
+                                                         ip(prev), fp(times), fp(cp), ip(rs)))
+        return dict(edge=edge[:n_new], prev=prev[:n_new], times=times[:self._mc_parents],
+                    cp=cp[:n_new], row_src=rs[:n_new])
+
     def replay(self, stage, reps=1):
-        """Mean ms per replay of stage 0 (pair pass) or 1 (edge pass + half-spaces)."""
+        """Mean ms per replay of stage 0 (pair pass), 1 (edge pass + half-spaces)
+        or 2 (the MC step of every instance)."""
         ms = c_float()
         check("pump_batch_replay", lib.pump_batch_replay(self._h, stage, reps, ctypes.byref(ms)))
         return ms.value

@@ -31,6 +31,18 @@
 using namespace pump;
 using namespace pump::rmb;
 
+// One instance's MC step in the batch's third graph: its wavefront (free
+// edges expanded from all-valid parents), parent and child validity bits,
+// parent and child CPs, row sources and workspace.
+struct BatchMc {
+    int *edge = nullptr, *prev = nullptr, *row_src = nullptr;
+    float *times = nullptr, *parent_cp = nullptr, *cp = nullptr;
+    uint64_t *src = nullptr, *dst = nullptr;
+    char *ws = nullptr;
+    size_t ws_bytes = 0;
+    int n_new = 0;
+};
+
 struct pump_batch {
     hipStream_t stream = nullptr;
     std::vector<pump_roadmap *> rms;
@@ -43,9 +55,11 @@ struct pump_batch {
     int64_t *edge_prefix = nullptr; // one table of n_inst + 1 per edges-per-block candidate
     HsJob *hs_jobs = nullptr;
     int64_t *hs_prefix = nullptr;
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[3] = {nullptr, nullptr, nullptr};
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};
     pump_batch_info info{};
+    std::vector<BatchMc> mc;
+    int mc_parents = 0;
 };
 
 namespace {
@@ -335,11 +349,22 @@ extern "C" void pump_batch_free(pump_batch *b)
         return;
     if (b->stream)
         hipStreamSynchronize(b->stream);
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < 3; ++i) {
         if (b->exec[i])
             hipGraphExecDestroy(b->exec[i]);
         if (b->graph[i])
             hipGraphDestroy(b->graph[i]);
+    }
+    for (BatchMc &m : b->mc) {
+        dfree(m.edge);
+        dfree(m.prev);
+        dfree(m.row_src);
+        dfree(m.times);
+        dfree(m.parent_cp);
+        dfree(m.cp);
+        dfree(m.src);
+        dfree(m.dst);
+        dfree(m.ws);
     }
     for (pump_roadmap *rm : b->rms)
         rm_release(rm);
@@ -404,7 +429,7 @@ extern "C" int pump_batch_roadmap(pump_batch *b, int k, pump_roadmap **rm)
 
 extern "C" int pump_batch_replay(pump_batch *b, int stage, int reps, float *ms)
 {
-    if (!b || stage < 0 || stage > 1 || reps < 1 || !b->exec[stage])
+    if (!b || stage < 0 || stage > 2 || reps < 1 || !b->exec[stage])
         return PUMP_E_INVALID;
     EventTimer tm(b->stream);
     tm.start();
@@ -414,5 +439,136 @@ extern "C" int pump_batch_replay(pump_batch *b, int stage, int reps, float *ms)
     TRYH(hipStreamSynchronize(b->stream));
     if (ms)
         *ms = t / reps;
+    return PUMP_OK;
+}
+
+namespace {
+
+// 64-bit splitmix step: the parent times of the batch's MC wavefronts
+uint64_t splitmix(uint64_t &x)
+{
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// the MC step (copy-on-write, bit validity, FP32 tests) of every instance
+// with a wavefront, in instance order on the batch stream
+int mc_launch(pump_batch *b)
+{
+    for (size_t k = 0; k < b->rms.size(); ++k) {
+        const BatchMc &m = b->mc[k];
+        if (m.n_new == 0)
+            continue;
+        const pump_roadmap *rm = b->rms[k];
+        TRY(pump_propagate_cp_cow(
+            1, m.n_new, 0, rm->wp_offsets, m.edge, m.prev, rm->as, rm->bs, rm->edge_hs_mask,
+            m.times, rm->topts_edge, rm->prm.dt, rm->max_hs, b->offsets_time, rm->tmax_steps,
+            rm->prm.offset_mult, rm->n_mc, m.src, m.parent_cp, m.dst, 0, m.row_src, rm->prm.n_p,
+            m.cp, 0, 0, nullptr, nullptr, nullptr, m.ws, m.ws_bytes, (pump_stream_t)b->stream));
+    }
+    return PUMP_OK;
+}
+
+} // namespace
+
+extern "C" int pump_batch_mc_build(pump_batch *b, int n_parents, int max_paths, uint64_t seed,
+                                   float *ms)
+{
+    if (!b || n_parents < 1 || max_paths < 1 || !b->offsets_time || !b->mc.empty())
+        return PUMP_E_INVALID;
+    hipStream_t s = b->stream;
+    b->mc.resize(b->rms.size());
+    b->mc_parents = n_parents;
+    std::vector<uint8_t> free_h;
+    for (size_t k = 0; k < b->rms.size(); ++k) {
+        const pump_roadmap *rm = b->rms[k];
+        BatchMc &m = b->mc[k];
+        if (!rm->as || !rm->is_free_edges || rm->prm.n_p <= 0 || rm->n_mc <= 0 ||
+            rm->prm.n_p > rm->n_mc)
+            return PUMP_E_INVALID;
+        // the wavefront: the instance's free edges in edge order, child i
+        // from parent i % n_parents
+        free_h.resize((size_t)rm->n_edges);
+        TRYH(hipMemcpyAsync(free_h.data(), rm->is_free_edges, free_h.size(), hipMemcpyDeviceToHost,
+                            s));
+        TRYH(hipStreamSynchronize(s));
+        std::vector<int> edge, prev;
+        for (int64_t e = 0; e < rm->n_edges && (int)edge.size() < max_paths; ++e)
+            if (free_h[(size_t)e]) {
+                prev.push_back((int)(edge.size() % (size_t)n_parents));
+                edge.push_back((int)e);
+            }
+        std::vector<float> times((size_t)n_parents);
+        uint64_t x = seed ^ (0xD1B54A32D192ED03ull * (k + 1));
+        for (float &t : times)
+            t = (float)(2.0 * (double)(splitmix(x) >> 11) * 0x1p-53);
+        m.n_new = (int)edge.size();
+        const int nw = (rm->prm.n_p + 63) / 64;
+        TRY(upload(&m.times, times, s));
+        TRY(dalloc(&m.parent_cp, (size_t)n_parents));
+        TRYH(hipMemsetAsync(m.parent_cp, 0, sizeof(float) * n_parents, s));
+        TRY(dalloc(&m.src, (size_t)n_parents * nw));
+        TRY(pump_setup_valid_bits(m.src, rm->prm.n_p, n_parents, (pump_stream_t)s));
+        if (m.n_new == 0) {
+            TRYH(hipStreamSynchronize(s));
+            continue;
+        }
+        TRY(upload(&m.edge, edge, s));
+        TRY(upload(&m.prev, prev, s));
+        TRY(dalloc(&m.row_src, (size_t)m.n_new));
+        TRY(dalloc(&m.cp, (size_t)m.n_new));
+        TRY(dalloc(&m.dst, (size_t)m.n_new * nw));
+        m.ws_bytes = pump_propagate_workspace(m.n_new, rm->tmax_steps);
+        TRY(dalloc(&m.ws, m.ws_bytes));
+        // the host vectors end with this iteration: finish their uploads
+        TRYH(hipStreamSynchronize(s));
+    }
+    TRYH(hipStreamSynchronize(s));
+    TRY(capture(b, 2, [&] { return mc_launch(b); }));
+    float t = 0;
+    TRY(launch_timed(b, 2, &t));
+    if (ms)
+        *ms = t;
+    return PUMP_OK;
+}
+
+extern "C" int pump_batch_mc_eager(pump_batch *b, int reps, float *ms)
+{
+    if (!b || reps < 1 || b->mc.empty())
+        return PUMP_E_INVALID;
+    EventTimer tm(b->stream);
+    tm.start();
+    for (int i = 0; i < reps; ++i)
+        TRY(mc_launch(b));
+    const float t = tm.stop();
+    TRYH(hipStreamSynchronize(b->stream));
+    if (ms)
+        *ms = t / reps;
+    return PUMP_OK;
+}
+
+extern "C" int pump_batch_mc_get(pump_batch *b, int k, int cap, int *n_new, int *edge, int *prev,
+                                 float *times, float *cp, int *row_src)
+{
+    if (!b || k < 0 || k >= (int)b->mc.size() || cap < 0 || !n_new)
+        return PUMP_E_INVALID;
+    const BatchMc &m = b->mc[(size_t)k];
+    *n_new = m.n_new;
+    const size_t n = (size_t)std::min(cap, m.n_new);
+    hipStream_t s = b->stream;
+    TRYH(hipStreamSynchronize(s));
+    if (edge && n)
+        TRYH(hipMemcpy(edge, m.edge, n * sizeof(int), hipMemcpyDeviceToHost));
+    if (prev && n)
+        TRYH(hipMemcpy(prev, m.prev, n * sizeof(int), hipMemcpyDeviceToHost));
+    if (cp && n)
+        TRYH(hipMemcpy(cp, m.cp, n * sizeof(float), hipMemcpyDeviceToHost));
+    if (row_src && n)
+        TRYH(hipMemcpy(row_src, m.row_src, n * sizeof(int), hipMemcpyDeviceToHost));
+    if (times)
+        TRYH(hipMemcpy(times, m.times, (size_t)std::min(cap, b->mc_parents) * sizeof(float),
+                       hipMemcpyDeviceToHost));
     return PUMP_OK;
 }

@@ -205,3 +205,51 @@ def test_batch_all_256_instances(noise):
     assert_same(tables(b.roadmap(k)), tables(rm), name + " after replay")
     rm.close()
     b.close()
+
+
+def test_batch_mc_step(noise):
+    """The HSMC step of every instance captured as the batch's third graph
+    (pump_batch_mc_build): each instance's children's CPs equal the CPU
+    oracle's propagateValidParticles + calculateCP_HSMC on the same wavefront
+    and tables, after graph replays and an eager pass too."""
+    import oracle
+    from oracle import bp, fp, ip, lp
+    O = oracle.load()
+    ot, xc = noise
+    probs = problems()
+    b = P.Batch([(s, o, g, prm) for _, s, o, g, prm in probs], ot, xc, autotune=False)
+    n_par = 64
+    ms = b.mc_build(n_parents=n_par, max_paths=20000, seed=7)
+    print("batch MC step, first launch", ms, "ms", flush=True)
+
+    def check_all(label):
+        for k in range(len(probs)):
+            name, s, o, g, prm = probs[k]
+            rm = b.roadmap(k)
+            got = b.mc_get(k)
+            n_new = got["edge"].size
+            fl, hs, e = rm.flags(), rm.halfspaces(), rm.edges()
+            free = np.nonzero(fl["is_free_edges"])[0][:20000]
+            assert np.array_equal(got["edge"], free), (name, label)
+            assert np.array_equal(got["prev"], np.arange(n_new) % n_par)
+            n_p = int(prm.n_p)
+            valid = np.zeros((n_par + n_new, n_p), np.uint8)
+            valid[:n_par] = 1
+            O.po_propagate_valid_particles(
+                n_new, lp(fl["wp_offsets"]), ip(got["edge"]), ip(got["prev"]), n_par,
+                fp(hs["as_"]), fp(hs["bs"]), fp(got["times"]), fp(e["topts_edge"]), prm.dt,
+                rm.info.max_hs, fp(ot), rm.info.tmax_steps, prm.offset_mult, rm.info.n_mc,
+                bp(valid), n_p, 8)
+            cp = np.zeros(n_par + n_new, np.float32)
+            O.po_calculate_cp_hsmc(n_new, n_par, fp(cp), bp(valid), n_p)
+            assert np.array_equal(got["cp"], cp[n_par:]), (name, label)
+            # children that meet no half-space keep their parent's row
+            own = got["row_src"] >= 0
+            assert np.array_equal(got["row_src"][~own], -1 - got["prev"][~own])
+
+    check_all("first launch")
+    assert b.replay(2, reps=2) > 0
+    check_all("after replays")
+    print("eager", b.mc_eager(reps=2), "ms; graph", b.replay(2, reps=3), "ms", flush=True)
+    check_all("after the eager pass")
+    b.close()
