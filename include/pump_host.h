@@ -264,8 +264,11 @@ int pump_batch_replay(pump_batch *b, int stage, int reps, float *ms);
  * instance's wavefront is its free edges in edge order (at most max_paths),
  * child i expanded from parent i % n_parents; the parents are all-valid
  * rows with CP 0 and times uniform in [0, 2) s drawn from seed (splitmix64).
- * The graph runs the instances' steps one after another on the batch
- * stream; ms receives its first launch.  Needs the batch's noise tables. */
+ * The steps are independent: they fork from the batch stream onto eight
+ * side streams of default priority (instance k on stream k mod 8) and join
+ * back, so the graph has one branch per side stream and kernels of
+ * different instances overlap; ms receives its first launch.  Needs the
+ * batch's noise tables. */
 int pump_batch_mc_build(pump_batch *b, int n_parents, int max_paths, uint64_t seed, float *ms);
 /* the same steps launched eagerly, reps times; ms = mean per pass */
 int pump_batch_mc_eager(pump_batch *b, int reps, float *ms);

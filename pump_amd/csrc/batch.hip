@@ -60,6 +60,9 @@ struct pump_batch {
     pump_batch_info info{};
     std::vector<BatchMc> mc;
     int mc_parents = 0;
+    // side streams (default priority) and fork/join events of the MC stage
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev;
 };
 
 namespace {
@@ -355,6 +358,10 @@ extern "C" void pump_batch_free(pump_batch *b)
         if (b->graph[i])
             hipGraphDestroy(b->graph[i]);
     }
+    for (hipStream_t st : b->side)
+        hipStreamDestroy(st);
+    for (hipEvent_t e : b->ev)
+        hipEventDestroy(e);
     for (BatchMc &m : b->mc) {
         dfree(m.edge);
         dfree(m.prev);
@@ -453,10 +460,19 @@ uint64_t splitmix(uint64_t &x)
     return z ^ (z >> 31);
 }
 
-// the MC step (copy-on-write, bit validity, FP32 tests) of every instance
-// with a wavefront, in instance order on the batch stream
+constexpr int kMcStreams = 8; // side streams the MC stage spreads its instances over
+
+// The MC step (copy-on-write, bit validity, FP32 tests) of every instance
+// with a wavefront.  The steps are independent, so they fork from the batch
+// stream onto the side streams (instance k on stream k mod kMcStreams) and
+// join back: kernels of different instances overlap on the chip, and under
+// capture the graph gets one branch per side stream.
 int mc_launch(pump_batch *b)
 {
+    const int S = (int)b->side.size();
+    TRYH(hipEventRecord(b->ev[0], b->stream));
+    for (int j = 0; j < S; ++j)
+        TRYH(hipStreamWaitEvent(b->side[j], b->ev[0], 0));
     for (size_t k = 0; k < b->rms.size(); ++k) {
         const BatchMc &m = b->mc[k];
         if (m.n_new == 0)
@@ -466,7 +482,12 @@ int mc_launch(pump_batch *b)
             1, m.n_new, 0, rm->wp_offsets, m.edge, m.prev, rm->as, rm->bs, rm->edge_hs_mask,
             m.times, rm->topts_edge, rm->prm.dt, rm->max_hs, b->offsets_time, rm->tmax_steps,
             rm->prm.offset_mult, rm->n_mc, m.src, m.parent_cp, m.dst, 0, m.row_src, rm->prm.n_p,
-            m.cp, 0, 0, nullptr, nullptr, nullptr, m.ws, m.ws_bytes, (pump_stream_t)b->stream));
+            m.cp, 0, 0, nullptr, nullptr, nullptr, m.ws, m.ws_bytes,
+            (pump_stream_t)b->side[k % (size_t)S]));
+    }
+    for (int j = 0; j < S; ++j) {
+        TRYH(hipEventRecord(b->ev[1 + j], b->side[j]));
+        TRYH(hipStreamWaitEvent(b->stream, b->ev[1 + j], 0));
     }
     return PUMP_OK;
 }
@@ -479,6 +500,16 @@ extern "C" int pump_batch_mc_build(pump_batch *b, int n_parents, int max_paths, 
     if (!b || n_parents < 1 || max_paths < 1 || !b->offsets_time || !b->mc.empty())
         return PUMP_E_INVALID;
     hipStream_t s = b->stream;
+    for (int j = 0; j < kMcStreams; ++j) {
+        hipStream_t st = nullptr;
+        TRYH(hipStreamCreate(&st));
+        b->side.push_back(st);
+    }
+    for (int j = 0; j <= kMcStreams; ++j) {
+        hipEvent_t e = nullptr;
+        TRYH(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->ev.push_back(e);
+    }
     b->mc.resize(b->rms.size());
     b->mc_parents = n_parents;
     std::vector<uint8_t> free_h;

@@ -394,6 +394,68 @@ __device__ __forceinline__ int seg_state32(const float *v, const float *w, const
     return state;
 }
 
+// The reference's own test on segment (k, k+1) of block-local edge le: its
+// FP64 waypoints k and k+1 (pathPoint, 2pBVP.cu:117-131, from the edge's
+// EdgePoly) against the edge's candidate boxes (collisionCheck.cu:65-143).
+// A hit clears the edge's flag (only zeros are written: concurrent writes
+// agree).  Out of line: it runs a few times per pass, and its FP64
+// registers, and any spills, stay off the rest of the kernel.
+__device__ __noinline__ void exact32(const EdgeRec32 *recs, const int *loff,
+                                     const unsigned long long *emask_row, int n_words,
+                                     const BoxRec *boxes, const float *__restrict__ obstacles,
+                                     int n_masked, int n_obs, uint8_t *ok, int le, int k)
+{
+    const EdgeRec32 &R = recs[le];
+    const int Wq = loff[le + 1] - loff[le];
+    const float x0[kDim] = {R.c[0][0], R.c[1][0], R.c[2][0], R.c[0][1], R.c[1][1], R.c[2][1]};
+    const TauPowers P = tau_powers(R.tau);
+    const EdgePoly E = edge_poly(x0, R.x1, P);
+    float ve[kDim], we[kDim], bmin[3], bmax[3];
+    waypoint_dt(k, Wq, P, E, R.dt, x0, R.x1, ve);
+    waypoint_dt(k + 1, Wq, P, E, R.dt, x0, R.x1, we);
+    seg_bbox(ve, we, bmin, bmax);
+    bool hit = false;
+    for (int wd = 0; wd < n_words && !hit; ++wd) {
+        unsigned long long m = emask_row[wd];
+        while (m && !hit) {
+            const BoxRec B = boxes[wd * 64 + __builtin_ctzll(m)];
+            m &= m - 1;
+            hit = segment_hits_box(ve, we, bmin, bmax, B.lo, B.hi);
+        }
+    }
+    for (int b = n_masked; b < n_obs && !hit; ++b) {
+        const float *ob = obstacles + (int64_t)b * 12;
+        const float lo[3] = {ob[0], ob[1], ob[2]};
+        const float hi[3] = {ob[6], ob[7], ob[8]};
+        hit = segment_hits_box(ve, we, bmin, bmax, lo, hi);
+    }
+    if (hit)
+        ok[le] = 0;
+}
+
+// A listed segment, decided after the tiles: from its FP32 waypoints by
+// seg_state32 (a sure hit clears the edge's flag), by exact32 when
+// undecided.  Returns 1 when the exact re-test ran.
+__device__ __forceinline__ int decide32(const EdgeRec32 *recs, const int *loff,
+                                        const unsigned long long *emask, int n_words,
+                                        const BoxRec *boxes, const float *__restrict__ obstacles,
+                                        int n_masked, int n_obs, uint8_t *ok, int le, int k)
+{
+    const EdgeRec32 &R = recs[le];
+    const int Wq = loff[le + 1] - loff[le];
+    const unsigned long long *emask_row = emask + le * n_words;
+    float v[kDim], w[kDim];
+    waypoint32(R, k, Wq, v);
+    waypoint32(R, k + 1, Wq, w);
+    const int st = seg_state32(v, w, R, emask_row, n_words, boxes, obstacles, n_masked, n_obs);
+    if (st == kHit)
+        ok[le] = 0;
+    if (st != kUndecided)
+        return 0;
+    exact32(recs, loff, emask_row, n_words, boxes, obstacles, n_masked, n_obs, ok, le, k);
+    return 1;
+}
+
 // A workgroup owns a run of EPB consecutive edges.  Set-up, per edge: the
 // pathPoint constants, and a candidate mask over the boxes -- a box
 // separated from the edge's waypoint bounding box (with the reference's <=
@@ -578,63 +640,18 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
         }
     }
     if (kF32) {
-        // the exact re-test: the reference's FP64 waypoints k and k+1 and its
-        // segment/box test against the edge's candidate boxes
+        // the listed segments, decided by the whole workgroup (decide32)
         __syncthreads();
         const int n_set = *rcnt;
-        auto exact = [&](int le, int k) {
-            const EdgeRec32 &R = reinterpret_cast<const EdgeRec32 *>(recs)[le];
-            const int Wq = loff[le + 1] - loff[le];
-            const float x0[kDim] = {R.c[0][0], R.c[1][0], R.c[2][0],
-                                    R.c[0][1], R.c[1][1], R.c[2][1]};
-            const TauPowers P = tau_powers(R.tau);
-            const EdgePoly E = edge_poly(x0, R.x1, P);
-            float ve[kDim], we[kDim], bmin[3], bmax[3];
-            waypoint_dt(k, Wq, P, E, R.dt, x0, R.x1, ve);
-            waypoint_dt(k + 1, Wq, P, E, R.dt, x0, R.x1, we);
-            seg_bbox(ve, we, bmin, bmax);
-            bool hit = false;
-            for (int wd = 0; wd < n_words && !hit; ++wd) {
-                unsigned long long m = emask[le * n_words + wd];
-                while (m && !hit) {
-                    const int b = wd * 64 + __builtin_ctzll(m);
-                    m &= m - 1;
-                    const BoxRec B = boxes[b];
-                    hit = segment_hits_box(ve, we, bmin, bmax, B.lo, B.hi);
-                }
-            }
-            for (int b = n_masked; b < n_obs && !hit; ++b) {
-                const float *ob = obstacles + (int64_t)b * 12;
-                const float lo[3] = {ob[0], ob[1], ob[2]};
-                const float hi[3] = {ob[6], ob[7], ob[8]};
-                hit = segment_hits_box(ve, we, bmin, bmax, lo, hi);
-            }
-            if (hit)
-                ok[le] = 0; // only zeros are written: concurrent writes agree
-        };
-        // a listed segment: decided from its FP32 waypoints, re-tested
-        // exactly when undecided
-        auto decide = [&](int le, int k) {
-            const EdgeRec32 &R = reinterpret_cast<const EdgeRec32 *>(recs)[le];
-            const int Wq = loff[le + 1] - loff[le];
-            float v[kDim], w[kDim];
-            waypoint32(R, k, Wq, v);
-            waypoint32(R, k + 1, Wq, w);
-            const int st = seg_state32(v, w, R, emask + le * n_words, n_words, boxes, obstacles,
-                                       n_masked, n_obs);
-            if (st == kHit)
-                ok[le] = 0;
-            else if (st == kUndecided)
-                exact(le, k);
-            return st == kUndecided;
-        };
         int n_exact = 0;
         if (n_set <= kRetestCap) {
             for (int i = t; i < n_set; i += kTile) {
                 const uint32_t r = rlist[i];
                 const int le = (int)(r >> 24), k = (int)(r & 0xffffffu);
                 if (ok[le])
-                    n_exact += decide(le, k);
+                    n_exact += decide32(reinterpret_cast<const EdgeRec32 *>(recs), loff, emask,
+                                        n_words, boxes, obstacles, n_masked, n_obs, ok, le,
+                                        k);
             }
         } else {
             // more than the list holds: every segment of the block's edges
@@ -651,7 +668,8 @@ __device__ __forceinline__ void edge_block(const EdgeJob &J, int64_t e0, char *s
                 waypoint32(R, k + 1, Wq, w);
                 if (seg_near32(v, w, R, emask + le * n_words, n_words, boxes, obstacles, n_masked,
                                n_obs))
-                    n_exact += decide(le, k);
+                    n_exact += decide32(reinterpret_cast<const EdgeRec32 *>(recs), loff, emask,
+                                        n_words, boxes, obstacles, n_masked, n_obs, ok, le, k);
             }
         }
         // exact re-tests (rare: a few per pass)
