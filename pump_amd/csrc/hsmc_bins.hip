@@ -16,6 +16,8 @@
 // reference's float order, so rows and CPs are bit-identical to the
 // per-path forms.  When the pairs overflow the workspace, the same launch
 // runs the group form instead (one wave per listed path and 512 particles).
+#include <algorithm>
+
 #include "hsmc_mfma.hpp"
 #include "hsmc_wave.hpp"
 #include "pump_internal.hpp"
@@ -95,19 +97,26 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan(int T, const int *__res
                                                          int64_t chunk_cap)
 {
     __shared__ int tot_e, tot_c;
+    constexpr int kKeep = 4; // time indices per thread whose totals stay in registers
     const int per = (T + kScanBlock - 1) / kScanBlock;
     const int t0 = threadIdx.x * per;
     const int t1 = (t0 + per) < T ? (t0 + per) : T;
     int se = 0, sc = 0, unused;
+    int keep[kKeep];
+#pragma unroll
+    for (int j = 0; j < kKeep; ++j)
+        keep[j] = (t0 + j < t1) ? time_total(hist, t0 + j, 0, &unused) : 0;
     for (int t = t0; t < t1; ++t) {
-        const int h = time_total(hist, t, 0, &unused);
+        const int j = t - t0;
+        const int h = j < kKeep ? keep[j] : time_total(hist, t, 0, &unused);
         se += h;
         sc += (h + kBinChunk - 1) / kBinChunk;
     }
     block_scan2(se, sc, &tot_e, &tot_c);
     const bool overflow = (int64_t)tot_e > entry_cap || (int64_t)tot_c > chunk_cap;
     for (int t = t0; t < t1; ++t) {
-        const int h = time_total(hist, t, 0, &unused);
+        const int j = t - t0;
+        const int h = j < kKeep ? keep[j] : time_total(hist, t, 0, &unused);
         tstart[t] = se;
         if (!overflow)
             for (int j = 0; j * kBinChunk < h; ++j) {
@@ -686,7 +695,10 @@ int bins_finish(const BinArgs &a, bool bits, hipStream_t s)
     const int64_t groups = (a.n_p + 63) / 64;
     const int nq = (a.nq == 1 || a.nq == 2 || a.nq == 8) ? a.nq : kTestGroups;
     const unsigned g_hs = wave_blocks(a.chunk_cap * ((groups + nq - 1) / nq));
-    const unsigned g_gr = wave_blocks((int64_t)a.n_new * ((groups + kHsGroups - 1) / kHsGroups));
+    // the group form runs only after an overflow (its waves stride over the
+    // work): a small grid keeps the usual early exit cheap
+    const unsigned g_gr = std::min(
+        wave_blocks((int64_t)a.n_new * ((groups + kHsGroups - 1) / kHsGroups)), 1024u);
     const unsigned g_cp = wave_blocks(a.n_new);
     const bool mfma = a.split && a.max_hs <= kSlotTile;
     if (bits) {
