@@ -292,7 +292,14 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
     g_cp = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
     ws_bytes = int(P.lib.pump_propagate_workspace(n_new, T))
     g_ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
+    # H_w histogram: expanded waypoints by their number of stored
+    # half-spaces (waypoint k of edge e counts once per child expanding e)
+    edge_of_wp = np.repeat(np.arange(len(wo) - 1), np.diff(wo))
+    mult = np.bincount(free, minlength=len(wo) - 1)
+    hw_hist = np.bincount(np.minimum(counts, info.max_hs).astype(np.int64),
+                          weights=mult[edge_of_wp], minlength=int(info.max_hs) + 1).astype(np.int64)
     out = {"n_new_paths": n_new, "particle_steps": steps, "stored_halfspace_tests": n_p * hs_total,
+           "hw_histogram": [int(x) for x in hw_hist],
            "paths_with_halfspaces": int((marked_per_path > 0).sum()),
            "marked_waypoints": n_marked,
            # offsets read by the half-space tests: 12 B per (particle, marked waypoint)

@@ -267,8 +267,10 @@ int pump_batch_replay(pump_batch *b, int stage, int reps, float *ms);
  * The steps are independent: they fork from the batch stream onto eight
  * side streams of default priority (instance k on stream k mod 8) and join
  * back, so the graph has one branch per side stream and kernels of
- * different instances overlap; ms receives its first launch.  Needs the
- * batch's noise tables. */
+ * different instances overlap; ms receives its first launch.  Before the
+ * capture, the particles per wave of each MC configuration (particle count,
+ * half-space slots) are tuned once (pump_hsmc_autotune_cow) on its first
+ * instance.  Needs the batch's noise tables. */
 int pump_batch_mc_build(pump_batch *b, int n_parents, int max_paths, uint64_t seed, float *ms);
 /* the same steps launched eagerly, reps times; ms = mean per pass */
 int pump_batch_mc_eager(pump_batch *b, int reps, float *ms);

@@ -557,6 +557,23 @@ extern "C" int pump_batch_mc_build(pump_batch *b, int n_parents, int max_paths, 
         TRYH(hipStreamSynchronize(s));
     }
     TRYH(hipStreamSynchronize(s));
+    // particles per wave, tuned once per MC configuration (particle count,
+    // half-space slots) on the first instance that has it
+    std::vector<std::array<int, 2>> tuned;
+    for (size_t k = 0; k < b->rms.size(); ++k) {
+        const BatchMc &m = b->mc[k];
+        const pump_roadmap *rm = b->rms[k];
+        const std::array<int, 2> key{rm->prm.n_p, rm->max_hs};
+        if (m.n_new < 4096 || std::find(tuned.begin(), tuned.end(), key) != tuned.end())
+            continue;
+        tuned.push_back(key);
+        int ppw = 0;
+        TRY(pump_hsmc_autotune_cow(
+            1, m.n_new, 0, rm->wp_offsets, m.edge, m.prev, rm->as, rm->bs, rm->edge_hs_mask,
+            m.times, rm->topts_edge, rm->prm.dt, rm->max_hs, b->offsets_time, rm->tmax_steps,
+            rm->prm.offset_mult, rm->n_mc, m.src, m.parent_cp, m.dst, 0, m.row_src, rm->prm.n_p,
+            m.cp, 0, 0, nullptr, nullptr, nullptr, m.ws, m.ws_bytes, (pump_stream_t)s, &ppw));
+    }
     TRY(capture(b, 2, [&] { return mc_launch(b); }));
     float t = 0;
     TRY(launch_timed(b, 2, &t));
