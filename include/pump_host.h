@@ -147,6 +147,9 @@ typedef struct pump_roadmap_info {
     /* stage times (ms, hipEvent) of the last build */
     float ms_pairs, ms_select, ms_edges_build, ms_nn, ms_sample_free, ms_edge_pass, ms_hs,
         ms_total;
+    /* bisection steps over all pairs of the last build (each step is two
+     * cost evaluations); 0 below 65,536 pairs, where they are not counted */
+    int64_t pair_steps;
 } pump_roadmap_info;
 
 int pump_roadmap_build(const float *samples_host, int num, const float *obstacles_host, int n_obs,
@@ -346,6 +349,9 @@ void pump_host_path_point(float t, float tau, const float *x0, const float *x1, 
 /* the kernels' FP64 division a/b (reciprocal + residual steps), n pairs */
 void pump_host_div_rn(const double *a, const double *b, double *q, int64_t n);
 void pump_host_div_sq(const double *a, const float *tau, double *q, int64_t n);
+/* the pair pass's row of a pair index: n[i] / d by multiply-high with d's
+ * magic number (d >= 1), count dividends */
+void pump_host_div_magic(const uint32_t *n, int64_t count, uint32_t d, uint32_t *q);
 /* the edge pass's bounding box of an edge's waypoint positions (lo[3], hi[3]) */
 void pump_host_edge_bbox(const float *x0, const float *x1, float tau, float *lo, float *hi);
 

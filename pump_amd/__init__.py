@@ -83,7 +83,7 @@ class RoadmapInfo(ctypes.Structure):
         ("n_waypoints", c_int64), ("n_free_edges", c_int64), ("rn", c_float), ("maxd2", c_float),
         ("ms_pairs", c_float), ("ms_select", c_float), ("ms_edges_build", c_float),
         ("ms_nn", c_float), ("ms_sample_free", c_float), ("ms_edge_pass", c_float),
-        ("ms_hs", c_float), ("ms_total", c_float),
+        ("ms_hs", c_float), ("ms_total", c_float), ("pair_steps", c_int64),
     ]
 
 
@@ -262,6 +262,7 @@ _SIGS = {
     "pump_host_path_point": (None, [c_float, c_float, f32p, f32p, f32p]),
     "pump_host_div_rn": (None, [S, S, S, c_int64]),
     "pump_host_div_sq": (None, [S, S, S, c_int64]),
+    "pump_host_div_magic": (None, [S, c_int64, ctypes.c_uint32, S]),
     "pump_host_edge_bbox": (None, [f32p, f32p, c_float, f32p, f32p]),
 }
 

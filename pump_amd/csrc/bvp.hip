@@ -6,8 +6,9 @@
 // One lane per pair.  Pairs are enumerated directly in the output order
 // idx = i*(num-1) + k (k skips the diagonal), so stores of copts/topts are
 // fully coalesced and the source state x_i is uniform across most of a wave.
-// The work per pair is ~40 dcost evaluations, each two FP64 cost
-// evaluations: the kernel is FP64-VALU bound.  Lanes finish their bisection
+// The work per pair is a few dozen FP64 cost evaluations (12.3 bisection
+// steps of two evaluations per pair on average for 16000 uniform samples):
+// the kernel is bound by VALU issue.  Lanes finish their bisection
 // after different iteration counts; to keep them busy each wave pulls pairs
 // from a shared counter when its lanes run dry (persistent waves).
 #include "pump_device.hpp"
@@ -68,16 +69,27 @@ constexpr int kChunk = 2048; // pairs per claim of the work counter
 
 // Pair sources: map a global pair index to the pair's cost terms, tmax and
 // output slots.  One instance (idx in the reference's output order) ...
+// With fewer than 2^32 pairs the row i = g / (num - 1) is found by a
+// multiply-high with the divisor's magic number (DivMagic).
 struct OnePairSource {
     const float *samples;
     int num;
     float tmax;
     float *copts, *topts;
+    DivMagic dm;
     __device__ void take(int64_t g, CostTerms &c, float &tm, float *&oc, float *&ot,
                          int64_t &oi) const
     {
         int i, j;
-        pair_of(g, num, i, j);
+        if (dm.m) {
+            const uint32_t n = (uint32_t)g;
+            const uint32_t q = div_magic_apply(n, dm);
+            const int k = (int)(n - q * (uint32_t)(num - 1));
+            i = (int)q;
+            j = k + (k >= i ? 1 : 0);
+        } else {
+            pair_of(g, num, i, j);
+        }
         float a[kDim], b[kDim];
         load_state(samples, i, a);
         load_state(samples, j, b);
@@ -120,9 +132,13 @@ struct ManyPairSource {
     }
 };
 
+// steps_out (may be null) receives the number of (lane, step) pairs
+// executed, i.e. the bisection steps of all pairs: each step is two cost
+// evaluations.
 template <class Src>
 __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs,
-                                                 unsigned long long *__restrict__ next_chunk)
+                                                 unsigned long long *__restrict__ next_chunk,
+                                                 unsigned long long *__restrict__ steps_out)
 {
     const int lane = threadIdx.x & 63;
     // each wave claims kChunk pairs at a time from the global counter (one
@@ -137,6 +153,7 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
     int64_t oi = 0;
     int state = ST_TMAX, itr = 0, halvings = 0;
     bool done_all = false;
+    unsigned long long steps = 0; // wave-uniform
 
     while (true) {
         // refill idle lanes from the wave's chunk, claiming a new chunk as needed
@@ -178,11 +195,13 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
             pool += took;
             idle = __ballot(my < 0);
         }
-        if (__ballot(my >= 0) == 0) {
+        const unsigned long long busy = __ballot(my >= 0);
+        if (busy == 0) {
             if (done_all)
                 break;
             continue;
         }
+        steps += (unsigned long long)__popcll(busy);
         if (my >= 0) {
             // the step's two cost evaluations
             const float x = (state == ST_FIN)    ? topt
@@ -230,14 +249,17 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
             }
         }
     }
+    if (steps_out && lane == 0)
+        atomicAdd(steps_out, steps);
 }
 
 __global__ __launch_bounds__(256) void k_fill_pairs_persistent(
     const float *__restrict__ samples, int num, float tmax, float *__restrict__ copts,
-    float *__restrict__ topts, int64_t n_pairs, unsigned long long *__restrict__ next_chunk)
+    float *__restrict__ topts, int64_t n_pairs, DivMagic dm,
+    unsigned long long *__restrict__ next_chunk, unsigned long long *__restrict__ steps_out)
 {
-    const OnePairSource src{samples, num, tmax, copts, topts};
-    pairs_persistent(src, n_pairs, next_chunk);
+    const OnePairSource src{samples, num, tmax, copts, topts, dm};
+    pairs_persistent(src, n_pairs, next_chunk, steps_out);
 }
 
 // all ordered pairs of many instances in one persistent launch
@@ -247,7 +269,7 @@ __global__ __launch_bounds__(256) void k_fill_pairs_many(const PairJob *__restri
                                                          unsigned long long *__restrict__ next_chunk)
 {
     const ManyPairSource src{jobs, prefix, n_inst};
-    pairs_persistent(src, n_pairs, next_chunk);
+    pairs_persistent(src, n_pairs, next_chunk, nullptr);
 }
 
 __global__ void k_zero_counter(unsigned long long *c)
@@ -358,21 +380,27 @@ __global__ __launch_bounds__(256) void k_polish(const float *__restrict__ sample
 
 using namespace pump;
 
-extern "C" int pump_fill_copts_topts(const float *d_samples, int num, float tmax, float *d_copts,
-                                     float *d_topts, pump_stream_t stream)
+namespace pump {
+// All ordered pairs of one instance; *steps (may be null) receives the
+// bisection steps of all pairs, 0 for a launch below 65,536 pairs (the
+// plain one-lane-per-pair kernel, which does not count them).
+int fill_copts_topts(const float *d_samples, int num, float tmax, float *d_copts, float *d_topts,
+                     hipStream_t s, int64_t *steps)
 {
     PUMP_REQUIRE(d_samples && d_copts && d_topts && num >= 2);
     const int64_t n_pairs = (int64_t)num * (num - 1);
-    hipStream_t s = as_stream(stream);
+    if (steps)
+        *steps = 0;
     if (n_pairs < (1 << 16)) {
         hipLaunchKernelGGL(k_fill_pairs, dim3(grid_for(n_pairs, 256)), dim3(256), 0, s, d_samples,
                            num, tmax, d_copts, d_topts, n_pairs);
         PUMP_LAUNCHED();
         return PUMP_OK;
     }
+    // [0]: work counter, [1]: step count
     unsigned long long *counter = nullptr;
-    PUMP_HIP(hipMalloc((void **)&counter, sizeof(unsigned long long)));
-    hipError_t me = hipMemsetAsync(counter, 0, sizeof(unsigned long long), s);
+    PUMP_HIP(hipMalloc((void **)&counter, 2 * sizeof(unsigned long long)));
+    hipError_t me = hipMemsetAsync(counter, 0, 2 * sizeof(unsigned long long), s);
     if (me != hipSuccess) {
         hipFree(counter);
         PUMP_HIP(me);
@@ -385,14 +413,29 @@ extern "C" int pump_fill_copts_topts(const float *d_samples, int num, float tmax
     int64_t need = (n_pairs + 255) / 256;
     if (blocks > need)
         blocks = need;
+    const DivMagic dm = n_pairs < (1ll << 32) ? div_magic((uint32_t)(num - 1)) : DivMagic{0, 0, 0};
     hipLaunchKernelGGL(k_fill_pairs_persistent, dim3((unsigned)blocks), dim3(256), 0, s,
-                       d_samples, num, tmax, d_copts, d_topts, n_pairs, counter);
+                       d_samples, num, tmax, d_copts, d_topts, n_pairs, dm, counter,
+                       steps ? counter + 1 : nullptr);
     hipError_t le = hipGetLastError();
     hipError_t se = hipStreamSynchronize(s);
+    unsigned long long n_steps = 0;
+    if (le == hipSuccess && se == hipSuccess && steps)
+        se = hipMemcpy(&n_steps, counter + 1, sizeof(n_steps), hipMemcpyDeviceToHost);
     hipFree(counter);
     PUMP_HIP(le);
     PUMP_HIP(se);
+    if (steps)
+        *steps = (int64_t)n_steps;
     return PUMP_OK;
+}
+} // namespace pump
+
+extern "C" int pump_fill_copts_topts(const float *d_samples, int num, float tmax, float *d_copts,
+                                     float *d_topts, pump_stream_t stream)
+{
+    return pump::fill_copts_topts(d_samples, num, tmax, d_copts, d_topts, as_stream(stream),
+                                  nullptr);
 }
 
 extern "C" int pump_polish_topts(const float *d_samples, int num, float tmax, const float *d_topts,

@@ -65,7 +65,8 @@ int rm_pairs(pump_roadmap *rm)
     const pump_params &P = rm->prm;
     EventTimer tm(rm->stream);
     tm.start();
-    TRY(pump_fill_copts_topts(rm->samples, rm->num, P.tmax, rm->copts, rm->topts, rm->stream));
+    TRY(fill_copts_topts(rm->samples, rm->num, P.tmax, rm->copts, rm->topts, rm->stream,
+                         &rm->info.pair_steps));
     rm->info.ms_pairs = tm.stop();
     return PUMP_OK;
 }
@@ -670,6 +671,13 @@ extern "C" void pump_host_edge_bbox(const float *x0, const float *x1, float tau,
                                     float *hi)
 {
     edge_bbox(x0, x1, tau, lo, hi);
+}
+
+extern "C" void pump_host_div_magic(const uint32_t *n, int64_t count, uint32_t d, uint32_t *q)
+{
+    const DivMagic dm = div_magic(d);
+    for (int64_t i = 0; i < count; ++i)
+        q[i] = div_magic_apply(n[i], dm);
 }
 
 extern "C" void pump_host_div_sq(const double *a, const float *tau, double *q, int64_t n)

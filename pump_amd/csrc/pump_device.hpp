@@ -150,6 +150,31 @@ __host__ __device__ __forceinline__ float topt_bisection(const CostTerms &c, flo
 // ---------------------------------------------------------------------------
 // pathPoint(t, tau, x0, x1), 2pBVP.cu:117-131
 // ---------------------------------------------------------------------------
+// Unsigned 32-bit division by a fixed d >= 1 (Granlund and Montgomery,
+// PLDI 1994, the round-up form with a 33-bit multiplier 2^32 + m):
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, and for every
+// n < 2^32, n / d = (t + ((n - t) >> s1)) >> s2 with t = mulhi(m, n),
+// s1 = min(l, 1), s2 = max(l - 1, 0).  m >= 1, so m = 0 can mark "not in use".
+struct DivMagic {
+    uint32_t m;
+    uint32_t s1, s2;
+};
+
+inline DivMagic div_magic(uint32_t d)
+{
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < d)
+        ++l;
+    const uint64_t m = ((1ull << 32) * ((1ull << l) - d)) / d + 1;
+    return DivMagic{(uint32_t)m, l ? 1u : 0u, l ? l - 1 : 0u};
+}
+
+__host__ __device__ __forceinline__ uint32_t div_magic_apply(uint32_t n, const DivMagic &dm)
+{
+    const uint32_t t = (uint32_t)(((uint64_t)dm.m * n) >> 32);
+    return (t + ((n - t) >> dm.s1)) >> dm.s2;
+}
+
 struct TauPowers {
     float tau;
     double t3;     // tau^3 (tau^2 is exact and formed where it is used)

@@ -80,6 +80,10 @@ struct HsJob {
     uint64_t *edge_mask;
 };
 
+// all ordered pairs of one instance (pump_fill_copts_topts), with the
+// bisection step count of the persistent launch in *steps (may be null)
+int fill_copts_topts(const float *d_samples, int num, float tmax, float *d_copts, float *d_topts,
+                     hipStream_t s, int64_t *steps);
 int fill_pairs_many(const PairJob *d_jobs, const int64_t *d_prefix, int n_inst, int64_t n_pairs,
                     int blocks_per_cu, unsigned long long *d_counter, hipStream_t s);
 // edges per workgroup of the batched edge pass: 64, 128 or 256

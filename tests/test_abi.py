@@ -312,3 +312,23 @@ def test_fp64_division_by_a_float_square_matches_ieee():
     ref = a / b
     assert np.array_equal(out.view(np.uint64), ref.view(np.uint64))
     assert not np.array_equal(a * (1.0 / b), ref)
+
+
+def test_pair_row_division_by_magic_number():
+    """The pair pass finds a pair's row i = idx / (num - 1) by a multiply-high
+    with the divisor's magic number; it must equal integer division for
+    every divisor of the workloads (1 .. 65535: num up to 65,536 keeps the
+    pair count below 2^32) over random and boundary dividends: 0, 2^32 - 1,
+    and the multiples of d and the values just below them."""
+    rng = np.random.default_rng(5)
+    ds = np.concatenate([np.arange(1, 1025), rng.integers(1025, 65536, 600),
+                         [3999, 15999, 31999, 63999, 65535, 32768, 65534]]).astype(np.uint32)
+    for d in ds.tolist():
+        n = np.concatenate([rng.integers(0, 1 << 32, 4000, dtype=np.uint64),
+                            [0, 1, d - 1, d, d + 1, (1 << 32) - 1],
+                            (rng.integers(1, (1 << 32) // d, 200, dtype=np.uint64) * d),
+                            (rng.integers(1, (1 << 32) // d, 200, dtype=np.uint64) * d - 1)])
+        n = np.ascontiguousarray(n.astype(np.uint32))
+        q = np.zeros_like(n)
+        P.lib.pump_host_div_magic(n.ctypes.data, n.size, d, q.ctypes.data)
+        assert np.array_equal(q, n // np.uint32(d)), d
