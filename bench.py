@@ -537,8 +537,10 @@ def hs_sweep(P, configs, reps):
     waypoints, each expanded once from one of 4096 all-valid parents, 2000
     particles, the noise table of the NUM=16000 workload; a fraction f of
     the waypoints holds k live half-spaces (a ~ N(0, 0.05^2) per component,
-    b = a.a / 2 rescaled as the cache stores it).  Both forms must give the
-    same CPs."""
+    b = a.a / 2 rescaled as the cache stores it).  The workspace holds every
+    (path, waypoint) pair, so the binned form runs at every density; each
+    form is timed at each particles-per-wave choice and reported at its
+    best.  Both forms must give the same CPs."""
     cfg = configs.config(1)
     prm = cfg["params"]
     ot, _ = P.make_noise(cfg["noise"])
@@ -558,7 +560,8 @@ def hs_sweep(P, configs, reps):
     nw = (n_p + 63) // 64
     src = P.DeviceArray(shape=(n_par, nw), dtype=np.uint64, fill=0)
     P.check("pump_setup_valid_bits", P.lib.pump_setup_valid_bits(src.p, n_p, n_par, None))
-    ws_bytes = int(P.lib.pump_propagate_workspace(n_edges, T))
+    # room for every (path, waypoint) pair in the binned form
+    ws_bytes = int(P.lib.pump_propagate_workspace_pairs(n_edges, T, n_edges * W))
     ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
     stream = P.Stream()
     timer = P.Timer(stream)
@@ -590,18 +593,34 @@ def hs_sweep(P, configs, reps):
                     n_mc, src.p, dst.p, 0, n_p, cp.p, 0, prec, split.p if prec else None,
                     band.p if prec else None, None, ws.p, ws_bytes, stream.h))
 
-            # the first timed configuration also warms the clocks up
-            for _ in range(1 if cps or f != 0.01 else 20):
-                run()
-            stream.sync()
-            cps[name] = cp.numpy()
+            # each form at its best particles per wave
+            times_ms = {}
+            for ppw in (64, 128, 256, 512):
+                P.check("pump_hsmc_set_particles_per_wave",
+                        P.lib.pump_hsmc_set_particles_per_wave(1, prec, n_p, max_hs, ppw))
+                # the first timed configuration also warms the clocks up
+                for _ in range(1 if cps or times_ms or f != 0.01 else 20):
+                    run()
+                stream.sync()
+                c_now = cp.numpy()
+                if name in cps:
+                    assert np.array_equal(c_now, cps[name]), (name, ppw)
+                cps[name] = c_now
+                timer.start()
+                for _ in range(reps):
+                    run()
+                times_ms[ppw] = timer.stop() / reps
+            P.check("pump_hsmc_set_particles_per_wave",
+                    P.lib.pump_hsmc_set_particles_per_wave(1, prec, n_p, max_hs, 0))
+            best = min(times_ms, key=times_ms.get)
+            ms = times_ms[best]
             if prec:
-                row["band_retests"] = int(band.numpy()[0])
-            timer.start()
-            for _ in range(reps):
+                band.upload(np.zeros(1, np.uint64))
                 run()
-            ms = timer.stop() / reps
+                stream.sync()
+                row["band_retests"] = int(band.numpy()[0])
             row[name + "_ms"] = ms
+            row[name + "_particles_per_wave"] = best
             row[name + "_tests_per_s"] = tests / (ms / 1e3)
             dst.free()
             cp.free()

@@ -239,6 +239,13 @@ int pump_calculate_cp_hsmc(int num_new_paths, int64_t path_base, float *d_path_c
  * summary written by pump_cache_halfspaces for these d_bs, or NULL (then
  * every waypoint's slots are scanned, as the reference does). */
 size_t pump_propagate_workspace(int num_new_paths, int tmax_steps);
+/* A workspace whose binned form holds marked_pairs (path, waypoint) pairs
+ * with a stored half-space (at least the default 4 per path + 65536): a
+ * wavefront with more such pairs than its workspace holds runs the slower
+ * group form (same results).  Any larger workspace passed to the calls
+ * below is used the same way.  The capacity stops at 2^30 pairs.  No
+ * counterpart in the reference. */
+size_t pump_propagate_workspace_pairs(int num_new_paths, int tmax_steps, int64_t marked_pairs);
 int pump_propagate_cp(int num_new_paths, int waypoints_count, const int64_t *d_wp_offsets,
                       const int *d_wf_edge, const int *d_wf_prev, const float *d_as,
                       const float *d_bs, const uint64_t *d_edge_hs_mask, const float *d_times,

@@ -172,6 +172,7 @@ _SIGS = {
                                                S, S]),
     "pump_calculate_cp_hsmc": (c_int, [c_int, c_int64, S, S, c_int, S]),
     "pump_propagate_workspace": (c_size_t, [c_int, c_int]),
+    "pump_propagate_workspace_pairs": (c_size_t, [c_int, c_int, c_int64]),
     "pump_propagate_cp": (c_int, [c_int, c_int, S, S, S, S, S, S, S, S, c_float, c_int, S, c_int,
                                   c_float, c_int, S, S, c_int64, c_int, S, c_int64, S, S, c_size_t,
                                   S]),

@@ -935,13 +935,16 @@ int search(pump_roadmap *rm, const pump_params &P, pump_search_result *res)
         }
         sizeG[gidx] = 0;
         res->num_considered += n_new;
+        int64_t wf_waypoints = 0;
         for (int w = 0; w < n_new; ++w)
-            res->particle_steps +=
-                (int64_t)n_p * (S.wp_offsets[wf_edge[w] + 1] - S.wp_offsets[wf_edge[w]]);
+            wf_waypoints += S.wp_offsets[wf_edge[w] + 1] - S.wp_offsets[wf_edge[w]];
+        res->particle_steps += (int64_t)n_p * wf_waypoints;
 
-        // HSMC step on the device (PUMP.cu:80-160)
+        // HSMC step on the device (PUMP.cu:80-160); the workspace holds every
+        // (path, waypoint) pair, so the binned form runs however dense the
+        // stored half-spaces are
         TRY(stage.ensure((size_t)n_new * row_bytes));
-        TRY(d_ws.ensure(pump_propagate_workspace(n_new, rm->tmax_steps)));
+        TRY(d_ws.ensure(pump_propagate_workspace_pairs(n_new, rm->tmax_steps, wf_waypoints)));
         TRY(d_cp.ensure(sizeof(float) * n_new));
         TRY(d_prev.ensure(sizeof(int) * n_new));
         TRY(d_edge.ensure(sizeof(int) * n_new));

@@ -1376,17 +1376,43 @@ static size_t round256(size_t x)
 // Workspace of the batched form, in order: control words, the sub-binned
 // per-time histogram and the sub-bin cursors (cleared together), time-index
 // starts, per-path flags, listed-path records, CP accumulators of the group
-// form, chunks and binned entries.  The entry capacity, 4 per path + 65536,
-// covers typical wavefronts; beyond it the group form runs instead of the
-// binned one.
+// form, chunks and binned entries.  The entry capacity (one entry per
+// (path, waypoint) pair that holds a half-space) is 4 per path + 65536 in
+// the workspace of pump_propagate_workspace, which covers typical
+// wavefronts; a larger workspace holds more (the entries fill it), and
+// beyond the capacity the group form runs instead of the binned one.
+constexpr int64_t kMaxEntries = int64_t(1) << 30; // entry positions are ints
 struct BatchLayout {
     size_t hist, cursor, tstart, has, recs, acc, chunks, entries, total;
     int64_t entry_cap, chunk_cap;
-    BatchLayout(int64_t n, int64_t T)
+    BatchLayout(int64_t n, int64_t T, int64_t cap = 0)
     {
         n = n > 0 ? n : 1;
         T = T > 0 ? T : 1;
-        entry_cap = 4 * n + 65536;
+        set(n, T, cap > 4 * n + 65536 ? cap : 4 * n + 65536);
+    }
+    // the layout whose entries fill ws_bytes (the default one if smaller)
+    static BatchLayout fill(int64_t n, int64_t T, size_t ws_bytes)
+    {
+        BatchLayout L(n, T);
+        if (ws_bytes <= L.total)
+            return L;
+        // the largest capacity that fits (total grows with it)
+        int64_t lo = L.entry_cap, hi = kMaxEntries;
+        if (BatchLayout(n, T, hi).total <= ws_bytes)
+            return BatchLayout(n, T, hi);
+        while (hi - lo > 1) {
+            const int64_t mid = lo + (hi - lo) / 2;
+            if (BatchLayout(n, T, mid).total <= ws_bytes)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        return BatchLayout(n, T, lo);
+    }
+    void set(int64_t n, int64_t T, int64_t cap)
+    {
+        entry_cap = cap < kMaxEntries ? cap : kMaxEntries;
         chunk_cap = entry_cap / kBinChunk + T + 1;
         const size_t bins = (size_t)T * kSubBins;
         hist = 256;
@@ -1404,6 +1430,14 @@ struct BatchLayout {
 extern "C" size_t pump_propagate_workspace(int num_new_paths, int tmax_steps)
 {
     return BatchLayout(num_new_paths, tmax_steps).total;
+}
+
+extern "C" size_t pump_propagate_workspace_pairs(int num_new_paths, int tmax_steps,
+                                                 int64_t marked_pairs)
+{
+    return BatchLayout(num_new_paths, tmax_steps,
+                       marked_pairs < kMaxEntries ? marked_pairs : kMaxEntries)
+        .total;
 }
 
 __global__ __launch_bounds__(256) void k_zero_words(int *__restrict__ p, int64_t n)
@@ -1446,7 +1480,7 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
                             int *row_src)
 {
     const int64_t row_bytes = kBits ? (int64_t)((n_p + 63) / 64) * 8 : (int64_t)n_p;
-    const BatchLayout L(n_new, tmax_steps);
+    const BatchLayout L = BatchLayout::fill(n_new, tmax_steps, ws ? ws_bytes : 0);
     if (ws && ws_bytes >= L.total && n_new >= 4096 && !use_block_kernel()) {
         uint8_t *base = reinterpret_cast<uint8_t *>(ws);
         int *ctl = reinterpret_cast<int *>(base);

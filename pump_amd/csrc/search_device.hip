@@ -424,7 +424,11 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
 
         // HSMC step (PUMP.cu:80-160)
         TRY(stage.ensure((size_t)n_new * row_bytes));
-        TRY(d_ws.ensure(pump_propagate_workspace(n_new, rm->tmax_steps)));
+        // room for n_new paths of the longest edge's waypoints, so the
+        // binned form runs however dense the stored half-spaces are
+        const int64_t w_bound = rm->prm.w_fixed > 0 ? rm->prm.w_fixed : rm->prm.w_cap;
+        TRY(d_ws.ensure(pump_propagate_workspace_pairs(n_new, rm->tmax_steps,
+                                                       (int64_t)n_new * w_bound)));
         TRY(d_cp.ensure(sizeof(float) * n_new));
         TRY(row_src.ensure(sizeof(int) * n_new));
         unsigned long long *cl = as<unsigned long long>(clamp);

@@ -387,6 +387,27 @@ def test_propagate_batched_form(n_p, w_range, T):
             None, None, 0, None))
         assert np.array_equal(host(dst1), ref_v[npar:])
         assert np.array_equal(host(cp1), ref_cp[npar:])
+    if w_range[1] > 64:
+        # a workspace sized for every (path, waypoint) pair of the wavefront
+        # keeps the binned form where the default one overflowed
+        n_pairs = int((d["off"][d["wf_edge"] + 1] - d["off"][d["wf_edge"]]).sum())
+        big = int(P.lib.pump_propagate_workspace_pairs(nn, d["T"], n_pairs))
+        assert big > ws_bytes
+        wsb = P.DeviceArray(shape=(big,), dtype=np.uint8, fill=0)
+        for precision, split in ((0, None), (1, g_split)):
+            bsrc = dev(pack_bits(d["valid"][:npar], n_p))
+            bdst = dev(np.zeros((nn, (n_p + 63) // 64), np.uint64))
+            cpb = dev(np.zeros(nn, np.float32))
+            P.check("propagate_cp_bits", P.lib.pump_propagate_cp_bits(
+                nn, 0, ptr(g["off"]), ptr(g["wf_edge"]), ptr(g["wf_prev"]), ptr(g["as_"]),
+                ptr(g["bs"]), ptr(g_mask), ptr(g["times"]), ptr(g["topts"]), d["dt"],
+                d["max_hs"], ptr(g["offs"]), d["T"], d["om"], d["n_mc"], ptr(bsrc), ptr(bdst),
+                0, n_p, ptr(cpb), 0, precision, split.p if split else None, None, None, wsb.p,
+                big, None))
+            assert np.array_equal(unpack_bits(host(bdst), n_p), ref_v[npar:]), precision
+            assert np.array_equal(host(cpb), ref_cp[npar:]), precision
+            ctl = wsb.numpy()[:16].view(np.int32)
+            assert ctl[2] == 0 and ctl[1] > 4 * nn + 65536, ctl[:4]
 
 
 @pytest.mark.parametrize("n_new", [6000, 500])
