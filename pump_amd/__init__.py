@@ -122,7 +122,8 @@ class BatchInfo(ctypes.Structure):
     _fields_ = [("n_inst", c_int), ("pairs_blocks_per_cu", c_int), ("edges_per_block", c_int),
                 ("n_pairs", c_int64), ("n_edges", c_int64), ("n_waypoints", c_int64),
                 ("ms_pairs", c_float), ("ms_edges", c_float), ("ms_autotune", c_float),
-                ("ms_total", c_float), ("edge_obs_tile", c_int), ("autotune_cached", c_int)]
+                ("ms_total", c_float), ("edge_obs_tile", c_int), ("autotune_cached", c_int),
+                ("mc_one_launch", c_int)]
 
 
 class ProblemFile(ctypes.Structure):
@@ -652,6 +653,7 @@ class Batch:
         check("pump_batch_mc_build", lib.pump_batch_mc_build(self._h, n_parents, max_paths, seed,
                                                              ctypes.byref(ms)))
         self._mc_parents = n_parents
+        check("pump_batch_info_get", lib.pump_batch_info_get(self._h, ctypes.byref(self.info)))
         return ms.value
 
     def mc_eager(self, reps=1):

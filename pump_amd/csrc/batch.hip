@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/pump_host.h"
+#include "hsmc_wave.hpp"
 #include "pump_internal.hpp"
 #include "roadmap_internal.hpp"
 
@@ -63,6 +64,12 @@ struct pump_batch {
     // side streams (default priority) and fork/join events of the MC stage
     std::vector<hipStream_t> side;
     std::vector<hipEvent_t> ev;
+    // the MC stage as one launch per stage for all problems (when every
+    // problem's step takes the binned form): jobs, block maps, grids
+    McJob *mc_jobs = nullptr;
+    void *mc_maps = nullptr;
+    McGrids mc_grids{};
+    bool mc_many = false;
 };
 
 namespace {
@@ -362,6 +369,9 @@ extern "C" void pump_batch_free(pump_batch *b)
         hipStreamDestroy(st);
     for (hipEvent_t e : b->ev)
         hipEventDestroy(e);
+    dfree(b->mc_jobs);
+    if (b->mc_maps)
+        hipFree(b->mc_maps);
     for (BatchMc &m : b->mc) {
         dfree(m.edge);
         dfree(m.prev);
@@ -469,6 +479,8 @@ constexpr int kMcStreams = 8; // side streams the MC stage spreads its instances
 // capture the graph gets one branch per side stream.
 int mc_launch(pump_batch *b)
 {
+    if (b->mc_many)
+        return mc_cow_many(b->mc_jobs, b->mc_grids, b->stream);
     const int S = (int)b->side.size();
     TRYH(hipEventRecord(b->ev[0], b->stream));
     for (int j = 0; j < S; ++j)
@@ -573,6 +585,33 @@ extern "C" int pump_batch_mc_build(pump_batch *b, int n_parents, int max_paths, 
             m.times, rm->topts_edge, rm->prm.dt, rm->max_hs, b->offsets_time, rm->tmax_steps,
             rm->prm.offset_mult, rm->n_mc, m.src, m.parent_cp, m.dst, 0, m.row_src, rm->prm.n_p,
             m.cp, 0, 0, nullptr, nullptr, nullptr, m.ws, m.ws_bytes, (pump_stream_t)s, &ppw));
+    }
+    // every problem's step in one launch per stage, where each takes the
+    // binned form; else one step per problem on the side streams
+    std::vector<McJob> jobs;
+    bool many = true;
+    for (size_t k = 0; k < b->rms.size() && many; ++k) {
+        const BatchMc &m = b->mc[k];
+        if (m.n_new == 0)
+            continue;
+        const pump_roadmap *rm = b->rms[k];
+        McJob J;
+        many = mc_cow_job(J, m.n_new, rm->wp_offsets, m.edge, m.prev, rm->as, rm->bs,
+                          rm->edge_hs_mask, m.times, rm->topts_edge, rm->prm.dt, rm->max_hs,
+                          b->offsets_time, rm->tmax_steps, rm->prm.offset_mult, rm->n_mc, m.src,
+                          m.parent_cp, m.dst, m.row_src, rm->prm.n_p, m.cp, m.ws,
+                          m.ws_bytes) == PUMP_OK;
+        if (many)
+            jobs.push_back(J);
+    }
+    if (many && !jobs.empty()) {
+        TRY(mc_plan(jobs.data(), (int)jobs.size(), b->mc_grids, &b->mc_maps, s));
+        TRY(dalloc(&b->mc_jobs, jobs.size()));
+        TRYH(hipMemcpyAsync(b->mc_jobs, jobs.data(), jobs.size() * sizeof(McJob),
+                            hipMemcpyHostToDevice, s));
+        TRYH(hipStreamSynchronize(s));
+        b->mc_many = true;
+        b->info.mc_one_launch = 1;
     }
     TRY(capture(b, 2, [&] { return mc_launch(b); }));
     float t = 0;

@@ -16,10 +16,12 @@
 //     pairs staged in LDS (selected with PUMP_HSMC_KERNEL=block);
 //   * the batched form for large wavefronts: classify, copy, and the
 //     time-binned half-space pass of hsmc_bins.hip.
+#include <algorithm>
 #include <array>
 #include <cstdlib>
 #include <map>
 #include <mutex>
+#include <vector>
 
 #include "hsmc_wave.hpp"
 #include "pump_device.hpp"
@@ -716,7 +718,7 @@ __global__ __launch_bounds__(256) void k_propagate_wave(
 // one wave per (listed path, 512 particles).
 constexpr int kClassifyBlock = 1024;
 
-__global__ __launch_bounds__(kClassifyBlock) void k_classify(
+__device__ __forceinline__ void classify_body(
     int n_new, const int64_t *__restrict__ wp_offsets, int w_fixed,
     const int *__restrict__ wf_edge, const int *__restrict__ wf_prev,
     const float *__restrict__ bs, const uint64_t *__restrict__ edge_mask, int max_hs,
@@ -724,14 +726,15 @@ __global__ __launch_bounds__(kClassifyBlock) void k_classify(
     uint8_t *__restrict__ has_hs, HsPath *__restrict__ recs, int *__restrict__ ctl,
     int *__restrict__ hist, unsigned long long *__restrict__ acc,
     unsigned long long *__restrict__ clamped, int *__restrict__ row_src,
-    const float *__restrict__ parent_cp, float *__restrict__ path_cp, int64_t cp_base)
+    const float *__restrict__ parent_cp, float *__restrict__ path_cp, int64_t cp_base,
+    const VGrid &G)
 {
     __shared__ int wcount[kClassifyBlock / 64];
     __shared__ int block_base;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int64_t stride = (int64_t)gridDim.x * kClassifyBlock;
+    const int64_t stride = G.nb * kClassifyBlock;
     // workgroup-uniform trip count: every thread reaches the barriers below
-    for (int64_t base = (int64_t)blockIdx.x * kClassifyBlock; base < n_new; base += stride) {
+    for (int64_t base = G.bid * kClassifyBlock; base < n_new; base += stride) {
         const int64_t w = base + threadIdx.x;
         bool has = false;
         int n_clamped = 0;
@@ -796,6 +799,21 @@ __global__ __launch_bounds__(kClassifyBlock) void k_classify(
         }
         __syncthreads(); // wcount and block_base are reused by the next pass
     }
+}
+
+__global__ __launch_bounds__(kClassifyBlock) void k_classify(
+    int n_new, const int64_t *__restrict__ wp_offsets, int w_fixed,
+    const int *__restrict__ wf_edge, const int *__restrict__ wf_prev,
+    const float *__restrict__ bs, const uint64_t *__restrict__ edge_mask, int max_hs,
+    const float *__restrict__ times, const float *__restrict__ topts, float dt, int tmax_steps,
+    uint8_t *__restrict__ has_hs, HsPath *__restrict__ recs, int *__restrict__ ctl,
+    int *__restrict__ hist, unsigned long long *__restrict__ acc,
+    unsigned long long *__restrict__ clamped, int *__restrict__ row_src,
+    const float *__restrict__ parent_cp, float *__restrict__ path_cp, int64_t cp_base)
+{
+    classify_body(n_new, wp_offsets, w_fixed, wf_edge, wf_prev, bs, edge_mask, max_hs, times,
+                  topts, dt, tmax_steps, has_hs, recs, ctl, hist, acc, clamped, row_src,
+                  parent_cp, path_cp, cp_base, hw_grid());
 }
 
 // Copy unit per lane: 16-byte chunks (bit masks or byte rows), 8-byte words
@@ -1058,11 +1076,11 @@ __global__ __launch_bounds__(256) void k_propagate_b64(
 // Copy-on-write form: the listed paths' child rows start as their parents'
 // rows (the half-space pass ANDs into them).  One wave per listed path.
 template <int kUnit>
-__global__ __launch_bounds__(256) void k_copy_listed(const HsPath *__restrict__ recs,
-                                                     const int *__restrict__ ctl,
-                                                     const uint8_t *__restrict__ src,
-                                                     uint8_t *__restrict__ dst, int64_t dst_base,
-                                                     int64_t row_bytes)
+__device__ __forceinline__ void copy_listed_body(const HsPath *__restrict__ recs,
+                                                 const int *__restrict__ ctl,
+                                                 const uint8_t *__restrict__ src,
+                                                 uint8_t *__restrict__ dst, int64_t dst_base,
+                                                 int64_t row_bytes, const VGrid &G)
 {
     using T = typename CopyWord<kUnit>::T;
     if (ctl[kCtlOverflow]) // the group form writes whole rows itself
@@ -1070,14 +1088,68 @@ __global__ __launch_bounds__(256) void k_copy_listed(const HsPath *__restrict__ 
     const int lane = threadIdx.x & 63;
     const int n = ctl[kCtlListed];
     const int units = (int)(row_bytes / kUnit);
-    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t slot = wave_index(); slot < n; slot += n_waves) {
+    const int64_t n_waves = wave_count(G);
+    for (int64_t slot = wave_index(G); slot < n; slot += n_waves) {
         const HsPath r = recs[slot];
         const T *sp = reinterpret_cast<const T *>(src + (int64_t)r.prev * row_bytes);
         T *dp = reinterpret_cast<T *>(dst + (dst_base + r.w) * row_bytes);
         for (int j = lane; j < units; j += 64)
             dp[j] = sp[j];
     }
+}
+
+template <int kUnit>
+__global__ __launch_bounds__(256) void k_copy_listed(const HsPath *__restrict__ recs,
+                                                     const int *__restrict__ ctl,
+                                                     const uint8_t *__restrict__ src,
+                                                     uint8_t *__restrict__ dst, int64_t dst_base,
+                                                     int64_t row_bytes)
+{
+    copy_listed_body<kUnit>(recs, ctl, src, dst, dst_base, row_bytes, hw_grid());
+}
+
+// The classification, row copies and workspace clearing of many problems'
+// copy-on-write steps (see McJob); workgroup b serves problem map[b] >> 16.
+__device__ __forceinline__ const McJob &mc_job(const McJob *__restrict__ jobs,
+                                               const uint32_t *__restrict__ map, int stage,
+                                               VGrid &G)
+{
+    const uint32_t m = map[blockIdx.x];
+    const McJob &J = jobs[m >> 16];
+    G = VGrid{(int64_t)(m & 0xffffu), (int64_t)J.nb[stage]};
+    return J;
+}
+
+__global__ __launch_bounds__(256) void k_zero_many(const McJob *__restrict__ jobs,
+                                                   const uint32_t *__restrict__ map)
+{
+    VGrid G;
+    const McJob &J = mc_job(jobs, map, kMcZero, G);
+    int *p = J.B.ctl;
+    for (int64_t i = G.bid * blockDim.x + threadIdx.x; i < J.n_zero; i += G.nb * blockDim.x)
+        p[i] = 0;
+}
+
+__global__ __launch_bounds__(kClassifyBlock) void k_classify_many(
+    const McJob *__restrict__ jobs, const uint32_t *__restrict__ map)
+{
+    VGrid G;
+    const McJob &J = mc_job(jobs, map, kMcClassify, G);
+    const BinArgs &a = J.B;
+    classify_body(a.n_new, J.wp_offsets, 0, J.wf_edge, J.wf_prev, a.bs, J.edge_mask, a.max_hs,
+                  J.times, J.topts, a.dt, a.tmax_steps, J.has, const_cast<HsPath *>(a.recs),
+                  a.ctl, a.hist, a.acc, J.clamped, J.row_src, J.parent_cp, a.path_cp, a.cp_base,
+                  G);
+}
+
+__global__ __launch_bounds__(256) void k_copy_listed_many(const McJob *__restrict__ jobs,
+                                                          const uint32_t *__restrict__ map)
+{
+    VGrid G;
+    const McJob &J = mc_job(jobs, map, kMcCopy, G);
+    const BinArgs &a = J.B;
+    copy_listed_body<16>(a.recs, a.ctl, (const uint8_t *)a.valid_src, (uint8_t *)a.valid_dst,
+                         a.dst_base, J.row_bytes, G);
 }
 
 __global__ __launch_bounds__(256) void k_row_src_iota(int n, int *__restrict__ row_src)
@@ -1614,6 +1686,168 @@ static int launch_propagate(int n_new, int W_fixed, const int64_t *wp_offsets, c
         PUMP_LAUNCHED();
     }
     return PUMP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The copy-on-write step of many problems, one launch per stage (McJob).
+// ---------------------------------------------------------------------------
+int pump::mc_cow_job(McJob &J, int n_new, const int64_t *wp_offsets, const int *wf_edge,
+                     const int *wf_prev, const float *as, const float *bs,
+                     const uint64_t *edge_mask, const float *times, const float *topts, float dt,
+                     int max_hs, const float *offsets_time, int tmax_steps, float om, int n_mc,
+                     const void *src, const float *parent_cp, void *dst, int *row_src, int n_p,
+                     float *path_cp, void *ws, size_t ws_bytes)
+{
+    if (n_new < 1 || n_p < 1 || !wp_offsets || !ws || !src || !dst || !row_src || !path_cp ||
+        !parent_cp)
+        return PUMP_E_INVALID;
+    const BatchLayout L = BatchLayout::fill(n_new, tmax_steps, ws_bytes);
+    const int64_t row_bytes = (int64_t)((n_p + 63) / 64) * 8;
+    if (ws_bytes < L.total || row_bytes % 16 != 0 ||
+        (((uintptr_t)src | (uintptr_t)dst) & 15) != 0)
+        return PUMP_E_INVALID;
+    uint8_t *base = reinterpret_cast<uint8_t *>(ws);
+    J = McJob{};
+    BinArgs &B = J.B;
+    B.n_new = n_new;
+    B.n_p = n_p;
+    B.n_mc = n_mc;
+    B.tmax_steps = tmax_steps;
+    B.max_hs = max_hs;
+    B.known = edge_mask ? 1 : 0;
+    B.dt = dt;
+    B.om = om;
+    B.as = as;
+    B.bs = bs;
+    B.offsets_time = offsets_time;
+    B.recs = reinterpret_cast<HsPath *>(base + L.recs);
+    B.acc = reinterpret_cast<unsigned long long *>(base + L.acc);
+    B.ctl = reinterpret_cast<int *>(base);
+    B.hist = reinterpret_cast<int *>(base + L.hist);
+    B.cursor = reinterpret_cast<int *>(base + L.cursor);
+    B.tstart = reinterpret_cast<int *>(base + L.tstart);
+    B.chunks = reinterpret_cast<BinChunk *>(base + L.chunks);
+    B.entries = reinterpret_cast<BinEntry *>(base + L.entries);
+    B.entry_cap = L.entry_cap;
+    B.chunk_cap = L.chunk_cap;
+    B.valid_src = src;
+    B.valid_dst = dst;
+    B.dst_base = 0;
+    B.path_cp = path_cp;
+    B.cp_base = 0;
+    B.split = nullptr;
+    B.band = nullptr;
+    B.nq = ppw_groups(true, false, n_p, max_hs);
+    J.wp_offsets = wp_offsets;
+    J.wf_edge = wf_edge;
+    J.wf_prev = wf_prev;
+    J.edge_mask = edge_mask;
+    J.times = times;
+    J.topts = topts;
+    J.has = base + L.has;
+    J.clamped = nullptr;
+    J.row_src = row_src;
+    J.parent_cp = parent_cp;
+    J.row_bytes = row_bytes;
+    J.n_zero = (int64_t)(L.tstart / 4);
+    return PUMP_OK;
+}
+
+int pump::mc_plan(McJob *jobs, int n_jobs, McGrids &g, void **d_maps, hipStream_t s)
+{
+    *d_maps = nullptr;
+    g = McGrids{};
+    if (!jobs || n_jobs < 1 || n_jobs > 65536)
+        return PUMP_E_INVALID;
+    int64_t total_new = 0;
+    for (int k = 0; k < n_jobs; ++k)
+        total_new += jobs[k].B.n_new;
+    int dev = 0, cus = 256;
+    PUMP_HIP(hipGetDevice(&dev));
+    PUMP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // grid-stride stages share about one full grid (32 workgroups per CU)
+    // in proportion to each problem's wavefront
+    const int64_t budget = (int64_t)cus * 32;
+    const auto share = [&](int64_t single, int64_t n) {
+        const int64_t part = (budget * n + total_new - 1) / total_new;
+        return std::max<int64_t>(1, std::min(single, part));
+    };
+    std::vector<uint32_t> maps[kMcStages];
+    for (int k = 0; k < n_jobs; ++k) {
+        McJob &J = jobs[k];
+        const int64_t n = J.B.n_new;
+        const int groups = (J.B.n_p + 63) / 64;
+        const int nq = (J.B.nq == 1 || J.B.nq == 2 || J.B.nq == 8) ? J.B.nq : 4;
+        const int hs = nq == 1 ? kMcHs1 : nq == 2 ? kMcHs2 : nq == 8 ? kMcHs8 : kMcHs4;
+        int64_t nb[kMcStages] = {};
+        nb[kMcZero] = std::max<int64_t>(1, std::min<int64_t>(32, (J.n_zero + 2047) / 2048));
+        nb[kMcClassify] = grid_for(n, kClassifyBlock, 2048);
+        nb[kMcScan] = 1;
+        nb[kMcCopy] = share(wave_blocks_for(n), n);
+        nb[kMcEmit] = share(grid_for(n, 256, 4096), n);
+        nb[hs] = share(wave_blocks_for(J.B.chunk_cap * ((groups + nq - 1) / nq)), n);
+        nb[kMcGroups] = share(
+            std::min<int64_t>(wave_blocks_for(n * ((groups + kHsGroups - 1) / kHsGroups)), 1024),
+            n);
+        nb[kMcCp] = share(wave_blocks_for(n), n);
+        for (int st = 0; st < kMcStages; ++st) {
+            if (nb[st] > 65536)
+                return PUMP_E_INVALID;
+            J.nb[st] = (int)nb[st];
+            for (int64_t b = 0; b < nb[st]; ++b)
+                maps[st].push_back(((uint32_t)k << 16) | (uint32_t)b);
+        }
+    }
+    size_t total = 0;
+    for (int st = 0; st < kMcStages; ++st)
+        total += maps[st].size();
+    uint32_t *d = nullptr;
+    PUMP_HIP(hipMalloc((void **)&d, std::max<size_t>(total, 1) * sizeof(uint32_t)));
+    size_t off = 0;
+    for (int st = 0; st < kMcStages; ++st) {
+        g.block_map[st] = d + off;
+        g.blocks[st] = (int64_t)maps[st].size();
+        if (!maps[st].empty()) {
+            const hipError_t e = hipMemcpyAsync(d + off, maps[st].data(),
+                                                maps[st].size() * sizeof(uint32_t),
+                                                hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) {
+                hipFree(d);
+                PUMP_HIP(e);
+            }
+        }
+        off += maps[st].size();
+    }
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        hipFree(d);
+        PUMP_HIP(e);
+    }
+    *d_maps = d;
+    return PUMP_OK;
+}
+
+int pump::mc_cow_many(const McJob *d_jobs, const McGrids &g, hipStream_t s)
+{
+    if (g.blocks[kMcZero] > 0) {
+        hipLaunchKernelGGL(k_zero_many, dim3((unsigned)g.blocks[kMcZero]), dim3(256), 0, s,
+                           d_jobs, g.block_map[kMcZero]);
+        PUMP_LAUNCHED();
+    }
+    if (g.blocks[kMcClassify] > 0) {
+        hipLaunchKernelGGL(k_classify_many, dim3((unsigned)g.blocks[kMcClassify]),
+                           dim3(kClassifyBlock), 0, s, d_jobs, g.block_map[kMcClassify]);
+        PUMP_LAUNCHED();
+    }
+    int r = bins_many(d_jobs, g, false, s);
+    if (r)
+        return r;
+    if (g.blocks[kMcCopy] > 0) {
+        hipLaunchKernelGGL(k_copy_listed_many, dim3((unsigned)g.blocks[kMcCopy]), dim3(256), 0,
+                           s, d_jobs, g.block_map[kMcCopy]);
+        PUMP_LAUNCHED();
+    }
+    return bins_many(d_jobs, g, true, s);
 }
 
 extern "C" int pump_propagate_cp(int num_new_paths, int waypoints_count,

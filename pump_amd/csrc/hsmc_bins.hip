@@ -90,11 +90,10 @@ __device__ __forceinline__ int time_total(const int *__restrict__ hist, int t, i
 // One workgroup: time-index starts, chunks, entry/chunk totals, overflow.
 // The pairs of time index t occupy [tstart[t], tstart[t+1]), sub-bin by
 // sub-bin (sub-bin k: paths with w % kSubBins == k).
-__global__ __launch_bounds__(kScanBlock) void k_bin_scan(int T, const int *__restrict__ hist,
-                                                         int *__restrict__ tstart,
-                                                         BinChunk *__restrict__ chunks,
-                                                         int *__restrict__ ctl, int64_t entry_cap,
-                                                         int64_t chunk_cap)
+__device__ __forceinline__ void bin_scan_body(int T, const int *__restrict__ hist,
+                                              int *__restrict__ tstart,
+                                              BinChunk *__restrict__ chunks, int *__restrict__ ctl,
+                                              int64_t entry_cap, int64_t chunk_cap)
 {
     __shared__ int tot_e, tot_c;
     constexpr int kKeep = 4; // time indices per thread whose totals stay in registers
@@ -137,22 +136,30 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan(int T, const int *__res
     }
 }
 
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan(int T, const int *__restrict__ hist,
+                                                         int *__restrict__ tstart,
+                                                         BinChunk *__restrict__ chunks,
+                                                         int *__restrict__ ctl, int64_t entry_cap,
+                                                         int64_t chunk_cap)
+{
+    bin_scan_body(T, hist, tstart, chunks, ctl, entry_cap, chunk_cap);
+}
+
 // lane = listed path: its (waypoint, time index) pairs into the sorted array
 // at tstart[ti] + (pairs of lower sub-bins) + (rank within the sub-bin)
-__global__ __launch_bounds__(256) void k_bin_emit(const HsPath *__restrict__ recs,
-                                                  const int *__restrict__ ctl, int known,
-                                                  const float *__restrict__ bs, int max_hs,
-                                                  float dt, int tmax_steps,
-                                                  const int *__restrict__ hist,
-                                                  const int *__restrict__ tstart,
-                                                  int *__restrict__ cursor,
-                                                  BinEntry *__restrict__ entries)
+__device__ __forceinline__ void bin_emit_body(const HsPath *__restrict__ recs,
+                                              const int *__restrict__ ctl, int known,
+                                              const float *__restrict__ bs, int max_hs, float dt,
+                                              int tmax_steps, const int *__restrict__ hist,
+                                              const int *__restrict__ tstart,
+                                              int *__restrict__ cursor,
+                                              BinEntry *__restrict__ entries, const VGrid &G)
 {
     if (ctl[kCtlOverflow])
         return;
     const int n = ctl[kCtlListed];
-    for (int slot = blockIdx.x * blockDim.x + threadIdx.x; slot < n;
-         slot += gridDim.x * blockDim.x) {
+    for (int64_t slot = G.bid * blockDim.x + threadIdx.x; slot < n;
+         slot += G.nb * blockDim.x) {
         const HsPath r = recs[slot];
         PathInfo pi;
         pi.g0 = r.g0;
@@ -175,9 +182,22 @@ __global__ __launch_bounds__(256) void k_bin_emit(const HsPath *__restrict__ rec
     }
 }
 
+__global__ __launch_bounds__(256) void k_bin_emit(const HsPath *__restrict__ recs,
+                                                  const int *__restrict__ ctl, int known,
+                                                  const float *__restrict__ bs, int max_hs,
+                                                  float dt, int tmax_steps,
+                                                  const int *__restrict__ hist,
+                                                  const int *__restrict__ tstart,
+                                                  int *__restrict__ cursor,
+                                                  BinEntry *__restrict__ entries)
+{
+    bin_emit_body(recs, ctl, known, bs, max_hs, dt, tmax_steps, hist, tstart, cursor, entries,
+                  hw_grid());
+}
+
 // Binned test: one wave per (chunk, 64*NQ particles).
 template <bool kBits, int NQ>
-__device__ __forceinline__ void bin_test(const BinArgs &a, int64_t n_chunks)
+__device__ __forceinline__ void bin_test(const BinArgs &a, int64_t n_chunks, const VGrid &G)
 {
     const int lane = threadIdx.x & 63;
     const int n_p = a.n_p, max_hs = a.max_hs;
@@ -185,8 +205,8 @@ __device__ __forceinline__ void bin_test(const BinArgs &a, int64_t n_chunks)
     const int n_groups = (n_p + 63) >> 6;
     const int n_sg = (n_groups + NQ - 1) / NQ;
     const int64_t n_items = n_chunks * n_sg;
-    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t item = wave_index(); item < n_items; item += n_waves) {
+    const int64_t n_waves = wave_count(G);
+    for (int64_t item = wave_index(G); item < n_items; item += n_waves) {
         const int ch = (int)(item / n_sg);
         const int sg = (int)(item - (int64_t)ch * n_sg);
         const BinChunk c = a.chunks[ch];
@@ -512,15 +532,15 @@ __device__ __forceinline__ void bin_test_mfma(const BinArgs &a, int64_t n_chunks
 // the last of its waves (a 64-bit atomic ticket packing the finished-wave
 // count and the particle count).
 template <bool kBits, int NQ>
-__device__ __forceinline__ void group_pass(const BinArgs &a, int64_t n_listed)
+__device__ __forceinline__ void group_pass(const BinArgs &a, int64_t n_listed, const VGrid &G)
 {
     const int lane = threadIdx.x & 63;
     const int n_p = a.n_p;
     const int n_groups = (n_p + 63) >> 6;
     const int n_sg = (n_groups + NQ - 1) / NQ; // waves per listed path
     const int64_t n_items = n_listed * n_sg;
-    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t item = wave_index(); item < n_items; item += n_waves) {
+    const int64_t n_waves = wave_count(G);
+    for (int64_t item = wave_index(G); item < n_items; item += n_waves) {
         const int slot = (int)(item / n_sg);
         const int sg = (int)(item - (int64_t)slot * n_sg);
         const HsPath r = a.recs[slot];
@@ -601,7 +621,7 @@ __global__ __launch_bounds__(256) void k_hs_pass(const BinArgs a)
         __shared__ MfmaRow rows[256 / 64][kMfmaRows];
         bin_test_mfma<kBits, NQ>(a, a.ctl[kCtlChunks], rows[threadIdx.x >> 6]);
     } else {
-        bin_test<kBits, NQ>(a, a.ctl[kCtlChunks]);
+        bin_test<kBits, NQ>(a, a.ctl[kCtlChunks], hw_grid());
     }
 }
 
@@ -630,23 +650,24 @@ __global__ __launch_bounds__(256) void k_hs_groups(const BinArgs a)
 {
     if (a.ctl[kCtlOverflow] == 0)
         return;
-    group_pass<kBits, kHsGroups>(a, a.ctl[kCtlListed]);
+    group_pass<kBits, kHsGroups>(a, a.ctl[kCtlListed], hw_grid());
 }
 
 // One wave per listed path: CP from its finished child row.
 template <bool kBits>
-__global__ __launch_bounds__(256) void k_cp_listed(const HsPath *__restrict__ recs,
-                                                   const int *__restrict__ ctl,
-                                                   const void *__restrict__ valid_dst,
-                                                   int64_t dst_base, int n_p,
-                                                   float *__restrict__ path_cp, int64_t cp_base)
+__device__ __forceinline__ void cp_listed_body(const HsPath *__restrict__ recs,
+                                               const int *__restrict__ ctl,
+                                               const void *__restrict__ valid_dst,
+                                               int64_t dst_base, int n_p,
+                                               float *__restrict__ path_cp, int64_t cp_base,
+                                               const VGrid &G)
 {
     if (ctl[kCtlOverflow])
         return;
     const int lane = threadIdx.x & 63;
     const int n = ctl[kCtlListed];
-    const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-    for (int64_t slot = wave_index(); slot < n; slot += n_waves) {
+    const int64_t n_waves = wave_count(G);
+    for (int64_t slot = wave_index(G); slot < n; slot += n_waves) {
         const int64_t row = dst_base + recs[slot].w;
         int count = 0; // valid (bits) or invalid (bytes) particles
         if (kBits) {
@@ -665,6 +686,77 @@ __global__ __launch_bounds__(256) void k_cp_listed(const HsPath *__restrict__ re
     }
 }
 
+template <bool kBits>
+__global__ __launch_bounds__(256) void k_cp_listed(const HsPath *__restrict__ recs,
+                                                   const int *__restrict__ ctl,
+                                                   const void *__restrict__ valid_dst,
+                                                   int64_t dst_base, int n_p,
+                                                   float *__restrict__ path_cp, int64_t cp_base)
+{
+    cp_listed_body<kBits>(recs, ctl, valid_dst, dst_base, n_p, path_cp, cp_base, hw_grid());
+}
+
+// ---------------------------------------------------------------------------
+// The same stages for many problems per launch (copy-on-write step, bit
+// validity, FP32 tests): workgroup b serves problem map[b] >> 16 as its
+// workgroup map[b] & 0xffff of nb[stage].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ const McJob &mc_job(const McJob *__restrict__ jobs,
+                                               const uint32_t *__restrict__ map, int stage,
+                                               VGrid &G)
+{
+    const uint32_t m = map[blockIdx.x];
+    const McJob &J = jobs[m >> 16];
+    G = VGrid{(int64_t)(m & 0xffffu), (int64_t)J.nb[stage]};
+    return J;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_bin_scan_many(const McJob *__restrict__ jobs,
+                                                              const uint32_t *__restrict__ map)
+{
+    VGrid G;
+    const BinArgs &a = mc_job(jobs, map, kMcScan, G).B;
+    bin_scan_body(a.tmax_steps, a.hist, a.tstart, a.chunks, a.ctl, a.entry_cap, a.chunk_cap);
+}
+
+__global__ __launch_bounds__(256) void k_bin_emit_many(const McJob *__restrict__ jobs,
+                                                       const uint32_t *__restrict__ map)
+{
+    VGrid G;
+    const BinArgs &a = mc_job(jobs, map, kMcEmit, G).B;
+    bin_emit_body(a.recs, a.ctl, a.known, a.bs, a.max_hs, a.dt, a.tmax_steps, a.hist, a.tstart,
+                  a.cursor, a.entries, G);
+}
+
+template <int NQ>
+__global__ __launch_bounds__(256) void k_hs_pass_many(const McJob *__restrict__ jobs,
+                                                      const uint32_t *__restrict__ map, int stage)
+{
+    VGrid G;
+    const BinArgs &a = mc_job(jobs, map, stage, G).B;
+    if (a.ctl[kCtlOverflow] != 0)
+        return;
+    bin_test<true, NQ>(a, a.ctl[kCtlChunks], G);
+}
+
+__global__ __launch_bounds__(256) void k_hs_groups_many(const McJob *__restrict__ jobs,
+                                                        const uint32_t *__restrict__ map)
+{
+    VGrid G;
+    const BinArgs &a = mc_job(jobs, map, kMcGroups, G).B;
+    if (a.ctl[kCtlOverflow] == 0)
+        return;
+    group_pass<true, kHsGroups>(a, a.ctl[kCtlListed], G);
+}
+
+__global__ __launch_bounds__(256) void k_cp_listed_many(const McJob *__restrict__ jobs,
+                                                        const uint32_t *__restrict__ map)
+{
+    VGrid G;
+    const BinArgs &a = mc_job(jobs, map, kMcCp, G).B;
+    cp_listed_body<true>(a.recs, a.ctl, a.valid_dst, a.dst_base, a.n_p, a.path_cp, a.cp_base, G);
+}
+
 unsigned wave_blocks(int64_t waves)
 {
     int dev = 0, cus = 256;
@@ -676,6 +768,46 @@ unsigned wave_blocks(int64_t waves)
 }
 
 } // namespace
+
+// many problems' scan stage (finish = false, after their classification),
+// or their emit, test, group and CP stages (finish = true, after the row
+// copies)
+int bins_many(const McJob *d_jobs, const McGrids &g, bool finish, hipStream_t s)
+{
+    const auto launch = [&](int stage, auto kernel, int block) {
+        if (g.blocks[stage] > 0)
+            hipLaunchKernelGGL(kernel, dim3((unsigned)g.blocks[stage]), dim3(block), 0, s, d_jobs,
+                               g.block_map[stage]);
+    };
+    if (!finish) {
+        launch(kMcScan, k_bin_scan_many, kScanBlock);
+        PUMP_LAUNCHED();
+        return PUMP_OK;
+    }
+    launch(kMcEmit, k_bin_emit_many, 256);
+    PUMP_LAUNCHED();
+    const int hs_stage[4] = {kMcHs1, kMcHs2, kMcHs4, kMcHs8};
+    for (int i = 0; i < 4; ++i) {
+        const int st = hs_stage[i];
+        if (g.blocks[st] <= 0)
+            continue;
+        const dim3 grid((unsigned)g.blocks[st]);
+        if (i == 0)
+            hipLaunchKernelGGL(k_hs_pass_many<1>, grid, dim3(256), 0, s, d_jobs, g.block_map[st], st);
+        else if (i == 1)
+            hipLaunchKernelGGL(k_hs_pass_many<2>, grid, dim3(256), 0, s, d_jobs, g.block_map[st], st);
+        else if (i == 2)
+            hipLaunchKernelGGL(k_hs_pass_many<4>, grid, dim3(256), 0, s, d_jobs, g.block_map[st], st);
+        else
+            hipLaunchKernelGGL(k_hs_pass_many<8>, grid, dim3(256), 0, s, d_jobs, g.block_map[st], st);
+        PUMP_LAUNCHED();
+    }
+    launch(kMcGroups, k_hs_groups_many, 256);
+    PUMP_LAUNCHED();
+    launch(kMcCp, k_cp_listed_many, 256);
+    PUMP_LAUNCHED();
+    return PUMP_OK;
+}
 
 int bins_scan(const BinArgs &a, hipStream_t s)
 {

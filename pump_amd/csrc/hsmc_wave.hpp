@@ -254,6 +254,20 @@ __device__ __forceinline__ int64_t wave_index()
            __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 
+// The workgroup index and grid size a kernel body works with: the
+// hardware's for a one-problem launch, or the problem's own slice of the
+// grid when one launch serves many problems (the body's grid-stride loops
+// then cover that problem's work only).
+struct VGrid {
+    int64_t bid, nb;
+};
+__device__ __forceinline__ VGrid hw_grid() { return VGrid{(int64_t)blockIdx.x, (int64_t)gridDim.x}; }
+__device__ __forceinline__ int64_t wave_index(const VGrid &G)
+{
+    return G.bid * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+}
+__device__ __forceinline__ int64_t wave_count(const VGrid &G) { return G.nb * (blockDim.x >> 6); }
+
 constexpr int kWalkGroups = 4; // particle groups of 64 per walk, wave-per-path kernel
 constexpr int kHsGroups = 4;   // the same for the group form of the batched pass
 
@@ -323,5 +337,61 @@ struct BinArgs {
 };
 int bins_scan(const BinArgs &a, hipStream_t s);
 int bins_finish(const BinArgs &a, bool bits, hipStream_t s);
+
+// The copy-on-write MC step (bit validity, FP32 tests) of many problems in
+// one launch per stage.  A problem's job holds everything its stages read;
+// stage j of the launch gives problem k nb[j] workgroups, and block_map[j][b]
+// names the problem (high 16 bits) and the problem-local workgroup index
+// (low 16 bits) of workgroup b.
+enum : int {
+    kMcZero = 0,
+    kMcClassify,
+    kMcScan,
+    kMcCopy,
+    kMcEmit,
+    kMcHs1, // binned tests with 1, 2, 4, 8 particle groups per wave
+    kMcHs2,
+    kMcHs4,
+    kMcHs8,
+    kMcGroups,
+    kMcCp,
+    kMcStages
+};
+struct McJob {
+    BinArgs B;
+    // classification inputs
+    const int64_t *wp_offsets;
+    const int *wf_edge, *wf_prev;
+    const uint64_t *edge_mask;
+    const float *times, *topts;
+    uint8_t *has;
+    unsigned long long *clamped;
+    int *row_src;
+    const float *parent_cp;
+    int64_t row_bytes, n_zero;
+    int nb[kMcStages];
+};
+struct McGrids {
+    const uint32_t *block_map[kMcStages]; // device arrays
+    int64_t blocks[kMcStages];
+};
+int bins_many(const McJob *d_jobs, const McGrids &g, bool finish, hipStream_t s);
+
+// The many-problem copy-on-write MC step (hsmc.hip).  mc_cow_job fills one
+// problem's job from the arguments of pump_propagate_cp_cow (validity_bits
+// 1, precision 0, dst_base and cp_base 0); it fails with PUMP_E_INVALID where
+// the one-launch form does not apply (no workspace of the binned layout, no
+// waypoint offsets, rows not in 16-byte units).  mc_plan sizes every stage
+// for every job (setting their nb) and builds the block maps in one device
+// allocation (*d_maps, freed by the caller); mc_cow_many launches the
+// stages for jobs already in device memory.
+int mc_cow_job(McJob &J, int n_new, const int64_t *wp_offsets, const int *wf_edge,
+               const int *wf_prev, const float *as, const float *bs, const uint64_t *edge_mask,
+               const float *times, const float *topts, float dt, int max_hs,
+               const float *offsets_time, int tmax_steps, float om, int n_mc, const void *src,
+               const float *parent_cp, void *dst, int *row_src, int n_p, float *path_cp,
+               void *ws, size_t ws_bytes);
+int mc_plan(McJob *jobs, int n_jobs, McGrids &g, void **d_maps, hipStream_t s);
+int mc_cow_many(const McJob *d_jobs, const McGrids &g, hipStream_t s);
 
 } // namespace pump

@@ -209,9 +209,10 @@ def test_batch_all_256_instances(noise):
 
 def test_batch_mc_step(noise):
     """The HSMC step of every instance captured as the batch's third graph
-    (pump_batch_mc_build): each instance's children's CPs equal the CPU
-    oracle's propagateValidParticles + calculateCP_HSMC on the same wavefront
-    and tables, after graph replays and an eager pass too."""
+    (pump_batch_mc_build, one launch per stage for all instances): each
+    instance's children's CPs equal the CPU oracle's propagateValidParticles
+    + calculateCP_HSMC on the same wavefront and tables, after graph replays
+    and an eager pass too."""
     import oracle
     from oracle import bp, fp, ip, lp
     O = oracle.load()
@@ -221,6 +222,8 @@ def test_batch_mc_step(noise):
     n_par = 64
     ms = b.mc_build(n_parents=n_par, max_paths=20000, seed=7)
     print("batch MC step, first launch", ms, "ms", flush=True)
+    # every instance's step in one launch per stage
+    assert b.info.mc_one_launch == 1
 
     def check_all(label):
         for k in range(len(probs)):
