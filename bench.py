@@ -757,7 +757,8 @@ def batch_pass(P, n_inst, reps):
         mc_steps += int(probs[k][3].n_p) * int((wo[e + 1] - wo[e]).sum())
     one_launch = int(b.info.mc_one_launch)
     log(f"batch HSMC step: {ms_mc:.2f} ms graph vs {ms_mc_eager:.2f} ms eager, "
-        f"{mc_steps / (ms_mc / 1e3):.3e} particle-steps/s, one launch per stage {one_launch}")
+        f"{mc_steps / (ms_mc / 1e3):.3e} particle-steps/s, one launch per stage {one_launch} "
+        f"({b.info.mc_particles_per_wave} particles per wave)")
     out = {"n_inst": int(info.n_inst), "n_pairs": int(info.n_pairs),
            "n_edges": int(info.n_edges), "n_waypoints": int(info.n_waypoints),
            "pairs_blocks_per_cu": int(info.pairs_blocks_per_cu),
@@ -772,7 +773,8 @@ def batch_pass(P, n_inst, reps):
            "edge_obs_tile": int(info.edge_obs_tile),
            "hsmc": {"ms_graph": ms_mc, "ms_eager": ms_mc_eager, "particle_steps": mc_steps,
                     "particle_steps_per_s": mc_steps / (ms_mc / 1e3),
-                    "one_launch_per_stage": one_launch}}
+                    "one_launch_per_stage": one_launch,
+                    "particles_per_wave": int(b.info.mc_particles_per_wave)}}
     b.close()
     return out
 

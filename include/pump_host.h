@@ -251,6 +251,7 @@ typedef struct pump_batch_info {
                                  earlier build of the same configuration in this process */
     int mc_one_launch;        /* 1: the MC stage (pump_batch_mc_build) runs one launch per
                                  stage for all instances; 0: one step per instance */
+    int mc_particles_per_wave; /* the one-launch MC stage's particles per wave (timed) */
 } pump_batch_info;
 
 int pump_batch_build(int n_inst, const pump_batch_instance *inst, const float *offsets_time_host,
@@ -269,17 +270,17 @@ int pump_batch_replay(pump_batch *b, int stage, int reps, float *ms);
  * instance's wavefront is its free edges in edge order (at most max_paths),
  * child i expanded from parent i % n_parents; the parents are all-valid
  * rows with CP 0 and times uniform in [0, 2) s drawn from seed (splitmix64).
- * Before the capture, the particles per wave of each MC configuration
- * (particle count, half-space slots) are tuned once
- * (pump_hsmc_autotune_cow) on its first instance.  The steps of all
- * instances then run as one launch per stage (workspace clearing,
- * classification, time-index scan, row copies, entry emission, binned tests
- * by particles per wave, overflow group form, CP), each workgroup serving one
- * instance's share of the stage (pump_batch_info.mc_one_launch = 1).  Where
- * an instance's step cannot take that form, the steps instead fork from the
- * batch stream onto eight side streams of default priority (instance k on
- * stream k mod 8) and join back.  ms receives the graph's first launch.
- * Needs the batch's noise tables. */
+ * The steps of all instances run as one launch per stage (workspace
+ * clearing, classification, time-index scan, row copies, entry emission,
+ * binned tests, overflow group form, CP), each workgroup serving one
+ * instance's share of the stage (pump_batch_info.mc_one_launch = 1), with
+ * the particles per wave timed over the whole step before the capture
+ * (pump_batch_info.mc_particles_per_wave).  Where an instance's step cannot
+ * take that form, the particles per wave of each MC configuration (particle
+ * count, half-space slots) are tuned once (pump_hsmc_autotune_cow) on its
+ * first instance and the steps fork from the batch stream onto eight side
+ * streams of default priority (instance k on stream k mod 8) and join back.
+ * ms receives the graph's first launch.  Needs the batch's noise tables. */
 int pump_batch_mc_build(pump_batch *b, int n_parents, int max_paths, uint64_t seed, float *ms);
 /* the same steps launched eagerly, reps times; ms = mean per pass */
 int pump_batch_mc_eager(pump_batch *b, int reps, float *ms);

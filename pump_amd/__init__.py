@@ -123,7 +123,7 @@ class BatchInfo(ctypes.Structure):
                 ("n_pairs", c_int64), ("n_edges", c_int64), ("n_waypoints", c_int64),
                 ("ms_pairs", c_float), ("ms_edges", c_float), ("ms_autotune", c_float),
                 ("ms_total", c_float), ("edge_obs_tile", c_int), ("autotune_cached", c_int),
-                ("mc_one_launch", c_int)]
+                ("mc_one_launch", c_int), ("mc_particles_per_wave", c_int)]
 
 
 class ProblemFile(ctypes.Structure):

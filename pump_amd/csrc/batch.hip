@@ -569,23 +569,6 @@ extern "C" int pump_batch_mc_build(pump_batch *b, int n_parents, int max_paths, 
         TRYH(hipStreamSynchronize(s));
     }
     TRYH(hipStreamSynchronize(s));
-    // particles per wave, tuned once per MC configuration (particle count,
-    // half-space slots) on the first instance that has it
-    std::vector<std::array<int, 2>> tuned;
-    for (size_t k = 0; k < b->rms.size(); ++k) {
-        const BatchMc &m = b->mc[k];
-        const pump_roadmap *rm = b->rms[k];
-        const std::array<int, 2> key{rm->prm.n_p, rm->max_hs};
-        if (m.n_new < 4096 || std::find(tuned.begin(), tuned.end(), key) != tuned.end())
-            continue;
-        tuned.push_back(key);
-        int ppw = 0;
-        TRY(pump_hsmc_autotune_cow(
-            1, m.n_new, 0, rm->wp_offsets, m.edge, m.prev, rm->as, rm->bs, rm->edge_hs_mask,
-            m.times, rm->topts_edge, rm->prm.dt, rm->max_hs, b->offsets_time, rm->tmax_steps,
-            rm->prm.offset_mult, rm->n_mc, m.src, m.parent_cp, m.dst, 0, m.row_src, rm->prm.n_p,
-            m.cp, 0, 0, nullptr, nullptr, nullptr, m.ws, m.ws_bytes, (pump_stream_t)s, &ppw));
-    }
     // every problem's step in one launch per stage, where each takes the
     // binned form; else one step per problem on the side streams
     std::vector<McJob> jobs;
@@ -605,13 +588,60 @@ extern "C" int pump_batch_mc_build(pump_batch *b, int n_parents, int max_paths, 
             jobs.push_back(J);
     }
     if (many && !jobs.empty()) {
-        TRY(mc_plan(jobs.data(), (int)jobs.size(), b->mc_grids, &b->mc_maps, s));
+        // particles per wave of the one-launch form: the same for every
+        // problem, the fastest of 64, 128, 256 and 512 for the whole step
+        // (a warm-up pass, then two timed passes each)
         TRY(dalloc(&b->mc_jobs, jobs.size()));
-        TRYH(hipMemcpyAsync(b->mc_jobs, jobs.data(), jobs.size() * sizeof(McJob),
-                            hipMemcpyHostToDevice, s));
-        TRYH(hipStreamSynchronize(s));
+        const auto setup = [&](int nq) -> int {
+            for (McJob &J : jobs)
+                J.B.nq = nq;
+            if (b->mc_maps)
+                hipFree(b->mc_maps);
+            b->mc_maps = nullptr;
+            TRY(mc_plan(jobs.data(), (int)jobs.size(), b->mc_grids, &b->mc_maps, s));
+            TRYH(hipMemcpyAsync(b->mc_jobs, jobs.data(), jobs.size() * sizeof(McJob),
+                                hipMemcpyHostToDevice, s));
+            TRYH(hipStreamSynchronize(s));
+            return PUMP_OK;
+        };
+        int best = 4;
+        float best_ms = 0;
+        for (int nq : {1, 2, 4, 8}) {
+            TRY(setup(nq));
+            TRY(mc_cow_many(b->mc_jobs, b->mc_grids, s));
+            EventTimer tm(s);
+            tm.start();
+            for (int r = 0; r < 2; ++r)
+                TRY(mc_cow_many(b->mc_jobs, b->mc_grids, s));
+            const float t = tm.stop();
+            if (nq == 1 || t < best_ms) {
+                best = nq;
+                best_ms = t;
+            }
+        }
+        TRY(setup(best));
         b->mc_many = true;
         b->info.mc_one_launch = 1;
+        b->info.mc_particles_per_wave = 64 * best;
+    } else {
+        // particles per wave, tuned once per MC configuration (particle
+        // count, half-space slots) on the first instance that has it
+        std::vector<std::array<int, 2>> tuned;
+        for (size_t k = 0; k < b->rms.size(); ++k) {
+            const BatchMc &m = b->mc[k];
+            const pump_roadmap *rm = b->rms[k];
+            const std::array<int, 2> key{rm->prm.n_p, rm->max_hs};
+            if (m.n_new < 4096 || std::find(tuned.begin(), tuned.end(), key) != tuned.end())
+                continue;
+            tuned.push_back(key);
+            int ppw = 0;
+            TRY(pump_hsmc_autotune_cow(
+                1, m.n_new, 0, rm->wp_offsets, m.edge, m.prev, rm->as, rm->bs, rm->edge_hs_mask,
+                m.times, rm->topts_edge, rm->prm.dt, rm->max_hs, b->offsets_time, rm->tmax_steps,
+                rm->prm.offset_mult, rm->n_mc, m.src, m.parent_cp, m.dst, 0, m.row_src,
+                rm->prm.n_p, m.cp, 0, 0, nullptr, nullptr, nullptr, m.ws, m.ws_bytes,
+                (pump_stream_t)s, &ppw));
+        }
     }
     TRY(capture(b, 2, [&] { return mc_launch(b); }));
     float t = 0;

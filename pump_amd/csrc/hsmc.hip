@@ -1765,13 +1765,16 @@ int pump::mc_plan(McJob *jobs, int n_jobs, McGrids &g, void **d_maps, hipStream_
     int dev = 0, cus = 256;
     PUMP_HIP(hipGetDevice(&dev));
     PUMP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    // grid-stride stages share about one full grid (32 workgroups per CU)
-    // in proportion to each problem's wavefront
+    // grid-stride stages share a grid in proportion to each problem's
+    // wavefront: one full grid (32 workgroups per CU) for the binned tests,
+    // four for the per-listed-path stages (emission, row copies, CP), whose
+    // work items are short dependent load chains
     const int64_t budget = (int64_t)cus * 32;
-    const auto share = [&](int64_t single, int64_t n) {
-        const int64_t part = (budget * n + total_new - 1) / total_new;
+    const auto share_of = [&](int64_t single, int64_t n, int64_t grids) {
+        const int64_t part = (budget * grids * n + total_new - 1) / total_new;
         return std::max<int64_t>(1, std::min(single, part));
     };
+    const auto share = [&](int64_t single, int64_t n) { return share_of(single, n, 1); };
     std::vector<uint32_t> maps[kMcStages];
     for (int k = 0; k < n_jobs; ++k) {
         McJob &J = jobs[k];
@@ -1783,13 +1786,13 @@ int pump::mc_plan(McJob *jobs, int n_jobs, McGrids &g, void **d_maps, hipStream_
         nb[kMcZero] = std::max<int64_t>(1, std::min<int64_t>(32, (J.n_zero + 2047) / 2048));
         nb[kMcClassify] = grid_for(n, kClassifyBlock, 2048);
         nb[kMcScan] = 1;
-        nb[kMcCopy] = share(wave_blocks_for(n), n);
-        nb[kMcEmit] = share(grid_for(n, 256, 4096), n);
+        nb[kMcCopy] = share_of(wave_blocks_for(n), n, 4);
+        nb[kMcEmit] = share_of(grid_for(n, 256, 4096), n, 4);
         nb[hs] = share(wave_blocks_for(J.B.chunk_cap * ((groups + nq - 1) / nq)), n);
         nb[kMcGroups] = share(
             std::min<int64_t>(wave_blocks_for(n * ((groups + kHsGroups - 1) / kHsGroups)), 1024),
             n);
-        nb[kMcCp] = share(wave_blocks_for(n), n);
+        nb[kMcCp] = share_of(wave_blocks_for(n), n, 4);
         for (int st = 0; st < kMcStages; ++st) {
             if (nb[st] > 65536)
                 return PUMP_E_INVALID;
