@@ -22,6 +22,8 @@ Also measured in the same run, each over its own timed repetitions:
       expanded once from a synthetic parent path (particle-steps/s), in the
       byte-validity form (plain FP32, the reference layout) and the bit-mask
       form;
+    * full Monte Carlo CP (MCCP) of the search's 64 deepest paths
+      (segment-checks/s);
     * kernel (a): the all-pairs 2pBVP solve (pair-solves/s), and its
       algorithmic flops (from the counted bisection steps) against the
       FP64 vector peak;
@@ -200,6 +202,8 @@ def main():
 
     # ---------------- the PUMP search, bookkeeping on the device and host --
     search = search_pass(P, rm, prm)
+    mccp = (mccp_pass(P, rm, prm, samples, int(search["num_paths"]), 5)
+            if search["num_paths"] > 1 else None)
 
     # ---------------- kernel (a): all-pairs 2pBVP --------------------------
     # The build's own pair pass is the process's first GPU work (clocks still
@@ -295,13 +299,20 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": "k_edges<true,true,128> (pump_edges_discretize_check)",
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes,
+                     # the two ceilings in edge-checks/s: HBM (algorithmic bytes at
+                     # 8 TB/s) and FP64 VALU (~90 flops per interior waypoint of
+                     # pathPoint, 2pBVP.cu:117-131, at 78.6 TFLOP/s; endpoints are copies)
+                     "hbm_ceiling_edge_checks_per_s": E * HBM_PEAK / alg_bytes,
+                     "valu_ceiling_edge_checks_per_s":
+                         E * FP64_PEAK / (90.0 * max(n_wp - 2 * E, 1))},
         "cpu_baseline": cpu,
         "edges_plain": plain,
         "edges_fp32": edges_fp32,
         "hsmc": hsmc,
         "pairs": pairs,
         "search": search,
+        "mccp": mccp,
         "batch": batch,
         "pipeline_ms": {k: getattr(info, k) for k in (
             "ms_pairs", "ms_select", "ms_edges_build", "ms_nn", "ms_sample_free",
@@ -684,6 +695,63 @@ def search_pass(P, rm, prm):
             "num_considered": d["num_considered"], "status": d["status"],
             "soln_cost": d["soln_cost"], "soln_cp": d["soln_cp"], "same_result": same,
             "smoothing": sm}
+
+
+def mccp_pass(P, rm, prm, samples, num_paths, reps, n=64):
+    """Full Monte Carlo CP (collisionProbability.cu:4-92) in its batched form
+    (pump_collision_probability_many) over the n deepest paths of the last
+    search's path table, each traced back to the init and sampled by
+    findOptimalPath.  A segment-check is one particle's segment between
+    consecutive path rows tested against every box (the reference checks rows
+    t -> t+1 for t < T-2; a particle stops at its first hit, rare here)."""
+    tab = rm.paths(num_paths)
+    prev, node = tab["path_prev"], tab["path_node"]
+    depth = np.zeros(prev.size, np.int64)
+    for k in range(1, prev.size):
+        depth[k] = depth[prev[k]] + 1
+    pick = np.lexsort((np.arange(prev.size), -depth))[:n]
+    cap = int(prm.path_cap)
+    s = np.ascontiguousarray(samples, np.float32)
+    d = rm.device()
+    info = rm.info
+    paths = np.zeros((pick.size, cap, 6), np.float32)
+    T = np.zeros(pick.size, np.int32)
+    for q, k in enumerate(pick.tolist()):
+        chain = []
+        while k != -2:
+            chain.append(int(node[k]))
+            k = int(prev[k])
+        xs = s[np.asarray(chain[::-1])]
+        rows, _ = P.find_optimal_path(xs, dt=prm.dt, cap=cap)
+        paths[q, :rows.shape[0]] = rows
+        T[q] = rows.shape[0]
+    g_p, g_T = P.DeviceArray(paths), P.DeviceArray(T)
+    g_h = P.DeviceArray(shape=(pick.size,), dtype=np.int32, fill=0)
+    stream = P.Stream()
+    timer = P.Timer(stream)
+
+    def run():
+        P.check("pump_collision_probability_many", P.lib.pump_collision_probability_many(
+            d.obstacles, int(info.n_obs), d.xcomb, int(info.tmax_steps), int(info.n_mc),
+            np.float32(prm.offset_mult), g_p.p, cap, g_T.p, int(pick.size), g_h.p, 1, stream.h))
+
+    run()
+    stream.sync()
+    timer.start()
+    for _ in range(reps):
+        run()
+    ms = timer.stop() / reps
+    hits = g_h.numpy()
+    nseg = np.minimum(np.maximum(T.astype(np.int64) - 2, 0), min(int(info.tmax_steps), cap) - 1)
+    checks = int(info.n_mc) * int(nseg.sum())
+    out = {"paths": int(pick.size), "rows_min": int(T.min()), "rows_max": int(T.max()),
+           "n_mc": int(info.n_mc), "ms": ms, "segment_checks": checks,
+           "segment_checks_per_s": checks / (ms / 1e3), "mean_cp": float(hits.mean() / info.n_mc)}
+    log(f"full MC (MCCP) of {pick.size} paths ({T.min()}-{T.max()} rows, {info.n_mc} particles): "
+        f"{ms:.3f} ms, {out['segment_checks_per_s']:.3e} segment-checks/s")
+    for a in (g_p, g_T, g_h):
+        a.free()
+    return out
 
 
 def batch_pass(P, n_inst, reps):
