@@ -359,7 +359,8 @@ def hsmc_pass(P, rm, d, prm, info, stream, reps):
     g_edge, g_prev = P.DeviceArray(free), P.DeviceArray(prev)
     g_times = P.DeviceArray(times)
     g_cp = P.DeviceArray(shape=(n_new,), dtype=np.float32, fill=0)
-    ws_bytes = int(P.lib.pump_propagate_workspace(n_new, T))
+    # room for every marked (path, waypoint) pair: the binned form at any density
+    ws_bytes = int(P.lib.pump_propagate_workspace_pairs(n_new, T, n_marked))
     g_ws = P.DeviceArray(shape=(ws_bytes,), dtype=np.uint8, fill=0)
     # H_w histogram: expanded waypoints by their number of stored
     # half-spaces (waypoint k of edge e counts once per child expanding e)

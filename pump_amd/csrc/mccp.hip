@@ -5,7 +5,8 @@
 // the path's segments serially.  Here one wave owns one particle and its 64
 // lanes test 64 consecutive segments at once (the particle's noise row is
 // contiguous in t, so the loads coalesce); a wave ballot ends the particle at
-// the first blocked segment.  Obstacles are staged in LDS.  Paths of a batch
+// the first blocked segment.  Obstacles are staged in LDS, and each lane
+// tests only the boxes that meet the union of the wave's 64 segment boxes.  Paths of a batch
 // are the grid's y dimension, so many candidate paths share one launch.
 #include "pump_device.hpp"
 #include "pump_internal.hpp"
@@ -40,14 +41,44 @@ __global__ __launch_bounds__(256) void k_mccp(const float *__restrict__ obstacle
     const float *xc = xcomb + (int64_t)p * 3 * tmax_steps;
     for (int t0 = 0; t0 < nseg; t0 += 64) {
         const int t = t0 + lane;
-        bool hit = false;
-        if (t < nseg) {
-            float v[3], w[3];
+        const bool act = t < nseg;
+        float v[3] = {0, 0, 0}, w[3] = {0, 0, 0};
+        float bmin[3] = {INFINITY, INFINITY, INFINITY}, bmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        if (act) {
             for (int d = 0; d < 3; ++d) {
                 v[d] = path[t * kDim + d] + xc[t * 3 + d] * om;
                 w[d] = path[(t + 1) * kDim + d] + xc[(t + 1) * 3 + d] * om;
             }
-            hit = segment_blocked(v, w, o, obstacles, n_obs);
+            seg_bbox(v, w, bmin, bmax);
+        }
+        // Only boxes that meet the union of the wave's segment boxes can be
+        // hit (the broad phase of segment_hits_box uses the same <= test,
+        // and a box separated from the union is separated from each
+        // segment's box), so each lane tests just those: lane-parallel
+        // over the boxes, 64 at a time, then the candidates one by one.
+        float umin[3], umax[3];
+        for (int d = 0; d < 3; ++d) {
+            umin[d] = wave_min(bmin[d]);
+            umax[d] = wave_max(bmax[d]);
+        }
+        bool hit = false;
+        for (int b0 = 0; b0 < n_obs; b0 += 64) {
+            bool meets = false;
+            if (b0 + lane < n_obs) {
+                float lo[3], hi[3];
+                box_bounds(o, obstacles, b0 + lane, lo, hi);
+                meets = !((umax[0] <= lo[0]) | (hi[0] <= umin[0]) | (umax[1] <= lo[1]) |
+                          (hi[1] <= umin[1]) | (umax[2] <= lo[2]) | (hi[2] <= umin[2]));
+            }
+            unsigned long long cand = __ballot(meets);
+            while (cand) {
+                const int b = b0 + __builtin_ctzll(cand);
+                cand &= cand - 1;
+                float lo[3], hi[3];
+                box_bounds(o, obstacles, b, lo, hi);
+                if (act && !hit)
+                    hit = segment_hits_box(v, w, bmin, bmax, lo, hi);
+            }
         }
         if (__ballot(hit) != 0ull) {
             if (lane == 0)
