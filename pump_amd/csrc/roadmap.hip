@@ -202,6 +202,28 @@ __global__ __launch_bounds__(1024) void k_scan_sums(int64_t *__restrict__ tile_s
         tile_sums[nt] = carry;
 }
 
+// the whole scan in one workgroup, tile after tile with a running carry
+// (short arrays: one launch instead of three)
+__global__ __launch_bounds__(1024) void k_scan_small(int64_t *__restrict__ a, int64_t n)
+{
+    __shared__ int64_t sh[32];
+    int64_t carry = 0;
+    for (int64_t t0 = 0; t0 < n; t0 += kScanTile) {
+        const int64_t base = t0 + 2 * threadIdx.x;
+        const int64_t v0 = base < n ? a[base] : 0;
+        const int64_t v1 = base + 1 < n ? a[base + 1] : 0;
+        int64_t total;
+        const int64_t ex = block_exclusive_scan_1024(v0 + v1, sh, total);
+        if (base < n)
+            a[base] = carry + ex;
+        if (base + 1 < n)
+            a[base + 1] = carry + ex + v0;
+        carry += total;
+    }
+    if (threadIdx.x == 0)
+        a[n] = carry;
+}
+
 __global__ __launch_bounds__(1024) void k_scan_apply(int64_t *__restrict__ a, int64_t n,
                                                      const int64_t *__restrict__ tile_sums,
                                                      int64_t nt)
@@ -415,6 +437,11 @@ __global__ __launch_bounds__(256) void k_wcount(const float *__restrict__ topts_
 int scan_exclusive_i64(int64_t *d_a, int64_t n, void *d_ws, size_t ws_bytes, hipStream_t s)
 {
     const int64_t nt = (n + kScanTile - 1) / kScanTile;
+    if (n >= 0 && nt <= 8) {
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, d_a, n);
+        PUMP_LAUNCHED();
+        return PUMP_OK;
+    }
     if (ws_bytes < (size_t)(nt + 1) * sizeof(int64_t) || !d_ws)
         return PUMP_E_CAPACITY;
     int64_t *sums = reinterpret_cast<int64_t *>(d_ws);
