@@ -332,3 +332,18 @@ def test_pair_row_division_by_magic_number():
         q = np.zeros_like(n)
         P.lib.pump_host_div_magic(n.ctypes.data, n.size, d, q.ctypes.data)
         assert np.array_equal(q, n // np.uint32(d)), d
+
+
+def test_workspace_sizing_for_marked_pairs():
+    """pump_propagate_workspace_pairs: no smaller than the default workspace,
+    equal to it up to the default capacity (4 per path + 65536 pairs), and
+    16 bytes plus a chunk record's share per further pair (capped at 2^30)."""
+    for n, T in ((4096, 2001), (1000000, 2001), (6000, 120)):
+        base = P.lib.pump_propagate_workspace(n, T)
+        cap = 4 * n + 65536
+        assert P.lib.pump_propagate_workspace_pairs(n, T, 0) == base
+        assert P.lib.pump_propagate_workspace_pairs(n, T, cap) == base
+        more = P.lib.pump_propagate_workspace_pairs(n, T, cap + 160000)
+        assert 16 * 160000 <= more - base <= 17 * 160000 + 1024
+        huge = P.lib.pump_propagate_workspace_pairs(n, T, 1 << 40)
+        assert huge == P.lib.pump_propagate_workspace_pairs(n, T, 1 << 30)
