@@ -22,6 +22,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/pump_host.h"
@@ -297,6 +298,62 @@ int edge_pass(pump_batch *b, int autotune)
     return launch_timed(b, 1, &b->info.ms_edges);
 }
 
+// rn, edge numbering, neighbour tables and waypoint offsets of every
+// instance.  Each instance's stages are host-synchronous (a stage reads back a
+// count that sizes the next allocation), so instances run side by side on
+// kTableThreads host threads, each with its own stream (default priority);
+// an instance keeps the batch stream outside this call.
+constexpr int kTableThreads = 8;
+
+int tables_all(pump_batch *b)
+{
+    const int n = (int)b->rms.size();
+    if (n == 0)
+        return PUMP_OK;
+    TRYH(hipStreamSynchronize(b->stream)); // the pair tables are complete
+    const int T = std::min(kTableThreads, n);
+    int dev = 0;
+    TRYH(hipGetDevice(&dev));
+    std::vector<hipStream_t> st(T, nullptr);
+    int status = PUMP_OK;
+    for (int j = 0; j < T && status == PUMP_OK; ++j) {
+        const hipError_t e = hipStreamCreate(&st[j]);
+        if (e != hipSuccess) {
+            st[j] = nullptr;
+            note_hip_error(e, __FILE__, __LINE__);
+            status = PUMP_E_HIP;
+        }
+    }
+    std::vector<int> rs(T, PUMP_OK);
+    if (status == PUMP_OK) {
+        std::vector<std::thread> th;
+        for (int j = 0; j < T; ++j)
+            th.emplace_back([&, j] {
+                if (hipSetDevice(dev) != hipSuccess) {
+                    rs[j] = PUMP_E_HIP;
+                    return;
+                }
+                for (int k = j; k < n && rs[j] == PUMP_OK; k += T) {
+                    pump_roadmap *rm = b->rms[k];
+                    rm->stream = st[j];
+                    rs[j] = rm_tables(rm);
+                    if (rs[j] == PUMP_OK && hipStreamSynchronize(st[j]) != hipSuccess)
+                        rs[j] = PUMP_E_HIP;
+                    rm->stream = b->stream;
+                }
+            });
+        for (std::thread &t : th)
+            t.join();
+        for (int j = 0; j < T; ++j)
+            if (status == PUMP_OK && rs[j] != PUMP_OK)
+                status = rs[j];
+    }
+    for (hipStream_t x : st)
+        if (x)
+            hipStreamDestroy(x);
+    return status;
+}
+
 int build(pump_batch *b, int n, const pump_batch_instance *inst, const float *offsets_time_h,
           const float *xcomb_h, int n_mc, int tmax_steps, int autotune)
 {
@@ -317,11 +374,10 @@ int build(pump_batch *b, int n, const pump_batch_instance *inst, const float *of
     for (int k = 0; k < n; ++k)
         TRY(new_instance(b, inst[k], n_mc, tmax_steps));
     TRY(pair_pass(b, autotune));
+    TRY(tables_all(b));
     bool pairs_kept = true;
-    for (pump_roadmap *rm : b->rms) {
-        TRY(rm_tables(rm));
+    for (pump_roadmap *rm : b->rms)
         pairs_kept = pairs_kept && rm->prm.keep_pairs;
-    }
     if (!pairs_kept) {
         // the captured pair pass would write freed buffers
         hipGraphExecDestroy(b->exec[0]);
