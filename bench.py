@@ -202,8 +202,8 @@ def main():
 
     # ---------------- the PUMP search, bookkeeping on the device and host --
     search = search_pass(P, rm, prm)
-    mccp = (mccp_pass(P, rm, prm, samples, int(search["num_paths"]), 5)
-            if search["num_paths"] > 1 else None)
+    mccp, mccp_data = (mccp_pass(P, rm, prm, samples, int(search["num_paths"]), 5)
+                       if search["num_paths"] > 1 else (None, None))
 
     # ---------------- kernel (a): all-pairs 2pBVP --------------------------
     # The build's own pair pass is the process's first GPU work (clocks still
@@ -275,6 +275,9 @@ def main():
         cpu["pair_solves"] = cpu_pairs(rm, samples, prm, g_c, g_t, args.cpu_seconds / 3)
         if hsmc_cp is not None:
             cpu["particle_steps"] = cpu_particles(rm, prm, ot, hsmc_cp, args.cpu_seconds / 3)
+        if mccp_data is not None:
+            cpu["mccp_segment_checks"] = cpu_mccp(prm, obstacles, xc, *mccp_data,
+                                                  args.cpu_seconds / 3)
 
     g_c.free()
     g_t.free()
@@ -752,7 +755,34 @@ def mccp_pass(P, rm, prm, samples, num_paths, reps, n=64):
         f"{ms:.3f} ms, {out['segment_checks_per_s']:.3e} segment-checks/s")
     for a in (g_p, g_T, g_h):
         a.free()
-    return out
+    return out, (paths, T, hits)
+
+
+def cpu_mccp(prm, obstacles, xcomb, paths, T, gpu_hits, seconds):
+    """The CPU oracle's MCCP (collisionProbability.cu:41-92, one thread) over
+    the same paths as the GPU's full-MC batch, path after path until about
+    `seconds`; hits compared with the GPU's."""
+    import oracle
+    O = oracle.load()
+    obs = np.ascontiguousarray(obstacles, np.float32)
+    xc = np.ascontiguousarray(xcomb, np.float32)
+    n_mc, tsteps = xc.shape[0], xc.shape[1]
+    t0 = time.perf_counter()
+    done, checks, agree = 0, 0, True
+    for q in range(paths.shape[0]):
+        h = O.po_mccp_hits(oracle.fp(obs), obs.shape[0], oracle.fp(xc), tsteps, n_mc,
+                           np.float32(prm.offset_mult), oracle.fp(paths[q]), int(T[q]), None)
+        agree &= int(h) == int(gpu_hits[q])
+        done += 1
+        checks += n_mc * max(int(T[q]) - 2, 0)
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    log(f"cpu full MC: {done} paths ({checks} segment-checks) in {dt:.2f} s on one thread, "
+        f"hits agree: {agree}")
+    return {"value": checks / dt, "unit": "segment-checks/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} of the GPU's {paths.shape[0]} full-MC paths", "seconds": dt,
+            "hits_match_gpu": bool(agree)}
 
 
 def batch_pass(P, n_inst, reps):
