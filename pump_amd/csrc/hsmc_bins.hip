@@ -232,21 +232,26 @@ __device__ __forceinline__ void bin_test(const BinArgs &a, int64_t n_chunks, con
         }
         for (int h0 = 0; h0 < max_hs; h0 += kSlotTile) {
             // lane j stages slots h0..h0+kSlotTile-1 of pair j and marks the
-            // live ones (the reference skips b >= 1000, PUMP.cu:642)
+            // live ones (the reference skips b >= 1000, PUMP.cu:642); the
+            // normals are loaded for live slots only
             float b[kSlotTile], av[kSlotTile][3];
             int live = 0;
 #pragma unroll
             for (int h = 0; h < kSlotTile; ++h) {
                 b[h] = 1111.0f;
+                if (lane < ne && h0 + h < max_hs)
+                    b[h] = a.bs[my_wp * max_hs + h0 + h];
+                live |= !(b[h] >= 1000.0f) ? (1 << h) : 0;
+            }
+#pragma unroll
+            for (int h = 0; h < kSlotTile; ++h) {
                 av[h][0] = av[h][1] = av[h][2] = 0.0f;
-                if (lane < ne && h0 + h < max_hs) {
+                if (live & (1 << h)) {
                     const int64_t k = my_wp * max_hs + h0 + h;
-                    b[h] = a.bs[k];
                     av[h][0] = a.as[k * 3 + 0];
                     av[h][1] = a.as[k * 3 + 1];
                     av[h][2] = a.as[k * 3 + 2];
                 }
-                live |= !(b[h] >= 1000.0f) ? (1 << h) : 0;
             }
             for (int j = 0; j < ne; ++j) {
                 const int lm = __builtin_amdgcn_readlane(live, j);
