@@ -19,9 +19,12 @@ Also measured in the same run, each over its own timed repetitions:
       (pump_discretize_edges, then pump_free_edges over the stored
       waypoints), with its flags compared to the fused pass's;
     * kernel (c): one HSMC propagate + CP pass in which every free edge is
-      expanded once from a synthetic parent path (particle-steps/s), in the
-      byte-validity form (plain FP32, the reference layout) and the bit-mask
-      form;
+      expanded once from a synthetic parent path (particle-steps/s): the
+      drop-in pair (byte validity, the reference layout), the batched byte
+      and bit-mask forms and the copy-on-write forms, FP32 and split-BF16;
+    * the FP32-waypoint form of the edge pass (flags compared);
+    * the PUMP search with its bookkeeping on the device and on the host
+      (same path), and the smoothing of its result;
     * full Monte Carlo CP (MCCP) of the search's 64 deepest paths
       (segment-checks/s);
     * kernel (a): the all-pairs 2pBVP solve (pair-solves/s), and its
@@ -30,9 +33,12 @@ Also measured in the same run, each over its own timed repetitions:
     * the uneven batch workload (256 problems, NUM log-uniform in
       [1000, 8000], 5-50 boxes): the batched build's pair pass and edge pass +
       half-space cache (one launch each for all instances, hipGraph replays)
-      against the same passes launched instance by instance;
+      against the same passes launched instance by instance, and the HSMC
+      step of every instance (one launch per stage, hipGraph replays);
     * cpu_baseline: the CPU oracle's edge pass on a bounded sample of the
-      same edges with all host cores.
+      same edges with all host cores, beside it on one thread, and its pair
+      solves, HSMC step and full MC on bounded samples, each checked against
+      the GPU's results.
 Prints one JSON line.
 """
 import argparse
