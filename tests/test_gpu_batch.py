@@ -207,23 +207,20 @@ def test_batch_all_256_instances(noise):
     b.close()
 
 
-def test_batch_mc_step(noise):
-    """The HSMC step of every instance captured as the batch's third graph
-    (pump_batch_mc_build, one launch per stage for all instances): each
-    instance's children's CPs equal the CPU oracle's propagateValidParticles
-    + calculateCP_HSMC on the same wavefront and tables, after graph replays
-    and an eager pass too."""
+def check_mc_step(probs, noise, one_launch):
+    """Runs the batch's MC stage on probs and checks every instance's
+    children's CPs against the CPU oracle, after graph replays and an eager
+    pass too; one_launch: the form the build must pick."""
     import oracle
     from oracle import bp, fp, ip, lp
     O = oracle.load()
     ot, xc = noise
-    probs = problems()
     b = P.Batch([(s, o, g, prm) for _, s, o, g, prm in probs], ot, xc, autotune=False)
     n_par = 64
     ms = b.mc_build(n_parents=n_par, max_paths=20000, seed=7)
-    print("batch MC step, first launch", ms, "ms", flush=True)
-    # every instance's step in one launch per stage
-    assert b.info.mc_one_launch == 1
+    print("batch MC step, first launch", ms, "ms, one launch per stage", b.info.mc_one_launch,
+          flush=True)
+    assert b.info.mc_one_launch == one_launch
 
     def check_all(label):
         for k in range(len(probs)):
@@ -256,3 +253,23 @@ def test_batch_mc_step(noise):
     print("eager", b.mc_eager(reps=2), "ms; graph", b.replay(2, reps=3), "ms", flush=True)
     check_all("after the eager pass")
     b.close()
+
+
+def test_batch_mc_step(noise):
+    """The HSMC step of every instance captured as the batch's third graph
+    (pump_batch_mc_build, one launch per stage for all instances): each
+    instance's children's CPs equal the CPU oracle's propagateValidParticles
+    + calculateCP_HSMC on the same wavefront and tables, after graph replays
+    and an eager pass too."""
+    check_mc_step(problems(), noise, 1)
+
+
+def test_batch_mc_step_side_streams(noise):
+    """An instance whose bit rows are 8 bytes (64 particles) cannot take the
+    one-launch form: the build runs every instance's step on the side
+    streams instead, with the same results."""
+    probs = problems()
+    name, s, o, g, prm = probs[-1]
+    prm.n_p = 64
+    probs[-1] = (name + "_p64", s, o, g, prm)
+    check_mc_step(probs, noise, 0)
