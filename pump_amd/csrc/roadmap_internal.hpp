@@ -90,7 +90,10 @@ inline int ensure_ws(pump_roadmap *rm, size_t bytes)
     return r;
 }
 
-// A device buffer that grows (by half again) on demand.
+// A device buffer that grows on demand: to twice the request, and to at least
+// 4 MiB, so that a search whose wavefront grows from a few paths to hundreds
+// of thousands reallocates each buffer a few times, not dozens (each hipFree
+// also waits for the device).
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
@@ -102,7 +105,7 @@ struct DevBuf {
             hipFree(p);
         p = nullptr;
         bytes = 0;
-        size_t want = b + b / 2;
+        size_t want = b * 2 > ((size_t)4 << 20) ? b * 2 : ((size_t)4 << 20);
         if (hipMalloc(&p, want) != hipSuccess) {
             // the failed attempt leaves its error as the thread's last error:
             // clear it, so that a later launch check does not report it
