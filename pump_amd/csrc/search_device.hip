@@ -48,19 +48,28 @@ constexpr int kBlock = 256;
 inline unsigned blocks_for(int64_t n) { return grid_for(n, kBlock, 1ll << 30); }
 
 // free out-edges to free nodes, per node, in edge order
+// One wave per node, lanes over its out-edges: the count of the edges whose
+// flag and destination sample are free, then (with the counts scanned) those
+// edges in edge order.
 __global__ __launch_bounds__(kBlock) void k_valid_count(const int64_t *__restrict__ row_start,
                                                         const int *__restrict__ edge_dst,
                                                         const uint8_t *__restrict__ free_edges,
                                                         const uint8_t *__restrict__ free_samples,
                                                         int num, int64_t *__restrict__ vcnt)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= num)
+    const int n = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (n >= num) // uniform within the wave
         return;
+    const int64_t end = row_start[n + 1];
     int64_t c = 0;
-    for (int64_t e = row_start[n]; e < row_start[n + 1]; ++e)
-        c += (free_edges[e] && free_samples[edge_dst[e]]) ? 1 : 0;
-    vcnt[n] = c;
+    for (int64_t e0 = row_start[n]; e0 < end; e0 += 64) {
+        const int64_t e = e0 + lane;
+        const bool v = e < end && free_edges[e] && free_samples[edge_dst[e]];
+        c += __popcll(__ballot(v));
+    }
+    if (lane == 0)
+        vcnt[n] = c;
 }
 
 __global__ __launch_bounds__(kBlock) void k_valid_fill(const int64_t *__restrict__ row_start,
@@ -70,13 +79,20 @@ __global__ __launch_bounds__(kBlock) void k_valid_fill(const int64_t *__restrict
                                                        int num, const int64_t *__restrict__ voff,
                                                        int *__restrict__ vedge)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
     if (n >= num)
         return;
+    const int64_t end = row_start[n + 1];
     int64_t o = voff[n];
-    for (int64_t e = row_start[n]; e < row_start[n + 1]; ++e)
-        if (free_edges[e] && free_samples[edge_dst[e]])
-            vedge[o++] = (int)e;
+    for (int64_t e0 = row_start[n]; e0 < end; e0 += 64) {
+        const int64_t e = e0 + lane;
+        const bool v = e < end && free_edges[e] && free_samples[edge_dst[e]];
+        const uint64_t m = __ballot(v);
+        if (v)
+            vedge[o + __popcll(m & ((1ull << lane) - 1ull))] = (int)e;
+        o += __popcll(m);
+    }
 }
 
 __global__ void k_init_search(int *node_head, int num, int init, int *path_prev, int *path_node,
@@ -317,8 +333,8 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
     DevBuf &voff = W->voff, &vedge = W->vedge, &ws = W->scan;
     TRY(voff.ensure(sizeof(int64_t) * ((size_t)num + 1)));
     TRY(ws.ensure(pump_scan_workspace(std::max<int64_t>(num, maxp) + 1)));
-    hipLaunchKernelGGL(k_valid_count, dim3(blocks_for(num)), dim3(kBlock), 0, s, rm->row_start,
-                       rm->edge_dst, rm->is_free_edges, rm->is_free_samples, num,
+    hipLaunchKernelGGL(k_valid_count, dim3(blocks_for((int64_t)num * 64)), dim3(kBlock), 0, s,
+                       rm->row_start, rm->edge_dst, rm->is_free_edges, rm->is_free_samples, num,
                        as<int64_t>(voff));
     PUMP_LAUNCHED();
     TRY(scan_exclusive_i64(as<int64_t>(voff), num, ws.p, ws.bytes, s));
@@ -327,8 +343,8 @@ int rmb::search_device(pump_roadmap *rm, const pump_params &P, pump_search_resul
                         hipMemcpyDeviceToHost, s));
     TRYH(hipStreamSynchronize(s));
     TRY(vedge.ensure(sizeof(int) * (size_t)std::max<int64_t>(n_valid_edges, 1)));
-    hipLaunchKernelGGL(k_valid_fill, dim3(blocks_for(num)), dim3(kBlock), 0, s, rm->row_start,
-                       rm->edge_dst, rm->is_free_edges, rm->is_free_samples, num,
+    hipLaunchKernelGGL(k_valid_fill, dim3(blocks_for((int64_t)num * 64)), dim3(kBlock), 0, s,
+                       rm->row_start, rm->edge_dst, rm->is_free_edges, rm->is_free_samples, num,
                        as<int64_t>(voff), as<int>(vedge));
     PUMP_LAUNCHED();
 
