@@ -148,7 +148,8 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
     int64_t pool = 0;    // next unclaimed pair index in the wave's chunk
     int64_t pool_end = 0;
     CostTerms c;
-    float tu = 0, tl = 0, topt = 0, dcval = 0, tmax = 0;
+    float tu = 0, tl = 0, topt = 0, tmax = 0;
+    float dcval = 0; // numerator ca - cb of the last dcost (1 before the first)
     float *oc = nullptr, *ot = nullptr;
     int64_t oi = 0;
     int state = ST_TMAX, itr = 0, halvings = 0;
@@ -211,14 +212,25 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
             const float h = 0.000001f;
             const float xa = state == ST_FIN ? x : x + h / 2.0f;
             const float xb = state == ST_FIN ? x : x - h / 2.0f;
-            const float ca = cost_eval(c, xa);
-            const float cb = cost_eval(c, xb);
+            // For 2^-20 <= x <= 2^100 both points are at least 4.5e-7, so
+            // tau^3 stays within the reciprocal's exact range.
+            float ca, cb;
+            if (x >= 0x1p-20f && x <= 0x1p100f) {
+                ca = cost_eval_with(c, xa, RecipNewton{});
+                cb = cost_eval_with(c, xb, RecipNewton{});
+            } else {
+                ca = cost_eval(c, xa);
+                cb = cost_eval(c, xb);
+            }
             if (state == ST_FIN) {
                 ot[oi] = topt;
                 oc[oi] = ca;
                 my = -1;
             } else {
-                const float dc = (ca - cb) / h; // dcost (2pBVP.cu:109-115)
+                // dcost = (ca - cb) / h (2pBVP.cu:109-115) enters only
+                // through its sign, which is that of dc, and through
+                // |dcost| > 0.0001, which is |dc| >= kDcostBig
+                const float dc = ca - cb;
                 if (state == ST_TMAX) {
                     if (dc < 0.0f) {
                         state = ST_FIN;
@@ -232,7 +244,7 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
                         ++halvings;
                     } else {
                         state = ST_BIS;
-                        if (!((double)fabsf(dcval) > 0.0001 && itr < 50))
+                        if (!(fabsf(dcval) >= kDcostBig && itr < 50))
                             state = ST_FIN; // topt stays 0, as the reference returns it
                     }
                 } else {
@@ -243,7 +255,7 @@ __device__ __forceinline__ void pairs_persistent(const Src &src, int64_t n_pairs
                         tu = topt;
                     else
                         tl = topt;
-                    if (!((double)fabsf(dcval) > 0.0001 && itr < 50))
+                    if (!(fabsf(dcval) >= kDcostBig && itr < 50))
                         state = ST_FIN;
                 }
             }

@@ -34,8 +34,8 @@ struct CostTerms {
     double s7;                                     // 12a0^2+12a2^2+12a1^2+12b0^2+12b2^2-24a1b1+12b1^2
     float f8, f9, f10, f11, f12, f13, f14, f15;    // float products multiplied by tau
     double d16, d17, d18, d20, d22, d24;           // 4*pow(v,2), times tau^2
-    float f19, f21, f23;                           // 4*v0*v1 (float), times tau^2
-    float f25, f26;                                // 2*g*vz (float), times tau^3
+    double f19, f21, f23;                          // 4*v0*v1 (a float, held widened), times tau^2
+    double f25, f26;                               // 2*g*vz (a float, held widened), times tau^3
     float f29a, f29b, f29c, f30a, f30b, f30c;      // 12*p0*(-2*p1 + (v0+v1)*tau)
 };
 
@@ -63,14 +63,14 @@ __host__ __device__ __forceinline__ void cost_terms(CostTerms &c, const float *a
     c.d16 = 4.0 * sq(a[4]);
     c.d17 = 4.0 * sq(a[3]);
     c.d18 = 4.0 * sq(a[5]);
-    c.f19 = 4.0f * a[4] * b[4];
+    c.f19 = (double)(4.0f * a[4] * b[4]);
     c.d20 = 4.0 * sq(b[4]);
-    c.f21 = 4.0f * a[3] * b[3];
+    c.f21 = (double)(4.0f * a[3] * b[3]);
     c.d22 = 4.0 * sq(b[3]);
-    c.f23 = 4.0f * a[5] * b[5];
+    c.f23 = (double)(4.0f * a[5] * b[5]);
     c.d24 = 4.0 * sq(b[5]);
-    c.f25 = 2.0f * g * a[5];
-    c.f26 = 2.0f * g * b[5];
+    c.f25 = (double)(2.0f * g * a[5]);
+    c.f26 = (double)(2.0f * g * b[5]);
     c.f29a = 12.0f * a[0];
     c.f29b = -2.0f * b[0];
     c.f29c = a[3] + b[3];
@@ -79,7 +79,32 @@ __host__ __device__ __forceinline__ void cost_terms(CostTerms &c, const float *a
     c.f30c = a[5] + b[5];
 }
 
-__host__ __device__ __forceinline__ float cost_eval(const CostTerms &c, float tau)
+// 1/x by IEEE division (any x)
+struct RecipDiv {
+    __host__ __device__ double operator()(double x) const { return 1.0 / x; }
+};
+
+// 1/x for 2^-100 <= |x| <= 2^400, bit-identical to RecipDiv there.  The
+// compiler's FP64 division n/d is: d' and n' from v_div_scale, r = rcp(d'),
+// two Newton steps r1 = fma(r, fma(-d', r, 1), r) and r2 = fma(r1,
+// fma(-d', r1, 1), r1), q = n' r2, then v_div_fmas(fma(-d', q, n'), r2, q)
+// and v_div_fixup.  For n = 1 and such d the scale steps return their
+// operands unchanged (they act only for subnormal or near-overflow operands
+// and quotients, or exponent gaps of 768 and more), v_div_fmas is a plain
+// fma and v_div_fixup changes only zeros, infinities and NaNs.  What is left
+// is the sequence below, with q = r2 exactly.
+struct RecipNewton {
+    __device__ double operator()(double x) const
+    {
+        const double r0 = __builtin_amdgcn_rcp(x);
+        const double r1 = fma(r0, fma(-x, r0, 1.0), r0);
+        const double r2 = fma(r1, fma(-x, r1, 1.0), r1);
+        return fma(fma(-x, r2, 1.0), r2, r2);
+    }
+};
+
+template <class Recip>
+__host__ __device__ __forceinline__ float cost_eval_with(const CostTerms &c, float tau, Recip recip)
 {
     const double g2 = (double)kGravity * (double)kGravity;
     const double t = (double)tau;
@@ -98,20 +123,35 @@ __host__ __device__ __forceinline__ float cost_eval(const CostTerms &c, float ta
     s = s + c.d16 * t2;
     s = s + c.d17 * t2;
     s = s + c.d18 * t2;
-    s = s + (double)c.f19 * t2;
+    s = s + c.f19 * t2;
     s = s + c.d20 * t2;
-    s = s + (double)c.f21 * t2;
+    s = s + c.f21 * t2;
     s = s + c.d22 * t2;
-    s = s + (double)c.f23 * t2;
+    s = s + c.f23 * t2;
     s = s + c.d24 * t2;
-    s = s + (double)c.f25 * t3;
-    s = s - (double)c.f26 * t3;
+    s = s + c.f25 * t3;
+    s = s - c.f26 * t3;
     s = s + t4;
     s = s + g2 * t4;
     s = s + (double)(c.f29a * (c.f29b + c.f29c * tau));
     s = s + (double)(c.f30a * (c.f30b + c.f30c * tau));
-    return (float)((1.0 / t3) * s);
+    return (float)(recip(t3) * s);
 }
+
+__host__ __device__ __forceinline__ float cost_eval(const CostTerms &c, float tau)
+{
+    return cost_eval_with(c, tau, RecipDiv{});
+}
+
+// |RN(d / 1e-6f)| > 0.0001 exactly when |d| >= kDcostBig, for every float d
+// (the quotient is monotone in d; the one transition over all finite
+// positive floats lies between 0x1.b7cdfcp-34 and 0x1.b7cdfep-34).  NaN
+// fails both tests.  It lets the bisection test dcost's magnitude without
+// forming the quotient: toptBisection's loop condition (2pBVP.cu:85) and
+// the signs it branches on (2pBVP.cu:75,79,88) depend on dcost only
+// through this test and the sign of d, which the division by a positive
+// dtau keeps.
+constexpr float kDcostBig = 0x1.b7cdfep-34f;
 
 // dcost, 2pBVP.cu:109-115: float central difference, dtau = 1e-6f
 __host__ __device__ __forceinline__ float dcost_eval(const CostTerms &c, float tau)

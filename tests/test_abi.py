@@ -334,6 +334,36 @@ def test_pair_row_division_by_magic_number():
         assert np.array_equal(q, n // np.uint32(d)), d
 
 
+def test_dcost_magnitude_threshold():
+    """The pair pass tests |dcost| > 0.0001 (dcost = RN((ca - cb) / 1e-6f),
+    compared as a double) as |ca - cb| >= kDcostBig without dividing.  The
+    float quotient is monotone in the numerator, so the two tests agree for
+    every float exactly when kDcostBig satisfies the test and its predecessor
+    does not; checked there, over a window of 2^20 floats each side, over
+    random floats of both signs, and at 0, infinities and NaN."""
+    text = open(os.path.join(ROOT, "pump_amd", "csrc", "pump_device.hpp")).read()
+    m = re.search(r"constexpr float kDcostBig = (0x[0-9a-fp.+-]+)f;", text)
+    assert m
+    big = np.float32(float.fromhex(m.group(1)))
+    h = np.float32(0.000001)
+
+    def ref(d):
+        with np.errstate(invalid="ignore", over="ignore"):
+            return np.abs((d / h).astype(np.float32)).astype(np.float64) > 0.0001
+
+    bits = big.view(np.uint32)
+    window = (np.arange(-(1 << 20), 1 << 20, dtype=np.int64) + int(bits)).astype(np.uint32)
+    rng = np.random.default_rng(11)
+    rand = rng.integers(0, 0x7F800000, 1 << 20, dtype=np.uint32)
+    for u in (window, rand):
+        d = u.view(np.float32)
+        for x in (d, -d):
+            assert np.array_equal(ref(x), np.abs(x) >= big)
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan], dtype=np.float32)
+    assert np.array_equal(ref(special), np.abs(special) >= big)
+    assert ref(np.array([big]))[0] and not ref(np.array([np.nextafter(big, np.float32(0))]))[0]
+
+
 def test_workspace_sizing_for_marked_pairs():
     """pump_propagate_workspace_pairs: no smaller than the default workspace,
     equal to it up to the default capacity (4 per path + 65536 pairs), and
