@@ -117,7 +117,7 @@ struct ManyPairSource {
             else
                 hi = mid;
         }
-        const PairJob J = jobs[lo];
+        const PairJob J = global_job(jobs[lo]);
         const int64_t idx = g - prefix[lo];
         int i, j;
         pair_of(idx, J.num, i, j);

@@ -725,7 +725,7 @@ __global__ __launch_bounds__(kTile) void k_edges_many(const EdgeJob *__restrict_
 {
     extern __shared__ __align__(16) char smem[];
     const int k = instance_of(prefix, n_inst, blockIdx.x);
-    const EdgeJob J = jobs[k];
+    const EdgeJob J = global_job(jobs[k]);
     edge_block<true, true, EPB>(J, ((int64_t)blockIdx.x - prefix[k]) * EPB, smem, obs_cap,
                                 edge_words(obs_cap));
 }

@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kHsBlock) void k_cache_hs_many(const HsJob *__restr
 {
     extern __shared__ __align__(16) char smem[];
     const int k = instance_of(prefix, n_inst, blockIdx.x);
-    const HsJob J = jobs[k];
+    const HsJob J = global_job(jobs[k]);
     hs_block<true>(J, ((int64_t)blockIdx.x - prefix[k]) * kEdgesPerBlock, smem, obs_cap,
                    stage_hs);
 }
@@ -1124,7 +1124,7 @@ __global__ __launch_bounds__(256) void k_zero_many(const McJob *__restrict__ job
                                                    const uint32_t *__restrict__ map)
 {
     VGrid G;
-    const McJob &J = mc_job(jobs, map, kMcZero, G);
+    const McJob J = global_job(mc_job(jobs, map, kMcZero, G));
     int *p = J.B.ctl;
     for (int64_t i = G.bid * blockDim.x + threadIdx.x; i < J.n_zero; i += G.nb * blockDim.x)
         p[i] = 0;
@@ -1134,7 +1134,7 @@ __global__ __launch_bounds__(kClassifyBlock) void k_classify_many(
     const McJob *__restrict__ jobs, const uint32_t *__restrict__ map)
 {
     VGrid G;
-    const McJob &J = mc_job(jobs, map, kMcClassify, G);
+    const McJob J = global_job(mc_job(jobs, map, kMcClassify, G));
     const BinArgs &a = J.B;
     classify_body(a.n_new, J.wp_offsets, 0, J.wf_edge, J.wf_prev, a.bs, J.edge_mask, a.max_hs,
                   J.times, J.topts, a.dt, a.tmax_steps, J.has, const_cast<HsPath *>(a.recs),
@@ -1146,7 +1146,7 @@ __global__ __launch_bounds__(256) void k_copy_listed_many(const McJob *__restric
                                                           const uint32_t *__restrict__ map)
 {
     VGrid G;
-    const McJob &J = mc_job(jobs, map, kMcCopy, G);
+    const McJob J = global_job(mc_job(jobs, map, kMcCopy, G));
     const BinArgs &a = J.B;
     copy_listed_body<16>(a.recs, a.ctl, (const uint8_t *)a.valid_src, (uint8_t *)a.valid_dst,
                          a.dst_base, J.row_bytes, G);

@@ -720,7 +720,7 @@ __global__ __launch_bounds__(kScanBlock) void k_bin_scan_many(const McJob *__res
                                                               const uint32_t *__restrict__ map)
 {
     VGrid G;
-    const BinArgs &a = mc_job(jobs, map, kMcScan, G).B;
+    const BinArgs a = global_args(mc_job(jobs, map, kMcScan, G).B);
     bin_scan_body(a.tmax_steps, a.hist, a.tstart, a.chunks, a.ctl, a.entry_cap, a.chunk_cap);
 }
 
@@ -728,7 +728,7 @@ __global__ __launch_bounds__(256) void k_bin_emit_many(const McJob *__restrict__
                                                        const uint32_t *__restrict__ map)
 {
     VGrid G;
-    const BinArgs &a = mc_job(jobs, map, kMcEmit, G).B;
+    const BinArgs a = global_args(mc_job(jobs, map, kMcEmit, G).B);
     bin_emit_body(a.recs, a.ctl, a.known, a.bs, a.max_hs, a.dt, a.tmax_steps, a.hist, a.tstart,
                   a.cursor, a.entries, G);
 }
@@ -738,7 +738,7 @@ __global__ __launch_bounds__(256) void k_hs_pass_many(const McJob *__restrict__ 
                                                       const uint32_t *__restrict__ map, int stage)
 {
     VGrid G;
-    const BinArgs &a = mc_job(jobs, map, stage, G).B;
+    const BinArgs a = global_args(mc_job(jobs, map, stage, G).B);
     if (a.ctl[kCtlOverflow] != 0)
         return;
     bin_test<true, NQ>(a, a.ctl[kCtlChunks], G);
@@ -748,7 +748,7 @@ __global__ __launch_bounds__(256) void k_hs_groups_many(const McJob *__restrict_
                                                         const uint32_t *__restrict__ map)
 {
     VGrid G;
-    const BinArgs &a = mc_job(jobs, map, kMcGroups, G).B;
+    const BinArgs a = global_args(mc_job(jobs, map, kMcGroups, G).B);
     if (a.ctl[kCtlOverflow] == 0)
         return;
     group_pass<true, kHsGroups>(a, a.ctl[kCtlListed], G);
@@ -758,7 +758,7 @@ __global__ __launch_bounds__(256) void k_cp_listed_many(const McJob *__restrict_
                                                         const uint32_t *__restrict__ map)
 {
     VGrid G;
-    const BinArgs &a = mc_job(jobs, map, kMcCp, G).B;
+    const BinArgs a = global_args(mc_job(jobs, map, kMcCp, G).B);
     cp_listed_body<true>(a.recs, a.ctl, a.valid_dst, a.dst_base, a.n_p, a.path_cp, a.cp_base, G);
 }
 

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pump_internal.hpp"
+
 #pragma clang fp contract(off)
 
 namespace pump {
@@ -371,6 +373,45 @@ struct McJob {
     int64_t row_bytes, n_zero;
     int nb[kMcStages];
 };
+// copies of a job whose pointers are read as global ones (gptr)
+__device__ __forceinline__ BinArgs global_args(const BinArgs &a)
+{
+    BinArgs g = a;
+    g.as = gptr(a.as);
+    g.bs = gptr(a.bs);
+    g.offsets_time = gptr(a.offsets_time);
+    g.recs = gptr(a.recs);
+    g.acc = gptr(a.acc);
+    g.ctl = gptr(a.ctl);
+    g.hist = gptr(a.hist);
+    g.cursor = gptr(a.cursor);
+    g.tstart = gptr(a.tstart);
+    g.chunks = gptr(a.chunks);
+    g.entries = gptr(a.entries);
+    g.valid_src = gptr(a.valid_src);
+    g.valid_dst = gptr(a.valid_dst);
+    g.path_cp = gptr(a.path_cp);
+    g.split = gptr(a.split);
+    g.band = gptr(a.band);
+    return g;
+}
+__device__ __forceinline__ McJob global_job(const McJob &j)
+{
+    McJob g = j;
+    g.B = global_args(j.B);
+    g.wp_offsets = gptr(j.wp_offsets);
+    g.wf_edge = gptr(j.wf_edge);
+    g.wf_prev = gptr(j.wf_prev);
+    g.edge_mask = gptr(j.edge_mask);
+    g.times = gptr(j.times);
+    g.topts = gptr(j.topts);
+    g.has = gptr(j.has);
+    g.clamped = gptr(j.clamped);
+    g.row_src = gptr(j.row_src);
+    g.parent_cp = gptr(j.parent_cp);
+    return g;
+}
+
 struct McGrids {
     const uint32_t *block_map[kMcStages]; // device arrays
     int64_t blocks[kMcStages];
