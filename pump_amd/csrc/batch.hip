@@ -27,6 +27,7 @@
 
 #include "../../include/pump_host.h"
 #include "hsmc_wave.hpp"
+#include "pump_device.hpp"
 #include "pump_internal.hpp"
 #include "roadmap_internal.hpp"
 
@@ -170,7 +171,10 @@ int pair_pass(pump_batch *b, int autotune)
     std::vector<int64_t> prefix(n + 1, 0);
     for (int k = 0; k < n; ++k) {
         const pump_roadmap *rm = b->rms[k];
-        jobs[k] = PairJob{rm->samples, rm->copts, rm->topts, rm->num, rm->prm.tmax};
+        const DivMagic dm = rm->n_pairs < (1ll << 32) ? div_magic((uint32_t)(rm->num - 1))
+                                                       : DivMagic{0, 0, 0};
+        jobs[k] = PairJob{rm->samples, rm->copts, rm->topts, rm->num,  rm->prm.tmax,
+                          dm.m,        dm.s1,     dm.s2};
         prefix[k + 1] = prefix[k] + rm->n_pairs;
     }
     const int64_t total = prefix[n];

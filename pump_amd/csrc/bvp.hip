@@ -102,25 +102,43 @@ struct OnePairSource {
 };
 
 // ... or many: instance k owns global indices [prefix[k], prefix[k+1]).
+// A lane keeps the range of the instance its last pair came from, so the
+// search over the prefix table runs only when a lane crosses into another
+// instance (a wave's 2048-pair chunks mostly lie inside one).
 struct ManyPairSource {
     const PairJob *jobs;
     const int64_t *prefix;
     int n_inst;
+    mutable int cur = 0;
+    mutable int64_t cur_lo = 0, cur_hi = 0;
     __device__ void take(int64_t g, CostTerms &c, float &tm, float *&oc, float *&ot,
                          int64_t &oi) const
     {
-        int lo = 0, hi = n_inst;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (prefix[mid] <= g)
-                lo = mid;
-            else
-                hi = mid;
+        if (!(g >= cur_lo && g < cur_hi)) {
+            int lo = 0, hi = n_inst;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (prefix[mid] <= g)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            cur = lo;
+            cur_lo = prefix[lo];
+            cur_hi = prefix[lo + 1];
         }
-        const PairJob J = global_job(jobs[lo]);
-        const int64_t idx = g - prefix[lo];
+        const PairJob J = global_job(jobs[cur]);
+        const int64_t idx = g - cur_lo;
         int i, j;
-        pair_of(idx, J.num, i, j);
+        if (J.dm_m) {
+            const uint32_t n = (uint32_t)idx;
+            const uint32_t q = div_magic_apply(n, DivMagic{J.dm_m, J.dm_s1, J.dm_s2});
+            const int k = (int)(n - q * (uint32_t)(J.num - 1));
+            i = (int)q;
+            j = k + (k >= i ? 1 : 0);
+        } else {
+            pair_of(idx, J.num, i, j);
+        }
         float a[kDim], b[kDim];
         load_state(J.samples, i, a);
         load_state(J.samples, j, b);

@@ -65,6 +65,9 @@ struct PairJob {
     float *copts, *topts;
     int num;
     float tmax;
+    // magic number of num - 1 for the pair's row (DivMagic's fields; m = 0:
+    // 64-bit division, for 2^32 pairs or more)
+    uint32_t dm_m, dm_s1, dm_s2;
 };
 
 struct EdgeJob {
