@@ -128,16 +128,21 @@ __device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0, char *smem,
     int64_t *off = reinterpret_cast<int64_t *>(smem + L.off);
     unsigned long long *em = reinterpret_cast<unsigned long long *>(smem + L.em);
     uint8_t *fr = reinterpret_cast<uint8_t *>(smem + L.fr);
-    float *olo = reinterpret_cast<float *>(smem + L.obs); // lo[d][b], then hi[d][b]
-    float *ohi = olo + 3 * obs_cap;
+    // The staged boxes and slots are typed as LDS pointers: where a branch
+    // reads or writes either them or global memory, the compiler would
+    // otherwise merge the two accesses into one flat access through a
+    // selected pointer.
+    typedef __attribute__((address_space(3))) float LdsFloat;
+    LdsFloat *olo = (LdsFloat *)(smem + L.obs); // lo[d][b], then hi[d][b]
+    LdsFloat *ohi = olo + 3 * obs_cap;
     uint8_t *wmap = reinterpret_cast<uint8_t *>(smem + L.wmap); // edge of each local waypoint
     const int ne = (int)((n_edges - e0) < kEdgesPerBlock ? (n_edges - e0) : kEdgesPerBlock);
     const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
     const int n_staged = n_obs < obs_cap ? n_obs : obs_cap;
     // slots staged in LDS and written out per wave (sentinel pass only)
     const bool stage = kSentinels && max_hs > 0 && max_hs <= stage_hs;
-    float *sa = reinterpret_cast<float *>(smem + L.stage) + wid * 64 * stage_hs * 4;
-    float *sb = sa + 64 * max_hs * 3;
+    LdsFloat *sa = (LdsFloat *)(smem + L.stage) + wid * 64 * stage_hs * 4;
+    LdsFloat *sb = sa + 64 * max_hs * 3;
     for (int q = t; q < n_staged * 3; q += kHsBlock) {
         const int b = q / 3, d = q - 3 * (q / 3);
         olo[d * obs_cap + b] = obstacles[b * 12 + d];
@@ -190,10 +195,23 @@ __device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0, char *smem,
                 pos[d] = disc[g * kDim + d];
                 vel[d] = disc[g * kDim + 3 + d];
             }
-        // slot h of this waypoint: as/bs[slot0 + h], in LDS when staged
+        // slot h of this waypoint: as/bs[slot0 + h], in LDS when staged.
+        // stage is uniform, and the two branches keep each store's address
+        // space known (LDS or global stores, not flat ones).
         const int64_t slot0 = stage ? (int64_t)lane * max_hs : g * max_hs;
-        float *const wa = stage ? sa : as;
-        float *const wb = stage ? sb : bs;
+        auto put = [&](int64_t hs, float a0, float a1, float a2, float b) {
+            if (stage) {
+                sa[hs * 3 + 0] = a0;
+                sa[hs * 3 + 1] = a1;
+                sa[hs * 3 + 2] = a2;
+                sb[hs] = b;
+            } else {
+                as[hs * 3 + 0] = a0;
+                as[hs * 3 + 1] = a1;
+                as[hs * 3 + 2] = a2;
+                bs[hs] = b;
+            }
+        };
         int stored = 0;
         bool live = false; // some stored b passes the sentinel test of PUMP.cu:642
         if (__ballot(work) != 0ull) {
@@ -237,12 +255,9 @@ __device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0, char *smem,
                     for (int d = 0; d < 3; ++d)
                         x[d] = x[d] - ratio * vel[d];
                     float xx = 0.0f;
-                    const int64_t hs = slot0 + stored;
-                    for (int d = 0; d < 3; ++d) {
-                        wa[hs * 3 + d] = x[d];
+                    for (int d = 0; d < 3; ++d)
                         xx = xx + x[d] * x[d];
-                    }
-                    wb[hs] = xx;
+                    put(slot0 + stored, x[0], x[1], x[2], xx);
                     live = live || !(xx >= 1000.0f);
                     ++stored;
                 }
@@ -255,13 +270,8 @@ __device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0, char *smem,
                 atomicOr(&em[le], 1ull << (k < 63 ? k : 63));
             }
             if (kSentinels)
-                for (int h = stored; h < max_hs; ++h) {
-                    const int64_t hs = slot0 + h;
-                    wa[hs * 3 + 0] = -1.0f;
-                    wa[hs * 3 + 1] = -1.0f;
-                    wa[hs * 3 + 2] = -1.0f;
-                    wb[hs] = 1111.0f;
-                }
+                for (int h = stored; h < max_hs; ++h)
+                    put(slot0 + h, -1.0f, -1.0f, -1.0f, 1111.0f);
         }
         if (stage) {
             // the wave's waypoints are the prefix of its lanes with have set;
@@ -272,8 +282,8 @@ __device__ __forceinline__ void hs_block(const HsJob &J, int64_t e0, char *smem,
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            copy_run(as + w0 * max_hs * 3, sa, nw * max_hs * 3, lane);
-            copy_run(bs + w0 * max_hs, sb, nw * max_hs, lane);
+            copy_run(as + w0 * max_hs * 3, (const float *)sa, nw * max_hs * 3, lane);
+            copy_run(bs + w0 * max_hs, (const float *)sb, nw * max_hs, lane);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
