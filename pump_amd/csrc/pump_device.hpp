@@ -482,14 +482,18 @@ __device__ __forceinline__ void stage_obstacles(ObsLdsN<N> &o, const float *__re
     }
 }
 
+// The staged rows are read through LDS-typed pointers: with two generic
+// pointers the compiler merges the branches into one flat load through a
+// selected address.
 template <int N>
 __device__ __forceinline__ void box_bounds(const ObsLdsN<N> &o, const float *__restrict__ obs, int b,
                                            float *lo, float *hi)
 {
+    typedef __attribute__((address_space(3))) const float LdsFloat;
     if (b < N) {
         for (int d = 0; d < 3; ++d) {
-            lo[d] = o.lo[d][b];
-            hi[d] = o.hi[d][b];
+            lo[d] = ((LdsFloat *)o.lo[d])[b];
+            hi[d] = ((LdsFloat *)o.hi[d])[b];
         }
     } else {
         for (int d = 0; d < 3; ++d) {
